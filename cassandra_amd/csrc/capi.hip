@@ -1,0 +1,628 @@
+// extern "C" boundary of libcassandra_amd.so (declared in include/cassandra_amd.h).
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "solver.hpp"
+
+namespace cad {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
+                        const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
+                        const Context& ctx);
+
+namespace {
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return CAD_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return CAD_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return CAD_ERR_ARGUMENT;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return CAD_ERR_ARGUMENT;
+    }
+}
+
+void need(bool c, const char* msg, int code = CAD_ERR_ARGUMENT)
+{
+    if (!c) throw Error(code, msg);
+}
+
+int grid1(int64_t n, int bs) { return int(std::max<int64_t>(1, (n + bs - 1) / bs)); }
+
+// Upload the index programs of a pattern group.
+void upload_programs(cad_patterns* p)
+{
+    const PatternGroup& g = *p->group;
+    std::vector<DevSlot> slots;
+    std::vector<uint8_t> sq;
+    for (const SlotProgram& s : g.slots) {
+        DevSlot d;
+        std::memset(&d, 0, sizeof(d));
+        d.kind = uint8_t(s.kind);
+        for (int l = 0; l < 3; l++) {
+            d.off[l] = uint16_t(sq.size());
+            d.len[l] = uint8_t(s.list[l].src.size());
+            sq.insert(sq.end(), s.list[l].src.begin(), s.list[l].src.end());
+        }
+        d.hs = s.hs;
+        d.ds = s.ds;
+        d.col_offset = s.col_offset;
+        d.dense_offset = s.dense_offset;
+        slots.push_back(d);
+    }
+    hipStream_t st = p->ctx->stream;
+    p->d_slots = to_device(slots, st);
+    p->d_sq = to_device(sq, st);
+    p->d_dense = to_device(g.dense, st);
+    p->slot_lo.clear();
+    p->slot_hi.clear();
+    for (const SlotProgram& s : g.slots) {
+        p->slot_lo.push_back(s.col_offset);
+        p->slot_hi.push_back(s.col_offset + s.block_size);
+    }
+    std::vector<uint32_t> zeros(std::max<size_t>(1, g.slots.size()), 0u);
+    p->d_base0 = to_device(zeros, st);
+    p->d_syscol0 = to_device(std::vector<uint32_t>{0u}, st);
+}
+
+void extract(cad_patterns* p, const uint64_t* pos, int64_t n, void* out, int bytes, int64_t slot_stride,
+             int64_t row_stride, int local)
+{
+    if (n <= 0) return;
+    const int k = p->group->k();
+    hipStream_t s = p->ctx->stream;
+    if (bytes == 2)
+        hipLaunchKernelGGL(k_index_extract<uint16_t>, dim3(grid1(n, kRowBlock)), dim3(kRowBlock), 0, s, pos, n,
+                           p->d_slots.as<DevSlot>(), k, p->d_sq.as<uint8_t>(), p->d_dense.as<uint32_t>(),
+                           static_cast<uint16_t*>(out), slot_stride, row_stride, local);
+    else
+        hipLaunchKernelGGL(k_index_extract<uint32_t>, dim3(grid1(n, kRowBlock)), dim3(kRowBlock), 0, s, pos, n,
+                           p->d_slots.as<DevSlot>(), k, p->d_sq.as<uint8_t>(), p->d_dense.as<uint32_t>(),
+                           static_cast<uint32_t*>(out), slot_stride, row_stride, local);
+    CAD_HIP(hipGetLastError());
+}
+
+cad_matrix* new_matrix(cad_context* ctx, int64_t rows, int k, int64_t cols)
+{
+    need(rows >= 0 && k > 0 && k <= 1024 && cols > 0, "invalid matrix shape");
+    need(cols <= int64_t(0xFFFFFFFFu), "more than 2^32 columns");
+    auto* m = new cad_matrix;
+    m->ctx = ctx;
+    m->rows = rows;
+    m->k = k;
+    m->cols = cols;
+    return m;
+}
+
+// The scatter plan of the evaluator backward for B rows of a pattern group.
+void eval_plan(cad_patterns* p, int64_t B)
+{
+    if (p->eval_plan_rows == B) return;
+    build_scatter_plan(p->eval_plan, p->slot_lo, p->slot_hi, std::vector<int64_t>{0, B}, std::vector<uint32_t>{0u},
+                       *p->ctx);
+    p->eval_plan_rows = B;
+}
+
+template <class VT>
+void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, VT* grad)
+{
+    hipStream_t s = p->ctx->stream;
+    const int64_t C = int64_t(p->group->cols());
+    hipLaunchKernelGGL(k_fill<VT>, dim3(grid1(C, 256)), dim3(256), 0, s, grad, C, VT(0));
+    if (B <= 0) return;
+    eval_plan(p, B);
+    const ScatterPlan& sp = p->eval_plan;
+    const bool f = sizeof(VT) == 4;
+    const size_t n = f ? sp.work4.size() : sp.work8.size();
+    const size_t lds = size_t(f ? sp.win4 : sp.win8) * sizeof(VT);
+    auto kern = k_scatter<VT, VT, RowMajor32>;
+    if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(lds));
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kScatterBlock), lds, s, RowMajor32{idx, p->group->k()},
+                       (f ? sp.d_work4 : sp.d_work8).as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
+                       sp.d_slot_list.as<uint16_t>(), p->d_syscol0.as<uint32_t>(), g, (const LsqrScalars*)nullptr,
+                       grad);
+    CAD_HIP(hipGetLastError());
+}
+
+template <class VT>
+void eval_forward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* w, VT* out)
+{
+    if (B <= 0) return;
+    hipLaunchKernelGGL(k_eval_forward<VT>, dim3(grid1(B, kRowBlock)), dim3(kRowBlock), 0, p->ctx->stream, idx, B,
+                       p->group->k(), w, uint32_t(p->group->cols()), out);
+    CAD_HIP(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace cad
+
+using namespace cad;
+
+extern "C" {
+
+const char* cad_last_error(void) { return g_last_error.c_str(); }
+const char* cad_version(void) { return "cassandra_amd 0.1 (gfx950)"; }
+
+int cad_context_create(int device, cad_context** out)
+{
+    return guarded([&] {
+        need(out != nullptr, "null output");
+        int n = 0;
+        CAD_HIP(hipGetDeviceCount(&n));
+        need(device >= 0 && device < n, "no such device", CAD_ERR_DEVICE);
+        CAD_HIP(hipSetDevice(device));
+        auto* c = new cad_context;
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            CAD_HIP(e);
+        }
+        c->stream = c->own;
+        hipDeviceProp_t prop;
+        CAD_HIP(hipGetDeviceProperties(&prop, device));
+        c->cus = prop.multiProcessorCount;
+        c->max_lds = prop.sharedMemPerBlock;
+        *out = c;
+    });
+}
+
+int cad_context_destroy(cad_context* ctx)
+{
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    });
+}
+
+int cad_context_set_stream(cad_context* ctx, void* stream)
+{
+    return guarded([&] {
+        need(ctx != nullptr, "null context");
+        ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own;
+    });
+}
+
+int cad_context_get_stream(cad_context* ctx, void** out)
+{
+    return guarded([&] {
+        need(ctx && out, "null argument");
+        *out = static_cast<void*>(ctx->stream);
+    });
+}
+
+int cad_context_synchronize(cad_context* ctx)
+{
+    return guarded([&] { CAD_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int cad_malloc(cad_context* ctx, size_t bytes, void** out)
+{
+    return guarded([&] {
+        need(ctx && out, "null argument");
+        CAD_HIP(hipSetDevice(ctx->device));
+        *out = nullptr;
+        if (bytes) CAD_HIP(hipMalloc(out, bytes));
+    });
+}
+
+int cad_free(cad_context* ctx, void* p)
+{
+    return guarded([&] {
+        (void)ctx;
+        if (p) CAD_HIP(hipFree(p));
+    });
+}
+
+int cad_memcpy_h2d(cad_context* ctx, void* d, const void* h, size_t bytes)
+{
+    return guarded([&] {
+        if (!bytes) return;
+        CAD_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+        CAD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cad_memcpy_d2h(cad_context* ctx, void* h, const void* d, size_t bytes)
+{
+    return guarded([&] {
+        if (!bytes) return;
+        CAD_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        CAD_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int cad_memcpy_d2d(cad_context* ctx, void* d, const void* s, size_t bytes)
+{
+    return guarded([&] {
+        if (!bytes) return;
+        CAD_HIP(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+// ------------------------------------------------------------ patterns ----
+int cad_patterns_create(cad_context* ctx, const uint64_t* masks, int n, cad_patterns** out)
+{
+    return guarded([&] {
+        need(out && (masks || n == 0) && n >= 0, "invalid argument");
+        std::vector<uint64_t> v(masks, masks + n);
+        std::unique_ptr<PatternGroup> g;
+        try {
+            g.reset(new PatternGroup(v));
+        } catch (const std::runtime_error& e) {
+            throw Error(CAD_ERR_PATTERN, e.what());
+        }
+        need(g->k() > 0, "empty pattern group", CAD_ERR_PATTERN);
+        auto* p = new cad_patterns;
+        p->ctx = ctx;
+        p->group = std::move(g);
+        try {
+            if (ctx) upload_programs(p);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int cad_patterns_destroy(cad_patterns* p)
+{
+    return guarded([&] { delete p; });
+}
+
+int cad_patterns_info(const cad_patterns* p, int* k, int64_t* cols, int* npat)
+{
+    return guarded([&] {
+        need(p != nullptr, "null patterns");
+        if (k) *k = p->group->k();
+        if (cols) *cols = int64_t(p->group->cols());
+        if (npat) *npat = int(p->group->patterns.size());
+    });
+}
+
+int cad_patterns_pattern_info(const cad_patterns* p, int i, int* cls, uint64_t* mask, int64_t* size, int* order,
+                              int64_t* offset)
+{
+    return guarded([&] {
+        need(p && i >= 0 && i < int(p->group->patterns.size()), "pattern index out of range");
+        const PatternInfo& pi = p->group->patterns[i];
+        if (cls) *cls = pi.cls;
+        if (mask) *mask = pi.mask;
+        if (size) *size = pi.size;
+        if (order) *order = pi.order;
+        if (offset) *offset = pi.offset;
+    });
+}
+
+int cad_patterns_variations(const cad_patterns* p, uint64_t* out)
+{
+    return guarded([&] {
+        need(p && out, "null argument");
+        int j = 0;
+        for (auto& pi : p->group->patterns)
+            for (uint64_t v : pi.variations) out[j++] = v;
+    });
+}
+
+int cad_patterns_indices_host(const cad_patterns* p, const uint64_t* pos, int64_t n, uint32_t* out)
+{
+    return guarded([&] {
+        need(p && (pos || n == 0) && (out || n == 0) && n >= 0, "invalid argument");
+        const int k = p->group->k();
+        for (int64_t r = 0; r < n; r++) p->group->row(pos[2 * r], pos[2 * r + 1], out + r * k);
+    });
+}
+
+int cad_index_extract(cad_patterns* p, const uint64_t* d_pos, int64_t n, uint32_t* d_out, int layout)
+{
+    return guarded([&] {
+        need(p && n >= 0 && (n == 0 || (d_pos && d_out)), "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        need(layout == CAD_LAYOUT_ROW_MAJOR || layout == CAD_LAYOUT_SLOT_MAJOR, "unknown layout");
+        const int k = p->group->k();
+        if (layout == CAD_LAYOUT_ROW_MAJOR) extract(p, d_pos, n, d_out, 4, 1, k, 0);
+        else extract(p, d_pos, n, d_out, 4, n, 1, 0);
+    });
+}
+
+// ------------------------------------------------------------ matrices ----
+int cad_matrix_from_positions(cad_patterns* p, const uint64_t* d_pos, int64_t n, cad_matrix** out)
+{
+    return guarded([&] {
+        need(p && out && n > 0 && d_pos, "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        const PatternGroup& g = *p->group;
+        std::unique_ptr<cad_matrix> m(new_matrix(p->ctx, n, g.k(), int64_t(g.cols())));
+        m->slot_lo = p->slot_lo;
+        m->slot_hi = p->slot_hi;
+        uint32_t widest = 0;
+        for (size_t s = 0; s < m->slot_lo.size(); s++) widest = std::max(widest, m->slot_hi[s] - m->slot_lo[s]);
+        m->ibytes = widest <= 65536 ? 2 : 4;
+        m->d_idx = DBuf(size_t(m->ibytes) * size_t(n) * size_t(g.k()));
+        std::vector<uint32_t> base(g.k());
+        for (int s = 0; s < g.k(); s++) base[s] = m->ibytes == 2 ? m->slot_lo[s] : 0u;
+        m->d_base = to_device(base, p->ctx->stream);
+        extract(p, d_pos, n, m->d_idx.p, m->ibytes, n, 1, m->ibytes == 2 ? 1 : 0);
+        build_plans(m.get());
+        CAD_HIP(hipStreamSynchronize(p->ctx->stream));
+        *out = m.release();
+    });
+}
+
+int cad_matrix_create(cad_context* ctx, const uint32_t* h_idx, int64_t rows, int k, int64_t cols, cad_matrix** out)
+{
+    return guarded([&] {
+        need(ctx && out && (h_idx || rows == 0), "invalid argument");
+        std::unique_ptr<cad_matrix> m(new_matrix(ctx, rows, k, cols));
+        std::vector<uint32_t> lo(k, 0xFFFFFFFFu), hi(k, 0u);
+        for (int64_t r = 0; r < rows; r++)
+            for (int s = 0; s < k; s++) {
+                const uint32_t c = h_idx[r * k + s];
+                need(int64_t(c) < cols, "column index out of range", CAD_ERR_SIZE_MISMATCH);
+                lo[s] = std::min(lo[s], c);
+                hi[s] = std::max(hi[s], c + 1);
+            }
+        uint32_t widest = 0;
+        for (int s = 0; s < k; s++) {
+            if (lo[s] > hi[s]) lo[s] = hi[s] = 0;  // no rows
+            widest = std::max(widest, hi[s] - lo[s]);
+        }
+        m->slot_lo = lo;
+        m->slot_hi = hi;
+        m->ibytes = widest <= 65536 ? 2 : 4;
+        std::vector<uint32_t> base(k);
+        for (int s = 0; s < k; s++) base[s] = m->ibytes == 2 ? lo[s] : 0u;
+        m->d_base = to_device(base, ctx->stream);
+        // transpose to slot-major (and narrow) on the host
+        if (m->ibytes == 2) {
+            std::vector<uint16_t> t(size_t(rows) * k);
+            for (int64_t r = 0; r < rows; r++)
+                for (int s = 0; s < k; s++) t[size_t(s) * rows + r] = uint16_t(h_idx[r * k + s] - lo[s]);
+            m->d_idx = to_device(t, ctx->stream);
+        } else {
+            std::vector<uint32_t> t(size_t(rows) * k);
+            for (int64_t r = 0; r < rows; r++)
+                for (int s = 0; s < k; s++) t[size_t(s) * rows + r] = h_idx[r * k + s];
+            m->d_idx = to_device(t, ctx->stream);
+        }
+        build_plans(m.get());
+        *out = m.release();
+    });
+}
+
+int cad_matrix_destroy(cad_matrix* m)
+{
+    return guarded([&] {
+        if (!m) return;
+        (void)hipStreamSynchronize(m->ctx->stream);
+        delete m;
+    });
+}
+
+int cad_matrix_info(const cad_matrix* m, int64_t* rows, int* k, int64_t* cols, int* ib)
+{
+    return guarded([&] {
+        need(m != nullptr, "null matrix");
+        if (rows) *rows = m->rows;
+        if (k) *k = m->k;
+        if (cols) *cols = m->cols;
+        if (ib) *ib = m->ibytes;
+    });
+}
+
+int cad_matrix_download(const cad_matrix* m, uint32_t* out)
+{
+    return guarded([&] {
+        need(m && (out || m->rows == 0), "null argument");
+        const int64_t R = m->rows;
+        const int k = m->k;
+        std::vector<uint32_t> base(k);
+        hipStream_t s = m->ctx->stream;
+        CAD_HIP(hipMemcpyAsync(base.data(), m->d_base.p, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, s));
+        if (m->ibytes == 2) {
+            std::vector<uint16_t> t(size_t(R) * k);
+            CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 2, hipMemcpyDeviceToHost, s));
+            CAD_HIP(hipStreamSynchronize(s));
+            for (int64_t r = 0; r < R; r++)
+                for (int j = 0; j < k; j++) out[r * k + j] = base[j] + t[size_t(j) * R + r];
+        } else {
+            std::vector<uint32_t> t(size_t(R) * k);
+            CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 4, hipMemcpyDeviceToHost, s));
+            CAD_HIP(hipStreamSynchronize(s));
+            for (int64_t r = 0; r < R; r++)
+                for (int j = 0; j < k; j++) out[r * k + j] = base[j] + t[size_t(j) * R + r];
+        }
+    });
+}
+
+int cad_spmv_f32(cad_matrix* m, const float* x, float* y)
+{
+    return guarded([&] {
+        need(m && x && (y || m->rows == 0), "null argument");
+        if (m->rows) matrix_spmv_f32(m, x, y);
+    });
+}
+
+int cad_spmv_f64(cad_matrix* m, const double* x, double* y)
+{
+    return guarded([&] {
+        need(m && x && (y || m->rows == 0), "null argument");
+        if (m->rows) matrix_spmv_f64(m, x, y);
+    });
+}
+
+int cad_spmtv_f32(cad_matrix* m, const float* y, float* z)
+{
+    return guarded([&] {
+        need(m && z && (y || m->rows == 0), "null argument");
+        matrix_spmtv_f32(m, y, z);
+    });
+}
+
+int cad_spmtv_f64(cad_matrix* m, const double* y, double* z)
+{
+    return guarded([&] {
+        need(m && z && (y || m->rows == 0), "null argument");
+        matrix_spmtv_f64(m, y, z);
+    });
+}
+
+int cad_jacobi(cad_matrix* m, float* d)
+{
+    return guarded([&] {
+        need(m && d, "null argument");
+        matrix_jacobi(m, d);
+    });
+}
+
+// ------------------------------------------------------------ solvers -----
+void cad_solve_options_default(cad_solve_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->solver = CAD_SOLVER_LSQR;
+    o->precision = CAD_PREC_MIXED;
+    o->precondition = 1;
+    o->max_iterations = 100000;
+    o->tol = 1e-10;
+    o->inner_tol = 1e-4;
+    o->max_refinements = 20;
+    o->check_every = 25;
+    o->use_graph = 1;
+    o->verbose = 0;
+}
+
+int cad_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options* opt, cad_solve_report* rep)
+{
+    return guarded([&] {
+        need(m && b && x, "null argument");
+        cad_solve_options o;
+        if (opt) o = *opt;
+        else cad_solve_options_default(&o);
+        need(o.precision >= 0 && o.precision <= 2, "unknown precision");
+        need(!(o.precision == CAD_PREC_MIXED && o.solver != CAD_SOLVER_LSQR), "mixed precision is LSQR only");
+        cad_solve_report r;
+        solve(m, b, x, o, r);
+        if (rep) *rep = r;
+    });
+}
+
+int cad_rel_normal_residual(cad_matrix* m, const double* b, const double* x, double* out, double* rn)
+{
+    return guarded([&] {
+        need(m && b && x && out, "null argument");
+        *out = rel_normal_residual(m, b, x, rn);
+    });
+}
+
+// ------------------------------------------------------------ evaluator ---
+int cad_eval_forward_f32(cad_patterns* p, const int32_t* idx, int64_t B, const float* w, float* out)
+{
+    return guarded([&] {
+        need(p && B >= 0 && (B == 0 || (idx && w && out)), "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        eval_forward<float>(p, idx, B, w, out);
+    });
+}
+
+int cad_eval_forward_f64(cad_patterns* p, const int32_t* idx, int64_t B, const double* w, double* out)
+{
+    return guarded([&] {
+        need(p && B >= 0 && (B == 0 || (idx && w && out)), "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        eval_forward<double>(p, idx, B, w, out);
+    });
+}
+
+int cad_eval_backward_f32(cad_patterns* p, const int32_t* idx, int64_t B, const float* g, float* grad)
+{
+    return guarded([&] {
+        need(p && grad && B >= 0 && (B == 0 || (idx && g)), "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        eval_backward<float>(p, idx, B, g, grad);
+    });
+}
+
+int cad_eval_backward_f64(cad_patterns* p, const int32_t* idx, int64_t B, const double* g, double* grad)
+{
+    return guarded([&] {
+        need(p && grad && B >= 0 && (B == 0 || (idx && g)), "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        eval_backward<double>(p, idx, B, g, grad);
+    });
+}
+
+// ------------------------------------------------------------ data --------
+int cad_random_play(cad_context* ctx, uint64_t seed, int64_t n, int e_lo, int e_hi, uint64_t* out)
+{
+    return guarded([&] {
+        need(ctx && n >= 0 && (n == 0 || out), "invalid argument");
+        need(e_lo >= 0 && e_hi <= 60 && e_lo <= e_hi, "empty count outside [0, 60]");
+        if (!n) return;
+        hipLaunchKernelGGL(k_random_play, dim3(grid1(n, kRowBlock)), dim3(kRowBlock), 0, ctx->stream, seed, n, e_lo,
+                           e_hi, out);
+        CAD_HIP(hipGetLastError());
+    });
+}
+
+uint64_t cad_flip(int sym, uint64_t b) { return apply_sym(sym, b); }
+uint64_t cad_possible_moves(uint64_t P, uint64_t O) { return possible_moves(P, O); }
+uint64_t cad_flips(uint64_t P, uint64_t O, int move) { return (move >= 0 && move < 64) ? flips(P, O, move) : 0; }
+
+int cad_random_play_host(uint64_t seed, int64_t n, int e_lo, int e_hi, uint64_t* out)
+{
+    return guarded([&] {
+        need(n >= 0 && (n == 0 || out), "invalid argument");
+        need(e_lo >= 0 && e_hi <= 60 && e_lo <= e_hi, "empty count outside [0, 60]");
+        for (int64_t i = 0; i < n; i++) {
+            const Pos p = random_play(seed, uint64_t(i), e_lo, e_hi);
+            out[2 * i] = p.P;
+            out[2 * i + 1] = p.O;
+        }
+    });
+}
+
+int cad_make_targets(cad_matrix* m, uint64_t seed_x, uint64_t seed_n, double sx, double sn, double* xt, double* b)
+{
+    return guarded([&] {
+        need(m && xt && (b || m->rows == 0), "null argument");
+        hipStream_t s = m->ctx->stream;
+        hipLaunchKernelGGL(k_targets_x, dim3(grid1(m->cols, 256)), dim3(256), 0, s, seed_x, m->cols, sx, xt);
+        CAD_HIP(hipGetLastError());
+        if (!m->rows) return;
+        const GridPlan& g = m->grid;
+        if (m->ibytes == 2)
+            hipLaunchKernelGGL((k_targets_b<SlotMajor<uint16_t>>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0,
+                               s, SlotMajor<uint16_t>{m->d_idx.as<uint16_t>(), m->rows, m->d_base.as<uint32_t>()},
+                               m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), xt, seed_n, sn, b);
+        else
+            hipLaunchKernelGGL((k_targets_b<SlotMajor<uint32_t>>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0,
+                               s, SlotMajor<uint32_t>{m->d_idx.as<uint32_t>(), m->rows, m->d_base.as<uint32_t>()},
+                               m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), xt, seed_n, sn, b);
+        CAD_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

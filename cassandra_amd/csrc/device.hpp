@@ -1,0 +1,84 @@
+// Plain-old-data structures shared by the host runtime and the HIP kernels.
+#pragma once
+#include <cstdint>
+
+namespace cad {
+
+// One slot of the index program (see patterns.hpp SlotProgram), packed for
+// uniform (scalar) loads on the device.
+struct DevSlot {
+    uint8_t kind;        // 0: a, 1: pair(a,b), 2: pair(a,b)*ds + d, 3: dense[a]
+    uint8_t len[3];      // square-list lengths
+    uint16_t off[3];     // offsets into the square array
+    uint16_t pad;
+    uint32_t hs, ds;
+    uint32_t col_offset;
+    uint32_t dense_offset;
+    uint32_t pad2;
+};
+static_assert(sizeof(DevSlot) == 32, "DevSlot layout");
+
+// A block of rows of one system, for the row-parallel kernels (SpMV and the
+// rows-length vector kernels).  Blocks of one system are contiguous in the
+// work list.
+struct RowWork {
+    int64_t row_begin, row_end;  // global rows
+    int32_t sys;
+    int32_t pad;
+};
+
+// A block of columns of one system, for the cols-length vector kernels.
+struct ColWork {
+    int64_t col_begin, col_end;  // global columns
+    int32_t sys;
+    int32_t pad;
+};
+
+// A scatter tile of A^T y: rows [row_begin,row_end) x the slots of `group`,
+// accumulating the column window [win_lo, win_hi) in LDS.
+struct ScatterWork {
+    int64_t row_begin, row_end;  // global rows
+    uint32_t win_lo, win_hi;     // global columns
+    int32_t group;               // index into the slot-group table
+    int32_t sys;
+};
+
+// Slots whose column ranges overlap: a column index can repeat inside a row
+// only among the slots of one group.
+struct SlotGroup {
+    uint32_t col_lo, col_hi;  // union of the slots' column ranges (local to a system)
+    uint16_t nslots, first;   // slots: slot_list[first .. first+nslots)
+};
+
+// Per-system state of the device-resident LSQR recurrence (FP64 scalars).
+struct LsqrScalars {
+    double alpha, beta;       // current bidiagonalisation scalars
+    double rhobar, phibar;
+    double step_x, step_w;    // phi/rho and theta/rho of the last rotation
+    double nres;              // |Abar^T r_k| = phibar_{k+1} alpha_{k+1} |c_k|
+    double nres0;             // alpha_1 beta_1 = |Abar^T b|
+    double rnorm;             // phibar = |b - Abar y_k|
+    double anorm2;            // sum of alpha^2 + beta^2 (Frobenius estimate of Abar)
+    double reduce_a, reduce_b;// scratch for explicit reductions
+    int32_t active;           // 0: converged / frozen
+    int32_t iters;
+    double pad;
+};
+static_assert(sizeof(LsqrScalars) % 16 == 0, "LsqrScalars layout");
+
+// Per-system state of the device-resident PCG recurrence.
+struct PcgScalars {
+    double rz, rz_new, pq, alpha, beta, rr;
+    int32_t active, iters;
+    double pad;
+};
+
+// Per-system grid bookkeeping shared by the reducing kernels.
+struct GridSys {
+    const int32_t* __restrict__ blk_first;  // first block of each system
+    const int32_t* __restrict__ blk_count;  // blocks of each system
+    unsigned* counters;                     // per system
+    double* partials;                       // per block (two slots: [2*b], [2*b+1])
+};
+
+}  // namespace cad

@@ -1,0 +1,742 @@
+// HIP kernels of the pattern-evaluator fit (gfx950 / CDNA4).
+//
+// Layout of a stored matrix (fixed k entries per row, implicit value 1):
+//   slot-major: idx[s * ld + r], r < rows, s < k, column = sys_base + base[s] + idx
+//   (u16 when every slot's column range is narrower than 65536, else u32).
+// Every row-parallel kernel walks a RowWork list, every column-parallel kernel a
+// ColWork list and the scatter kernel a ScatterWork list, so one launch covers
+// any number of independent systems (block-diagonal batches).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "board.hpp"
+#include "device.hpp"
+
+namespace cad {
+// Kernels have internal linkage: every translation unit instantiates the ones
+// it launches.
+namespace {
+
+constexpr int kRowBlock = 256;   // threads of the row / column kernels
+constexpr int kRowsPerThread = 4;
+constexpr int kScatterBlock = 1024;
+
+// ------------------------------------------------------------ index views --
+template <class IdxT>
+struct SlotMajor {
+    const IdxT* __restrict__ idx;
+    int64_t ld;
+    const uint32_t* __restrict__ base;  // per slot
+    __device__ __forceinline__ uint32_t col(int s, int64_t r) const
+    {
+        return base[s] + uint32_t(idx[int64_t(s) * ld + r]);
+    }
+};
+
+// Row-major int32 indices as torch hands them over ([B, k], contiguous).
+struct RowMajor32 {
+    const int32_t* __restrict__ idx;
+    int k;
+    __device__ __forceinline__ uint32_t col(int s, int64_t r) const
+    {
+        return uint32_t(idx[r * k + s]);
+    }
+};
+
+// ------------------------------------------------------------ reductions ---
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the block; the result is valid in thread 0.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < BS / 64; i++) t += sh[i];
+    return t;
+}
+
+// Single-pass grid reduction: every block of a system stores its partial, the
+// last block to arrive (ticket on the system's counter) sees all partials and
+// returns true.  The counter is re-armed by that last block, so the kernel can
+// be replayed (graphs) without clearing it.
+__device__ __forceinline__ bool arrive_last(unsigned* counter, unsigned nblocks)
+{
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned t = atomicAdd(counter, 1u);
+        last = (t == nblocks - 1);
+        if (last) atomicExch(counter, 0u);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// ------------------------------------------------------------ fill --------
+template <class T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+template <class TO, class TI>
+__global__ void k_convert(TO* __restrict__ out, const TI* __restrict__ in, int64_t n)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = TO(in[i]);
+}
+
+// ------------------------------------------------------------ index --------
+// Pattern/Indexer.cpp InsertIndices for every row: one thread per position.
+// out[s * slot_stride + r * row_stride] = (local ? 0 : col_offset) + index.
+__device__ __forceinline__ uint32_t list_value(const uint8_t* __restrict__ sq, int off, int len,
+                                               uint64_t P, uint64_t O)
+{
+    uint32_t v = 0;
+    for (int j = len - 1; j >= 0; --j) {
+        const int q = sq[off + j];
+        v = v * 3u + uint32_t((P >> q) & 1u) + 2u * uint32_t((O >> q) & 1u);
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t slot_index(const DevSlot& s, const uint8_t* __restrict__ sq,
+                                               const uint32_t* __restrict__ dense, uint64_t P, uint64_t O)
+{
+    const uint32_t a = list_value(sq, s.off[0], s.len[0], P, O);
+    if (s.kind == 0) return a;
+    if (s.kind == 3) return dense[s.dense_offset + a];
+    const uint32_t b = list_value(sq, s.off[1], s.len[1], P, O);
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    uint32_t idx = lo * s.hs + hi - lo * (lo + 1u) / 2u;
+    if (s.kind == 2) idx = idx * s.ds + list_value(sq, s.off[2], s.len[2], P, O);
+    return idx;
+}
+
+template <class OutT>
+__global__ __launch_bounds__(kRowBlock) void k_index_extract(
+    const uint64_t* __restrict__ pos, int64_t n, const DevSlot* __restrict__ slots, int k,
+    const uint8_t* __restrict__ sq, const uint32_t* __restrict__ dense, OutT* __restrict__ out,
+    int64_t slot_stride, int64_t row_stride, int local)
+{
+    const int64_t r = blockIdx.x * int64_t(kRowBlock) + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t P = pos[2 * r], O = pos[2 * r + 1];
+    for (int s = 0; s < k; s++) {
+        const DevSlot sl = slots[s];
+        uint32_t v = slot_index(sl, sq, dense, P, O);
+        if (!local) v += sl.col_offset;
+        out[int64_t(s) * slot_stride + r * row_stride] = OutT(v);
+    }
+}
+
+// ------------------------------------------------------------ SpMV ---------
+// y = A x with the in-row sum taken sequentially in slot order from zero, the
+// order of Math/MatrixCSR.cpp:59-62 (bit-exact with it in FP32).
+template <class VT, class View>
+__device__ __forceinline__ VT row_sum(const View& m, int k, int64_t r, uint32_t cb, const VT* __restrict__ x)
+{
+    VT acc = VT(0);
+#pragma unroll 8
+    for (int s = 0; s < k; s++) acc += x[cb + m.col(s, r)];
+    return acc;
+}
+
+template <class VT, class View>
+__global__ __launch_bounds__(kRowBlock) void k_spmv(View m, int k, const RowWork* __restrict__ work,
+                                                    const uint32_t* __restrict__ sys_col, const VT* __restrict__ x,
+                                                    VT* __restrict__ y)
+{
+    const RowWork w = work[blockIdx.x];
+    const uint32_t cb = sys_col[w.sys];
+    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock) y[r] = row_sum<VT>(m, k, r, cb, x);
+}
+
+// FP64 residual of the original system: r = b - A x.  |r|^2 is reduced per
+// system into scal[sys].reduce_a and scal[sys].beta = |r|; when u is given it
+// also receives r converted to VT (the start vector of a refinement solve).
+template <class VT, class View>
+__global__ __launch_bounds__(kRowBlock) void k_residual(View m, int k, const RowWork* __restrict__ work,
+                                                        const uint32_t* __restrict__ sys_col,
+                                                        const double* __restrict__ x, const double* __restrict__ b,
+                                                        double* __restrict__ r, VT* __restrict__ u, GridSys g,
+                                                        LsqrScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const RowWork w = work[blockIdx.x];
+    if (!scal[w.sys].active) return;
+    const uint32_t cb = sys_col[w.sys];
+    double acc = 0.0;
+    for (int64_t i = w.row_begin + threadIdx.x; i < w.row_end; i += kRowBlock) {
+        const double v = b[i] - row_sum<double>(m, k, i, cb, x);
+        r[i] = v;
+        if (u) u[i] = VT(v);
+        acc += v * v;
+    }
+    acc = block_sum<kRowBlock>(acc, sh);
+    if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
+        s = block_sum<kRowBlock>(s, sh);
+        if (threadIdx.x == 0) {
+            scal[w.sys].reduce_a = s;
+            scal[w.sys].beta = sqrt(s);
+            scal[w.sys].anorm2 = 0.0;
+        }
+    }
+}
+
+// LSQR step 1 (Math/Solver.h:161): u <- A v - alpha u.  u is kept unnormalised
+// as u_raw with u = u_raw / beta; vd = D v is the gather source.  The new beta
+// is |u_raw| (FP64 reduction).
+template <class VT, class View>
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_u(View m, int k, const RowWork* __restrict__ work,
+                                                      const uint32_t* __restrict__ sys_col,
+                                                      const VT* __restrict__ vd, VT* __restrict__ u,
+                                                      GridSys g, LsqrScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const RowWork w = work[blockIdx.x];
+    LsqrScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    const uint32_t cb = sys_col[w.sys];
+    const double beta = st->beta;
+    const VT f = VT(beta != 0.0 ? st->alpha / beta : 0.0);
+    double acc = 0.0;
+    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock) {
+        const VT v = row_sum<VT>(m, k, r, cb, vd) - f * u[r];
+        u[r] = v;
+        acc += double(v) * double(v);
+    }
+    acc = block_sum<kRowBlock>(acc, sh);
+    if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
+        s = block_sum<kRowBlock>(s, sh);
+        if (threadIdx.x == 0) {
+            st->beta = sqrt(s);
+            st->anorm2 += s;
+        }
+    }
+}
+
+// Rows init of LSQR: u_raw = src (converted), beta = |src| (FP64).
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_init_u(const RowWork* __restrict__ work,
+                                                           const double* __restrict__ src, VT* __restrict__ u,
+                                                           GridSys g, LsqrScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const RowWork w = work[blockIdx.x];
+    LsqrScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    double acc = 0.0;
+    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock) {
+        const double v = src[r];
+        u[r] = VT(v);
+        acc += v * v;
+    }
+    acc = block_sum<kRowBlock>(acc, sh);
+    if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
+        s = block_sum<kRowBlock>(s, sh);
+        if (threadIdx.x == 0) {
+            st->beta = sqrt(s);
+            st->anorm2 = 0.0;
+        }
+    }
+}
+
+// ------------------------------------------------------------ scatter ------
+// z += A^T (y * scale): each block owns a column window of one slot group in
+// LDS, scatters a range of rows into it (ds_add) and flushes the window to
+// global memory with one atomic per touched column.
+// scale = 1 / div[sys] when div is given (0 when div[sys] == 0), else 1.
+template <class VT, class YT, class View>
+__global__ __launch_bounds__(kScatterBlock) void k_scatter(View m, const ScatterWork* __restrict__ work,
+                                                           const SlotGroup* __restrict__ groups,
+                                                           const uint16_t* __restrict__ slot_list,
+                                                           const uint32_t* __restrict__ sys_col,
+                                                           const YT* __restrict__ y, const LsqrScalars* div,
+                                                           VT* __restrict__ z)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    VT* win = reinterpret_cast<VT*>(smem_raw);
+    const ScatterWork w = work[blockIdx.x];
+    if (div && !div[w.sys].active) return;
+    const uint32_t W = w.win_hi - w.win_lo;
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = VT(0);
+    __syncthreads();
+    const SlotGroup gr = groups[w.group];
+    const uint32_t cb = sys_col[w.sys];
+    VT sc = VT(1);
+    if (div) {
+        const double d = div[w.sys].beta;
+        sc = VT(d != 0.0 ? 1.0 / d : 0.0);
+    }
+    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kScatterBlock) {
+        const VT yv = VT(y[r]) * sc;
+        for (int j = 0; j < gr.nslots; j++) {
+            const int s = slot_list[gr.first + j];
+            const uint32_t uc = cb + m.col(s, r) - w.win_lo;
+            if (uc < W) atomicAdd(&win[uc], yv);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) {
+        const VT v = win[i];
+        if (v != VT(0)) atomicAdd(&z[w.win_lo + i], v);
+    }
+}
+
+// Jacobi of A^T A (Math/MatrixCSR.cpp:21-47): diag_c = sum over rows of the
+// squared multiplicity of c in the row.  Each (row, slot) adds the multiplicity
+// of its column inside the slot group (repeats can only occur within a group),
+// which sums to mult^2 per row.  Counts are exact 32-bit integers.
+template <class View>
+__global__ __launch_bounds__(kScatterBlock) void k_jacobi_count(View m, const ScatterWork* __restrict__ work,
+                                                                const SlotGroup* __restrict__ groups,
+                                                                const uint16_t* __restrict__ slot_list,
+                                                                const uint32_t* __restrict__ sys_col,
+                                                                uint32_t* __restrict__ cnt)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint32_t* win = reinterpret_cast<uint32_t*>(smem_raw);
+    const ScatterWork w = work[blockIdx.x];
+    const uint32_t W = w.win_hi - w.win_lo;
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = 0u;
+    __syncthreads();
+    const SlotGroup gr = groups[w.group];
+    const uint32_t cb = sys_col[w.sys];
+    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kScatterBlock) {
+        for (int j = 0; j < gr.nslots; j++) {
+            const uint32_t c = cb + m.col(slot_list[gr.first + j], r);
+            const uint32_t uc = c - w.win_lo;
+            if (uc >= W) continue;
+            uint32_t mult = 0;
+            for (int t = 0; t < gr.nslots; t++) mult += (cb + m.col(slot_list[gr.first + t], r)) == c;
+            atomicAdd(&win[uc], mult);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) {
+        const uint32_t v = win[i];
+        if (v) atomicAdd(&cnt[w.win_lo + i], v);
+    }
+}
+
+// 1 / diag in float, exactly as inv() of the reference (1/0 = inf).
+__global__ void k_jacobi_finish(const uint32_t* __restrict__ cnt, int64_t n, float* __restrict__ d)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        d[i] = 1.0f / float(cnt[i]);
+}
+
+// Column scaling of LSQR: D = 1/|a_c| = sqrt(jacobi), 1 for empty columns.
+__global__ void k_colscale(const uint32_t* __restrict__ cnt, int64_t n, double* __restrict__ D)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        D[i] = cnt[i] ? 1.0 / sqrt(double(cnt[i])) : 1.0;
+}
+
+// ------------------------------------------------------------ LSQR cols ----
+// LSQR step 2 (Math/Solver.h:162) and the rotation (:165-171):
+//   vbar = D z - beta v, alpha = |vbar|, then rho, c, s, theta, phi, phibar.
+// z is cleared for the next scatter.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict__ work,
+                                                      const double* __restrict__ D, VT* __restrict__ z,
+                                                      double* __restrict__ v, GridSys g, LsqrScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const ColWork w = work[blockIdx.x];
+    LsqrScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    const double beta = st->beta;
+    double acc = 0.0;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const double vb = D[c] * double(z[c]) - beta * v[c];
+        z[c] = VT(0);
+        v[c] = vb;
+        acc += vb * vb;
+    }
+    acc = block_sum<kRowBlock>(acc, sh);
+    if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
+        s = block_sum<kRowBlock>(s, sh);
+        if (threadIdx.x == 0) {
+            const double alpha = sqrt(s);
+            const double rho = sqrt(st->rhobar * st->rhobar + beta * beta);
+            const double c = rho != 0.0 ? st->rhobar / rho : 0.0;
+            const double sn = rho != 0.0 ? beta / rho : 0.0;
+            const double theta = sn * alpha;
+            const double phi = c * st->phibar;
+            st->rhobar = -c * alpha;
+            st->phibar = sn * st->phibar;
+            st->step_x = rho != 0.0 ? phi / rho : 0.0;
+            st->step_w = rho != 0.0 ? theta / rho : 0.0;
+            st->alpha = alpha;
+            st->nres = st->phibar * alpha * fabs(c);
+            st->rnorm = st->phibar;
+            st->anorm2 += s;
+            st->iters += 1;
+        }
+    }
+}
+
+// LSQR update (Math/Solver.h:173-175): v = vbar/alpha, vd = D v,
+// y += (phi/rho) w, w = v - (theta/rho) w.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_xw(const ColWork* __restrict__ work,
+                                                       const double* __restrict__ D, double* __restrict__ v,
+                                                       VT* __restrict__ vd, double* __restrict__ w_,
+                                                       double* __restrict__ y, const LsqrScalars* scal)
+{
+    const ColWork w = work[blockIdx.x];
+    const LsqrScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    const double ia = st->alpha != 0.0 ? 1.0 / st->alpha : 0.0;
+    const double sx = st->step_x, sw = st->step_w;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const double vn = v[c] * ia;
+        v[c] = vn;
+        vd[c] = VT(D[c] * vn);
+        const double wc = w_[c];
+        y[c] += sx * wc;
+        w_[c] = vn - sw * wc;
+    }
+}
+
+// LSQR start on the columns (Math/Solver.h:146-150), from an FP64 A^T u_raw:
+//   vbar = D z / beta, alpha = |vbar|; also |z|^2 = |A^T src|^2 into reduce_b.
+template <class ZT>
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_init_v(const ColWork* __restrict__ work,
+                                                           const double* __restrict__ D, ZT* __restrict__ z,
+                                                           double* __restrict__ v, GridSys g, LsqrScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const ColWork w = work[blockIdx.x];
+    LsqrScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    const double ib = st->beta != 0.0 ? 1.0 / st->beta : 0.0;
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const double zc = double(z[c]);
+        const double vb = D[c] * zc * ib;
+        z[c] = ZT(0);
+        v[c] = vb;
+        a0 += vb * vb;
+        a1 += zc * zc;
+    }
+    a0 = block_sum<kRowBlock>(a0, sh);
+    a1 = block_sum<kRowBlock>(a1, sh);
+    if (threadIdx.x == 0) {
+        g.partials[2 * blockIdx.x] = a0;
+        g.partials[2 * blockIdx.x + 1] = a1;
+    }
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) {
+            s0 += g.partials[2 * (g.blk_first[w.sys] + i)];
+            s1 += g.partials[2 * (g.blk_first[w.sys] + i) + 1];
+        }
+        s0 = block_sum<kRowBlock>(s0, sh);
+        s1 = block_sum<kRowBlock>(s1, sh);
+        if (threadIdx.x == 0) {
+            const double alpha = sqrt(s0);
+            st->alpha = alpha;
+            st->rhobar = alpha;
+            st->phibar = st->beta;
+            st->nres0 = alpha * st->beta;
+            st->nres = st->nres0;
+            st->rnorm = st->beta;
+            st->anorm2 = s0 + st->beta * 0.0;
+            st->reduce_b = s1;
+            st->iters = 0;
+        }
+    }
+}
+
+// v = vbar/alpha, vd = D v, w = v, y = 0.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_init_w(const ColWork* __restrict__ work,
+                                                           const double* __restrict__ D, double* __restrict__ v,
+                                                           VT* __restrict__ vd, double* __restrict__ w_,
+                                                           double* __restrict__ y, const LsqrScalars* scal)
+{
+    const ColWork w = work[blockIdx.x];
+    const LsqrScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    const double ia = st->alpha != 0.0 ? 1.0 / st->alpha : 0.0;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const double vn = v[c] * ia;
+        v[c] = vn;
+        vd[c] = VT(D[c] * vn);
+        w_[c] = vn;
+        y[c] = 0.0;
+    }
+}
+
+// x += D y (back to the unscaled space), y = 0.
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_apply(const ColWork* __restrict__ work,
+                                                          const double* __restrict__ D, double* __restrict__ y,
+                                                          double* __restrict__ x, const LsqrScalars* scal)
+{
+    const ColWork w = work[blockIdx.x];
+    if (!scal[w.sys].active) return;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        x[c] += D[c] * y[c];
+        y[c] = 0.0;
+    }
+}
+
+// |z|^2 per system into reduce_b (used for |A^T b| and |A^T r|); z is cleared.
+template <class ZT>
+__global__ __launch_bounds__(kRowBlock) void k_cols_norm2(const ColWork* __restrict__ work, ZT* __restrict__ z,
+                                                          GridSys g, LsqrScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const ColWork w = work[blockIdx.x];
+    LsqrScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    double acc = 0.0;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const double zc = double(z[c]);
+        z[c] = ZT(0);
+        acc += zc * zc;
+    }
+    acc = block_sum<kRowBlock>(acc, sh);
+    if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
+        s = block_sum<kRowBlock>(s, sh);
+        if (threadIdx.x == 0) st->reduce_b = s;
+    }
+}
+
+// out = D y (the unscaled solution of a column-scaled LSQR).
+__global__ void k_scale_mul(const double* __restrict__ D, const double* __restrict__ y, int64_t n,
+                            double* __restrict__ out)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = D[i] * y[i];
+}
+
+// ------------------------------------------------------------ PCG ---------
+// PCG on A^T A with a diagonal preconditioner (Math/Solver.h:78-117); the
+// FP64 scalars live on the device.  pq = p . q, alpha = rz / pq.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_pcg_alpha(const ColWork* __restrict__ work, const VT* __restrict__ p,
+                                                         const VT* __restrict__ q, GridSys g, PcgScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const ColWork w = work[blockIdx.x];
+    PcgScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    double acc = 0.0;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) acc += double(p[c]) * double(q[c]);
+    acc = block_sum<kRowBlock>(acc, sh);
+    if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
+        s = block_sum<kRowBlock>(s, sh);
+        if (threadIdx.x == 0) {
+            st->pq = s;
+            st->alpha = s != 0.0 ? st->rz / s : 0.0;
+        }
+    }
+}
+
+// x += alpha p, r -= alpha q, zz = P r; rz' = r.zz, rr = r.r; beta = rz'/rz.
+// faithful: the scalars are rounded to VT before use, as in the float solver.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_pcg_update(const ColWork* __restrict__ work, const VT* __restrict__ P,
+                                                          const VT* __restrict__ p, const VT* __restrict__ q,
+                                                          VT* __restrict__ x, VT* __restrict__ r, VT* __restrict__ zz,
+                                                          GridSys g, PcgScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const ColWork w = work[blockIdx.x];
+    PcgScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    const VT a = VT(st->alpha);
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        x[c] += a * p[c];
+        const VT rc = r[c] - a * q[c];
+        r[c] = rc;
+        const VT zc = P[c] * rc;
+        zz[c] = zc;
+        a0 += double(rc) * double(zc);
+        a1 += double(rc) * double(rc);
+    }
+    a0 = block_sum<kRowBlock>(a0, sh);
+    a1 = block_sum<kRowBlock>(a1, sh);
+    if (threadIdx.x == 0) {
+        g.partials[2 * blockIdx.x] = a0;
+        g.partials[2 * blockIdx.x + 1] = a1;
+    }
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) {
+            s0 += g.partials[2 * (g.blk_first[w.sys] + i)];
+            s1 += g.partials[2 * (g.blk_first[w.sys] + i) + 1];
+        }
+        s0 = block_sum<kRowBlock>(s0, sh);
+        s1 = block_sum<kRowBlock>(s1, sh);
+        if (threadIdx.x == 0) {
+            st->beta = st->rz != 0.0 ? s0 / st->rz : 0.0;
+            st->rz = s0;
+            st->rr = s1;
+            st->iters += 1;
+        }
+    }
+}
+
+// p = zz + beta p; q = 0 for the next scatter.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_pcg_p(const ColWork* __restrict__ work, const VT* __restrict__ zz,
+                                                     VT* __restrict__ p, VT* __restrict__ q, const PcgScalars* scal)
+{
+    const ColWork w = work[blockIdx.x];
+    const PcgScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    const VT b = VT(st->beta);
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        p[c] = zz[c] + b * p[c];
+        q[c] = VT(0);
+    }
+}
+
+// PCG start: r = rhs (A^T b, x0 = 0), zz = P r, p = zz, rz = r.zz, q = 0.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_pcg_init(const ColWork* __restrict__ work, const VT* __restrict__ P,
+                                                        VT* __restrict__ rhs_r, VT* __restrict__ zz, VT* __restrict__ p,
+                                                        VT* __restrict__ x, VT* __restrict__ q, GridSys g,
+                                                        PcgScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const ColWork w = work[blockIdx.x];
+    PcgScalars* st = &scal[w.sys];
+    if (!st->active) return;
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const VT rc = rhs_r[c];
+        const VT zc = P[c] * rc;
+        zz[c] = zc;
+        p[c] = zc;
+        x[c] = VT(0);
+        q[c] = VT(0);
+        a0 += double(rc) * double(zc);
+        a1 += double(rc) * double(rc);
+    }
+    a0 = block_sum<kRowBlock>(a0, sh);
+    a1 = block_sum<kRowBlock>(a1, sh);
+    if (threadIdx.x == 0) {
+        g.partials[2 * blockIdx.x] = a0;
+        g.partials[2 * blockIdx.x + 1] = a1;
+    }
+    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) {
+            s0 += g.partials[2 * (g.blk_first[w.sys] + i)];
+            s1 += g.partials[2 * (g.blk_first[w.sys] + i) + 1];
+        }
+        s0 = block_sum<kRowBlock>(s0, sh);
+        s1 = block_sum<kRowBlock>(s1, sh);
+        if (threadIdx.x == 0) {
+            st->rz = s0;
+            st->rr = s1;
+            st->iters = 0;
+        }
+    }
+}
+
+// DiagonalPreconditioner (Math/Solver.h:29-34): non-normal entries become 1.
+template <class VT>
+__global__ void k_precond_from_jacobi(const float* __restrict__ jac, int64_t n, VT* __restrict__ P)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const float d = jac[i];
+        P[i] = VT(__builtin_isnormal(d) ? d : 1.0f);
+    }
+}
+
+// ------------------------------------------------------------ evaluator ----
+// Forward of the differentiable evaluator: out[r] = sum_s w[idx[r,s]] in slot
+// order (Pattern/Evaluator.cpp:48-54 sums the same terms in the same order).
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_eval_forward(const int32_t* __restrict__ idx, int64_t B, int k,
+                                                            const VT* __restrict__ w, uint32_t cols,
+                                                            VT* __restrict__ out)
+{
+    const int64_t r = blockIdx.x * int64_t(kRowBlock) + threadIdx.x;
+    if (r >= B) return;
+    VT acc = VT(0);
+    for (int s = 0; s < k; s++) {
+        const uint32_t c = uint32_t(idx[r * k + s]);
+        if (c < cols) acc += w[c];
+    }
+    out[r] = acc;
+}
+
+// ------------------------------------------------------------ data ---------
+__global__ __launch_bounds__(kRowBlock) void k_random_play(uint64_t seed, int64_t n, int e_lo, int e_hi,
+                                                           uint64_t* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * int64_t(kRowBlock) + threadIdx.x;
+    if (i >= n) return;
+    const Pos p = random_play(seed, uint64_t(i), e_lo, e_hi);
+    out[2 * i] = p.P;
+    out[2 * i + 1] = p.O;
+}
+
+__global__ void k_targets_x(uint64_t seed, int64_t cols, double sx, double* __restrict__ xt)
+{
+    for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < cols; c += int64_t(gridDim.x) * blockDim.x) {
+        Stream s(seed, uint64_t(c));
+        xt[c] = __dmul_rn(sx, s.ih12());
+    }
+}
+
+// b = clip(2 * floor((A x_true + sn * N) / 2 + 1/2), -64, 64)
+template <class View>
+__global__ __launch_bounds__(kRowBlock) void k_targets_b(View m, int k, const RowWork* __restrict__ work,
+                                                         const uint32_t* __restrict__ sys_col,
+                                                         const double* __restrict__ xt, uint64_t seed, double sn,
+                                                         double* __restrict__ b)
+{
+    const RowWork w = work[blockIdx.x];
+    const uint32_t cb = sys_col[w.sys];
+    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock) {
+        const double s = row_sum<double>(m, k, r, cb, xt);
+        Stream rng(seed, uint64_t(r));
+        const double v = __dadd_rn(s, __dmul_rn(sn, rng.ih12()));
+        const double e = 2.0 * floor(v * 0.5 + 0.5);
+        b[r] = fmin(64.0, fmax(-64.0, e));
+    }
+}
+
+}  // namespace
+}  // namespace cad

@@ -1,0 +1,122 @@
+// Host runtime objects behind the C-ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "patterns.hpp"
+
+namespace cad {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define CAD_HIP(x) ::cad::hip_check((x), #x)
+
+struct Context;
+
+// Owning device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    explicit DBuf(size_t n);
+    ~DBuf();
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DBuf& operator=(DBuf&& o) noexcept;
+    template <class T> T* as() const { return static_cast<T*>(p); }
+    void upload(const void* h, size_t n, hipStream_t s);
+};
+
+template <class T>
+DBuf to_device(const std::vector<T>& v, hipStream_t s)
+{
+    DBuf b(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (!v.empty()) b.upload(v.data(), v.size() * sizeof(T), s);
+    return b;
+}
+
+// Slot groups + per-slot column ranges -> scatter work lists.
+struct ScatterPlan {
+    std::vector<SlotGroup> groups;
+    std::vector<uint16_t> slot_list;
+    DBuf d_groups, d_slot_list;
+    // one work list per value size (4: float windows, 8: double windows)
+    std::vector<ScatterWork> work4, work8, work_cnt;
+    DBuf d_work4, d_work8, d_work_cnt;
+    uint32_t win4 = 0, win8 = 0, win_cnt = 0;  // widest window of each list
+};
+
+// Blocks of rows / columns of the systems, with the per-system grid
+// reduction bookkeeping.
+struct GridPlan {
+    std::vector<RowWork> rows;
+    std::vector<ColWork> cols;
+    DBuf d_rows, d_cols;
+    std::vector<int32_t> row_first, row_count, col_first, col_count;
+    DBuf d_row_first, d_row_count, d_col_first, d_col_count;
+    DBuf d_row_counters, d_col_counters, d_partials;
+    GridSys row_sys() const;
+    GridSys col_sys() const;
+};
+
+struct Workspace;
+
+struct Context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    size_t max_lds = 0;
+    ~Context();
+};
+
+}  // namespace cad
+
+struct cad_context : cad::Context {};
+
+struct cad_patterns {
+    cad_context* ctx = nullptr;
+    std::unique_ptr<cad::PatternGroup> group;
+    cad::DBuf d_slots, d_sq, d_dense;
+    std::vector<uint32_t> slot_lo, slot_hi;  // global column range of every slot
+    cad::ScatterPlan eval_plan;              // for evaluator backward, built per B
+    int64_t eval_plan_rows = -1;
+    cad::DBuf d_base0, d_syscol0;            // zeros
+};
+
+struct cad_matrix {
+    cad_context* ctx = nullptr;
+    int64_t rows = 0, cols = 0;
+    int k = 0;
+    int ibytes = 4;                 // 2: u16 local + slot base, 4: u32 with base 0
+    cad::DBuf d_idx, d_base;        // slot-major
+    std::vector<uint32_t> slot_lo, slot_hi;
+    int nsys = 1;
+    std::vector<int64_t> sys_row_begin;  // nsys + 1
+    std::vector<uint32_t> sys_col;       // column base of each system
+    int64_t cols_per_sys = 0;
+    cad::DBuf d_sys_col;
+    cad::ScatterPlan splan;
+    cad::GridPlan grid;
+    cad::DBuf d_jacobi_cnt;         // cached exact multiplicity counts (u32)
+    bool have_cnt = false;
+    std::map<int, std::unique_ptr<cad::Workspace>> ws;
+    ~cad_matrix();
+};
+
+namespace cad {
+void build_plans(cad_matrix* m);
+void set_error(const std::string& s);
+}  // namespace cad

@@ -1,0 +1,601 @@
+// Products, Jacobi and the device-resident solvers (LSQR, mixed-precision
+// refined LSQR, PCG / CG on A^T A).
+#include "solver.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "kernels.hpp"
+
+namespace cad {
+
+// ------------------------------------------------------------ helpers -----
+template <class F>
+void with_view(cad_matrix* m, F&& f)
+{
+    if (m->ibytes == 2)
+        f(SlotMajor<uint16_t>{m->d_idx.as<uint16_t>(), m->rows, m->d_base.as<uint32_t>()});
+    else
+        f(SlotMajor<uint32_t>{m->d_idx.as<uint32_t>(), m->rows, m->d_base.as<uint32_t>()});
+}
+
+int grid_for(int64_t n, int bs = 256)
+{
+    int64_t g = (n + bs - 1) / bs;
+    return int(std::max<int64_t>(1, std::min<int64_t>(g, 65536)));
+}
+
+template <class T>
+void fill(T* p, int64_t n, T v, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(256), 0, s, p, n, v);
+    CAD_HIP(hipGetLastError());
+}
+
+template <class K>
+void allow_lds(K kernel, size_t bytes)
+{
+    if (bytes > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+}
+
+template <class VT, class YT, class View>
+void launch_scatter(const ScatterPlan& sp, const View& view, const uint32_t* sys_col, const YT* y,
+                    const LsqrScalars* div, VT* z, hipStream_t s)
+{
+    const bool f = sizeof(VT) == 4;
+    const size_t n = f ? sp.work4.size() : sp.work8.size();
+    if (!n) return;
+    const ScatterWork* wl = (f ? sp.d_work4 : sp.d_work8).as<ScatterWork>();
+    const size_t lds = size_t(f ? sp.win4 : sp.win8) * sizeof(VT);
+    auto kern = k_scatter<VT, YT, View>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kScatterBlock), lds, s, view, wl,
+                       sp.d_groups.as<SlotGroup>(), sp.d_slot_list.as<uint16_t>(), sys_col, y, div, z);
+    CAD_HIP(hipGetLastError());
+}
+
+template <class VT, class YT>
+void scatter(cad_matrix* m, const YT* y, const LsqrScalars* div, VT* z, hipStream_t s)
+{
+    with_view(m, [&](auto view) { launch_scatter<VT, YT>(m->splan, view, m->d_sys_col.as<uint32_t>(), y, div, z, s); });
+}
+
+template <class VT>
+void spmv(cad_matrix* m, const VT* x, VT* y, hipStream_t s)
+{
+    const GridPlan& g = m->grid;
+    with_view(m, [&](auto view) {
+        hipLaunchKernelGGL((k_spmv<VT, decltype(view)>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s, view,
+                           m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), x, y);
+    });
+    CAD_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ products ----
+void matrix_spmv_f32(cad_matrix* m, const float* x, float* y) { spmv<float>(m, x, y, m->ctx->stream); }
+void matrix_spmv_f64(cad_matrix* m, const double* x, double* y) { spmv<double>(m, x, y, m->ctx->stream); }
+
+void matrix_spmtv_f32(cad_matrix* m, const float* y, float* z)
+{
+    fill<float>(z, m->cols, 0.0f, m->ctx->stream);
+    scatter<float, float>(m, y, nullptr, z, m->ctx->stream);
+}
+
+void matrix_spmtv_f64(cad_matrix* m, const double* y, double* z)
+{
+    fill<double>(z, m->cols, 0.0, m->ctx->stream);
+    scatter<double, double>(m, y, nullptr, z, m->ctx->stream);
+}
+
+const uint32_t* jacobi_counts(cad_matrix* m)
+{
+    if (m->have_cnt) return m->d_jacobi_cnt.as<uint32_t>();
+    hipStream_t s = m->ctx->stream;
+    m->d_jacobi_cnt = DBuf(sizeof(uint32_t) * std::max<int64_t>(1, m->cols));
+    fill<uint32_t>(m->d_jacobi_cnt.as<uint32_t>(), m->cols, 0u, s);
+    const ScatterPlan& sp = m->splan;
+    if (!sp.work_cnt.empty()) {
+        const size_t lds = size_t(sp.win_cnt) * 4;
+        with_view(m, [&](auto view) {
+            auto kern = k_jacobi_count<decltype(view)>;
+            allow_lds(kern, lds);
+            hipLaunchKernelGGL(kern, dim3(unsigned(sp.work_cnt.size())), dim3(kScatterBlock), lds, s, view,
+                               sp.d_work_cnt.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
+                               sp.d_slot_list.as<uint16_t>(), m->d_sys_col.as<uint32_t>(),
+                               m->d_jacobi_cnt.as<uint32_t>());
+        });
+        CAD_HIP(hipGetLastError());
+    }
+    m->have_cnt = true;
+    return m->d_jacobi_cnt.as<uint32_t>();
+}
+
+void matrix_jacobi(cad_matrix* m, float* d)
+{
+    const uint32_t* cnt = jacobi_counts(m);
+    hipLaunchKernelGGL(k_jacobi_finish, dim3(grid_for(m->cols)), dim3(256), 0, m->ctx->stream, cnt, m->cols, d);
+    CAD_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ workspace ---
+Workspace::~Workspace()
+{
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (host_scal) (void)hipHostFree(host_scal);
+}
+
+static Workspace& workspace(cad_matrix* m, int key)
+{
+    auto& w = m->ws[key];
+    if (!w) {
+        w.reset(new Workspace);
+        w->key = key;
+        CAD_HIP(hipHostMalloc(&w->host_scal, sizeof(LsqrScalars) * std::max(1, m->nsys) * 2 + 4096));
+    }
+    return *w;
+}
+
+static void ensure(DBuf& b, size_t bytes)
+{
+    if (b.bytes < bytes) b = DBuf(bytes);
+}
+
+// Reads back the per-system scalars (synchronises the stream).
+template <class S>
+static const S* read_scalars(cad_matrix* m, Workspace& ws, const DBuf& d, hipStream_t s)
+{
+    const size_t bytes = sizeof(S) * m->nsys;
+    CAD_HIP(hipMemcpyAsync(ws.host_scal, d.p, bytes, hipMemcpyDeviceToHost, s));
+    CAD_HIP(hipStreamSynchronize(s));
+    return static_cast<const S*>(ws.host_scal);
+}
+
+static void set_active(cad_matrix* m, Workspace& ws, DBuf& d, const std::vector<int>& active, hipStream_t s)
+{
+    std::vector<LsqrScalars> h(m->nsys);
+    std::memset(h.data(), 0, sizeof(LsqrScalars) * m->nsys);
+    for (int i = 0; i < m->nsys; i++) h[i].active = active[i];
+    CAD_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(LsqrScalars) * m->nsys, hipMemcpyHostToDevice, s));
+    CAD_HIP(hipStreamSynchronize(s));
+    (void)ws;
+}
+
+// Column scaling D = 1/|a_c| (1 for empty columns) or all ones.
+static void column_scaling(cad_matrix* m, double* D, bool on, hipStream_t s)
+{
+    if (on) {
+        const uint32_t* cnt = jacobi_counts(m);
+        hipLaunchKernelGGL(k_colscale, dim3(grid_for(m->cols)), dim3(256), 0, s, cnt, m->cols, D);
+        CAD_HIP(hipGetLastError());
+    } else {
+        fill<double>(D, m->cols, 1.0, s);
+    }
+}
+
+// Explicit FP64 |A^T(b - A x)| and |b - A x| per system (one FP64 SpMV and one
+// FP64 scatter).  Uses its own scalar block so that solver state is untouched.
+static void explicit_residual(cad_matrix* m, Workspace& ws, const double* x, const double* b,
+                              std::vector<double>& nres, std::vector<double>& rnorm, hipStream_t s)
+{
+    const GridPlan& g = m->grid;
+    ensure(ws.r64, sizeof(double) * m->rows);
+    ensure(ws.z64, sizeof(double) * m->cols);
+    ensure(ws.scal_chk, sizeof(LsqrScalars) * m->nsys);
+    std::vector<int> act(m->nsys, 1);
+    set_active(m, ws, ws.scal_chk, act, s);
+    if (!ws.z64_clear) {
+        fill<double>(ws.z64.as<double>(), m->cols, 0.0, s);
+        ws.z64_clear = true;
+    }
+    LsqrScalars* sc = ws.scal_chk.as<LsqrScalars>();
+    with_view(m, [&](auto view) {
+        hipLaunchKernelGGL((k_residual<double, decltype(view)>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s,
+                           view, m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), x, b,
+                           ws.r64.as<double>(), (double*)nullptr, g.row_sys(), sc);
+    });
+    CAD_HIP(hipGetLastError());
+    scatter<double, double>(m, ws.r64.as<double>(), nullptr, ws.z64.as<double>(), s);
+    hipLaunchKernelGGL(k_cols_norm2<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
+                       g.d_cols.as<ColWork>(), ws.z64.as<double>(), g.col_sys(), sc);
+    CAD_HIP(hipGetLastError());
+    const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal_chk, s);
+    nres.resize(m->nsys);
+    rnorm.resize(m->nsys);
+    for (int i = 0; i < m->nsys; i++) {
+        nres[i] = std::sqrt(h[i].reduce_b);
+        rnorm[i] = std::sqrt(h[i].reduce_a);
+    }
+}
+
+// |A^T b| per system in FP64.
+static std::vector<double> atb_norms(cad_matrix* m, Workspace& ws, const double* b, hipStream_t s)
+{
+    const GridPlan& g = m->grid;
+    ensure(ws.z64, sizeof(double) * m->cols);
+    ensure(ws.scal_chk, sizeof(LsqrScalars) * m->nsys);
+    std::vector<int> act(m->nsys, 1);
+    set_active(m, ws, ws.scal_chk, act, s);
+    fill<double>(ws.z64.as<double>(), m->cols, 0.0, s);
+    ws.z64_clear = true;
+    scatter<double, double>(m, b, nullptr, ws.z64.as<double>(), s);
+    hipLaunchKernelGGL(k_cols_norm2<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
+                       g.d_cols.as<ColWork>(), ws.z64.as<double>(), g.col_sys(), ws.scal_chk.as<LsqrScalars>());
+    CAD_HIP(hipGetLastError());
+    const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal_chk, s);
+    std::vector<double> out(m->nsys);
+    for (int i = 0; i < m->nsys; i++) out[i] = std::sqrt(h[i].reduce_b);
+    return out;
+}
+
+// ------------------------------------------------------------ LSQR --------
+template <class VT>
+static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
+{
+    const GridPlan& g = m->grid;
+    LsqrScalars* sc = ws.scal.as<LsqrScalars>();
+    with_view(m, [&](auto view) {
+        hipLaunchKernelGGL((k_lsqr_u<VT, decltype(view)>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s, view,
+                           m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), ws.vd.as<VT>(), ws.u.as<VT>(),
+                           g.row_sys(), sc);
+        launch_scatter<VT, VT>(m->splan, view, m->d_sys_col.as<uint32_t>(), ws.u.as<VT>(), sc, ws.z.as<VT>(), s);
+    });
+    hipLaunchKernelGGL(k_lsqr_v<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
+                       ws.D.as<double>(), ws.z.as<VT>(), ws.v.as<double>(), g.col_sys(), sc);
+    hipLaunchKernelGGL(k_lsqr_xw<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
+                       ws.D.as<double>(), ws.v.as<double>(), ws.vd.as<VT>(), ws.w.as<double>(), ws.y.as<double>(),
+                       (const LsqrScalars*)sc);
+    CAD_HIP(hipGetLastError());
+}
+
+// Runs n LSQR iterations, replaying a captured graph of `batch` iterations
+// where possible.
+template <class VT>
+static void lsqr_run(cad_matrix* m, Workspace& ws, int n, int batch, bool graph, hipStream_t s)
+{
+    if (graph && batch > 1 && s != nullptr) {
+        if (!ws.exec || ws.graph_iters != batch || ws.graph_prec != int(sizeof(VT))) {
+            if (ws.exec) (void)hipGraphExecDestroy(ws.exec);
+            if (ws.graph) (void)hipGraphDestroy(ws.graph);
+            ws.exec = nullptr;
+            ws.graph = nullptr;
+            CAD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < batch; i++) lsqr_iteration<VT>(m, ws, s);
+            CAD_HIP(hipStreamEndCapture(s, &ws.graph));
+            CAD_HIP(hipGraphInstantiate(&ws.exec, ws.graph, nullptr, nullptr, 0));
+            ws.graph_iters = batch;
+            ws.graph_prec = int(sizeof(VT));
+        }
+        while (n >= batch) {
+            CAD_HIP(hipGraphLaunch(ws.exec, s));
+            n -= batch;
+        }
+    }
+    for (int i = 0; i < n; i++) lsqr_iteration<VT>(m, ws, s);
+}
+
+// LSQR start from the current FP64 residual r = b - A x (x64): u = r / |r|,
+// v = D A^T r / |D A^T r|, all of A^T r computed in FP64.  Returns
+// |A^T r| per system via reduce_b.
+template <class VT>
+static void lsqr_start_from_residual(cad_matrix* m, Workspace& ws, const double* b, hipStream_t s)
+{
+    const GridPlan& g = m->grid;
+    LsqrScalars* sc = ws.scal.as<LsqrScalars>();
+    if (!ws.z64_clear) {
+        fill<double>(ws.z64.as<double>(), m->cols, 0.0, s);
+        ws.z64_clear = true;
+    }
+    with_view(m, [&](auto view) {
+        hipLaunchKernelGGL((k_residual<VT, decltype(view)>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s, view,
+                           m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), ws.x64.as<double>(), b,
+                           ws.r64.as<double>(), ws.u.as<VT>(), g.row_sys(), sc);
+        launch_scatter<double, double>(m->splan, view, m->d_sys_col.as<uint32_t>(), ws.r64.as<double>(), nullptr,
+                                       ws.z64.as<double>(), s);
+    });
+    hipLaunchKernelGGL(k_lsqr_init_v<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
+                       g.d_cols.as<ColWork>(), ws.D.as<double>(), ws.z64.as<double>(), ws.v.as<double>(), g.col_sys(),
+                       sc);
+    hipLaunchKernelGGL(k_lsqr_init_w<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
+                       g.d_cols.as<ColWork>(), ws.D.as<double>(), ws.v.as<double>(), ws.vd.as<VT>(),
+                       ws.w.as<double>(), ws.y.as<double>(), (const LsqrScalars*)sc);
+    CAD_HIP(hipGetLastError());
+}
+
+template <class VT>
+static void lsqr_alloc(cad_matrix* m, Workspace& ws, hipStream_t s)
+{
+    ensure(ws.u, sizeof(VT) * m->rows);
+    ensure(ws.r64, sizeof(double) * m->rows);
+    ensure(ws.v, sizeof(double) * m->cols);
+    ensure(ws.w, sizeof(double) * m->cols);
+    ensure(ws.y, sizeof(double) * m->cols);
+    ensure(ws.x64, sizeof(double) * m->cols);
+    ensure(ws.D, sizeof(double) * m->cols);
+    ensure(ws.z64, sizeof(double) * m->cols);
+    ensure(ws.scal, sizeof(LsqrScalars) * m->nsys);
+    if (ws.vd.bytes < sizeof(VT) * m->cols || ws.vd_prec != int(sizeof(VT))) {
+        ws.vd = DBuf(sizeof(VT) * m->cols);
+        ws.z = DBuf(sizeof(VT) * m->cols);
+        ws.vd_prec = int(sizeof(VT));
+        fill<VT>(ws.z.as<VT>(), m->cols, VT(0), s);
+    }
+}
+
+// LSQR in one precision (FP64 or FP32 products and vectors) with column
+// scaling; stops on the explicit FP64 metric.
+template <class VT>
+static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep)
+{
+    hipStream_t s = m->ctx->stream;
+    Workspace& ws = workspace(m, 100 + int(sizeof(VT)));
+    lsqr_alloc<VT>(m, ws, s);
+    column_scaling(m, ws.D.as<double>(), o.precondition != 0, s);
+    const std::vector<double> atb = atb_norms(m, ws, b, s);
+    std::vector<int> act(m->nsys, 1);
+    set_active(m, ws, ws.scal, act, s);
+    fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
+    // x = 0: the residual start is u = b, exactly Math/Solver.h:146-150.
+    lsqr_start_from_residual<VT>(m, ws, b, s);
+
+    const int batch = std::max(1, o.check_every);
+    std::vector<double> trigger(m->nsys, 1.0), nres, rn;
+    int it = 0;
+    bool done = (o.max_iterations <= 0);
+    while (!done) {
+        const int n = std::min(batch, o.max_iterations - it);
+        lsqr_run<VT>(m, ws, n, batch, o.use_graph != 0 && n == batch, s);
+        it += n;
+        const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+        bool want_check = false;
+        for (int i = 0; i < m->nsys; i++) {
+            if (!h[i].active) continue;
+            const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
+            if (!std::isfinite(est)) throw Error(CAD_ERR_BREAKDOWN, "LSQR breakdown (non-finite recurrence)");
+            if (o.tol > 0 && est <= o.tol * trigger[i]) want_check = true;
+        }
+        if (o.verbose && m->nsys == 1)
+            std::fprintf(stderr, "[lsqr %s] it %d est %.3e\n", sizeof(VT) == 4 ? "fp32" : "fp64", it,
+                         h[0].nres0 > 0 ? h[0].nres / h[0].nres0 : 0.0);
+        if (want_check || it >= o.max_iterations) {
+            hipLaunchKernelGGL(k_scale_mul, dim3(grid_for(m->cols)), dim3(256), 0, s, ws.D.as<double>(),
+                               ws.y.as<double>(), m->cols, ws.x64.as<double>());
+            CAD_HIP(hipGetLastError());
+            explicit_residual(m, ws, ws.x64.as<double>(), b, nres, rn, s);
+            rep.explicit_checks++;
+            bool all = true;
+            std::vector<int> still(m->nsys, 0);
+            h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+            for (int i = 0; i < m->nsys; i++) {
+                const double rel = atb[i] > 0 ? nres[i] / atb[i] : nres[i];
+                if (o.tol > 0 && rel <= o.tol) continue;
+                if (h[i].active) {
+                    still[i] = 1;
+                    all = false;
+                    const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
+                    if (o.tol > 0 && est <= o.tol * trigger[i]) trigger[i] *= std::min(0.5, o.tol / std::max(rel, 1e-300));
+                }
+            }
+            if (all || it >= o.max_iterations) done = true;
+            else {
+                // freeze converged systems
+                for (int i = 0; i < m->nsys; i++)
+                    if (!still[i] && h[i].active) {
+                        LsqrScalars t = h[i];
+                        t.active = 0;
+                        CAD_HIP(hipMemcpyAsync(ws.scal.as<LsqrScalars>() + i, &t, sizeof(t), hipMemcpyHostToDevice, s));
+                    }
+                CAD_HIP(hipStreamSynchronize(s));
+            }
+        }
+    }
+    CAD_HIP(hipMemcpyAsync(x, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+    rep.iterations = it;
+    double worst = 0, rsum = 0;
+    for (int i = 0; i < m->nsys; i++) {
+        worst = std::max(worst, atb[i] > 0 ? nres[i] / atb[i] : nres[i]);
+        rsum += rn[i] * rn[i];
+    }
+    rep.rel_normal_residual = worst;
+    rep.residual_norm = std::sqrt(rsum);
+    rep.converged = o.tol > 0 && worst <= o.tol;
+}
+
+// Mixed precision with iterative refinement: FP32 products inside LSQR,
+// FP64 residual r = b - A x and A^T r outside.
+static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep)
+{
+    hipStream_t s = m->ctx->stream;
+    Workspace& ws = workspace(m, 200);
+    lsqr_alloc<float>(m, ws, s);
+    column_scaling(m, ws.D.as<double>(), o.precondition != 0, s);
+    const std::vector<double> atb = atb_norms(m, ws, b, s);
+    fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
+    std::vector<int> act(m->nsys, 1);
+    const int batch = std::max(1, o.check_every);
+    int it = 0;
+    double worst = 0, rsum = 0;
+    for (int ref = 0;; ref++) {
+        set_active(m, ws, ws.scal, act, s);
+        lsqr_start_from_residual<float>(m, ws, b, s);
+        const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+        worst = 0;
+        rsum = 0;
+        bool any = false;
+        for (int i = 0; i < m->nsys; i++) {
+            if (!act[i]) continue;
+            const double rel = atb[i] > 0 ? std::sqrt(h[i].reduce_b) / atb[i] : std::sqrt(h[i].reduce_b);
+            worst = std::max(worst, rel);
+            rsum += h[i].reduce_a;
+            if (o.tol > 0 && rel <= o.tol) act[i] = 0;
+            else any = true;
+        }
+        rep.explicit_checks++;
+        if (o.verbose && m->nsys == 1)
+            std::fprintf(stderr, "[mixed] refinement %d it %d rel %.3e\n", ref, it, worst);
+        if (!any || it >= o.max_iterations || ref >= o.max_refinements) {
+            rep.refinements = ref;
+            break;
+        }
+        // freeze the converged systems for the inner solve
+        {
+            std::vector<LsqrScalars> hv(h, h + m->nsys);
+            for (int i = 0; i < m->nsys; i++) hv[i].active = act[i];
+            CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * m->nsys, hipMemcpyHostToDevice, s));
+        }
+        // inner FP32 LSQR until the estimated normal residual dropped by inner_tol
+        for (;;) {
+            const int n = std::min(batch, o.max_iterations - it);
+            if (n <= 0) break;
+            lsqr_run<float>(m, ws, n, batch, o.use_graph != 0 && n == batch, s);
+            it += n;
+            h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+            bool inner_done = true;
+            for (int i = 0; i < m->nsys; i++) {
+                if (!h[i].active) continue;
+                const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
+                if (!std::isfinite(est)) throw Error(CAD_ERR_BREAKDOWN, "LSQR breakdown (non-finite recurrence)");
+                if (est > o.inner_tol) inner_done = false;
+            }
+            if (inner_done) break;
+        }
+        hipLaunchKernelGGL(k_lsqr_apply, dim3(unsigned(m->grid.cols.size())), dim3(kRowBlock), 0, s,
+                           m->grid.d_cols.as<ColWork>(), ws.D.as<double>(), ws.y.as<double>(), ws.x64.as<double>(),
+                           (const LsqrScalars*)ws.scal.as<LsqrScalars>());
+        CAD_HIP(hipGetLastError());
+    }
+    CAD_HIP(hipMemcpyAsync(x, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+    rep.iterations = it;
+    rep.rel_normal_residual = worst;
+    rep.residual_norm = std::sqrt(rsum);
+    rep.converged = o.tol > 0 && worst <= o.tol;
+}
+
+// ------------------------------------------------------------ PCG ---------
+template <class VT>
+static void pcg_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
+{
+    const GridPlan& g = m->grid;
+    PcgScalars* sc = ws.pscal.as<PcgScalars>();
+    spmv<VT>(m, ws.p.as<VT>(), ws.t.as<VT>(), s);
+    scatter<VT, VT>(m, ws.t.as<VT>(), nullptr, ws.q.as<VT>(), s);
+    hipLaunchKernelGGL(k_pcg_alpha<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
+                       ws.p.as<VT>(), ws.q.as<VT>(), g.col_sys(), sc);
+    hipLaunchKernelGGL(k_pcg_update<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
+                       g.d_cols.as<ColWork>(), ws.P.as<VT>(), ws.p.as<VT>(), ws.q.as<VT>(), ws.xr.as<VT>(),
+                       ws.rr.as<VT>(), ws.zz.as<VT>(), g.col_sys(), sc);
+    hipLaunchKernelGGL(k_pcg_p<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
+                       ws.zz.as<VT>(), ws.p.as<VT>(), ws.q.as<VT>(), (const PcgScalars*)sc);
+    CAD_HIP(hipGetLastError());
+}
+
+template <class VT>
+static void pcg_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
+                      bool jacobi)
+{
+    hipStream_t s = m->ctx->stream;
+    Workspace& ws = workspace(m, 300 + int(sizeof(VT)) + (jacobi ? 0 : 50));
+    const int64_t C = m->cols, R = m->rows;
+    ensure(ws.P, sizeof(VT) * C);
+    ensure(ws.p, sizeof(VT) * C);
+    ensure(ws.q, sizeof(VT) * C);
+    ensure(ws.zz, sizeof(VT) * C);
+    ensure(ws.xr, sizeof(VT) * C);
+    ensure(ws.rr, sizeof(VT) * C);
+    ensure(ws.t, sizeof(VT) * R);
+    ensure(ws.bvt, sizeof(VT) * R);
+    ensure(ws.pscal, sizeof(PcgScalars) * m->nsys);
+    ensure(ws.jac, sizeof(float) * C);
+    const GridPlan& g = m->grid;
+    if (jacobi) {
+        matrix_jacobi(m, ws.jac.as<float>());
+        hipLaunchKernelGGL(k_precond_from_jacobi<VT>, dim3(grid_for(C)), dim3(256), 0, s, ws.jac.as<float>(), C,
+                           ws.P.as<VT>());
+    } else {
+        fill<VT>(ws.P.as<VT>(), C, VT(1), s);
+    }
+    {
+        std::vector<PcgScalars> h(m->nsys);
+        std::memset(h.data(), 0, sizeof(PcgScalars) * m->nsys);
+        for (auto& e : h) e.active = 1;
+        CAD_HIP(hipMemcpyAsync(ws.pscal.p, h.data(), sizeof(PcgScalars) * m->nsys, hipMemcpyHostToDevice, s));
+    }
+    hipLaunchKernelGGL((k_convert<VT, double>), dim3(grid_for(R)), dim3(256), 0, s, ws.bvt.as<VT>(), b, R);
+    // rhs = A^T b into rr (x0 = 0 so r = rhs)
+    fill<VT>(ws.rr.as<VT>(), C, VT(0), s);
+    scatter<VT, VT>(m, ws.bvt.as<VT>(), nullptr, ws.rr.as<VT>(), s);
+    hipLaunchKernelGGL(k_pcg_init<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
+                       ws.P.as<VT>(), ws.rr.as<VT>(), ws.zz.as<VT>(), ws.p.as<VT>(), ws.xr.as<VT>(), ws.q.as<VT>(),
+                       g.col_sys(), ws.pscal.as<PcgScalars>());
+    CAD_HIP(hipGetLastError());
+    const int batch = std::max(1, o.check_every);
+    int it = 0;
+    while (it < o.max_iterations) {
+        const int n = std::min(batch, o.max_iterations - it);
+        for (int i = 0; i < n; i++) pcg_iteration<VT>(m, ws, s);
+        it += n;
+        const PcgScalars* h = read_scalars<PcgScalars>(m, ws, ws.pscal, s);
+        for (int i = 0; i < m->nsys; i++)
+            if (!std::isfinite(h[i].rz) || !std::isfinite(h[i].alpha))
+                throw Error(CAD_ERR_BREAKDOWN, "PCG breakdown (non-finite recurrence)");
+        if (o.verbose && m->nsys == 1) std::fprintf(stderr, "[pcg] it %d r.r %.6e\n", it, h[0].rr);
+    }
+    hipLaunchKernelGGL((k_convert<double, VT>), dim3(grid_for(C)), dim3(256), 0, s, x, ws.xr.as<VT>(), C);
+    CAD_HIP(hipGetLastError());
+    std::vector<double> atb = atb_norms(m, ws, b, s), nres, rn;
+    explicit_residual(m, ws, x, b, nres, rn, s);
+    double worst = 0, rsum = 0;
+    for (int i = 0; i < m->nsys; i++) {
+        worst = std::max(worst, atb[i] > 0 ? nres[i] / atb[i] : nres[i]);
+        rsum += rn[i] * rn[i];
+    }
+    rep.iterations = it;
+    rep.rel_normal_residual = worst;
+    rep.residual_norm = std::sqrt(rsum);
+    rep.converged = o.tol > 0 && worst <= o.tol;
+}
+
+// ------------------------------------------------------------ entry -------
+void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep)
+{
+    std::memset(&rep, 0, sizeof(rep));
+    auto t0 = std::chrono::steady_clock::now();
+    if (o.solver == CAD_SOLVER_LSQR) {
+        if (o.precision == CAD_PREC_MIXED) lsqr_mixed(m, b, x, o, rep);
+        else if (o.precision == CAD_PREC_FP32) lsqr_plain<float>(m, b, x, o, rep);
+        else lsqr_plain<double>(m, b, x, o, rep);
+    } else if (o.solver == CAD_SOLVER_PCG || o.solver == CAD_SOLVER_CG) {
+        const bool jac = o.solver == CAD_SOLVER_PCG;
+        if (o.precision == CAD_PREC_FP32) pcg_solve<float>(m, b, x, o, rep, jac);
+        else pcg_solve<double>(m, b, x, o, rep, jac);
+    } else if (o.solver == CAD_SOLVER_LSMR) {
+        lsmr_solve(m, b, x, o, rep);
+    } else {
+        throw Error(CAD_ERR_ARGUMENT, "unknown solver");
+    }
+    CAD_HIP(hipStreamSynchronize(m->ctx->stream));
+    rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+double rel_normal_residual(cad_matrix* m, const double* b, const double* x, double* rnorm)
+{
+    hipStream_t s = m->ctx->stream;
+    Workspace& ws = workspace(m, 1);
+    std::vector<double> atb = atb_norms(m, ws, b, s), nres, rn;
+    explicit_residual(m, ws, x, b, nres, rn, s);
+    double worst = 0, rsum = 0;
+    for (int i = 0; i < m->nsys; i++) {
+        worst = std::max(worst, atb[i] > 0 ? nres[i] / atb[i] : nres[i]);
+        rsum += rn[i] * rn[i];
+    }
+    if (rnorm) *rnorm = std::sqrt(rsum);
+    return worst;
+}
+
+}  // namespace cad
+
+cad_matrix::~cad_matrix() = default;

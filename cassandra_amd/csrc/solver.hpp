@@ -1,0 +1,33 @@
+// Device-resident solvers and products on a cad_matrix.
+#pragma once
+#include "../../include/cassandra_amd.h"
+#include "runtime.hpp"
+
+namespace cad {
+
+// Solver buffers cached on a matrix (one set per solver / precision), with
+// the captured iteration graph.
+struct Workspace {
+    int key = 0;
+    DBuf u, r64, v, w, y, x64, D, z64, vd, z, scal, scal_chk;
+    DBuf P, p, q, zz, xr, rr, t, bvt, pscal, jac;
+    int vd_prec = 0;
+    bool z64_clear = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_iters = 0, graph_prec = 0;
+    void* host_scal = nullptr;
+    ~Workspace();
+};
+
+void matrix_spmv_f32(cad_matrix* m, const float* x, float* y);
+void matrix_spmv_f64(cad_matrix* m, const double* x, double* y);
+void matrix_spmtv_f32(cad_matrix* m, const float* y, float* z);
+void matrix_spmtv_f64(cad_matrix* m, const double* y, double* z);
+void matrix_jacobi(cad_matrix* m, float* d);
+const uint32_t* jacobi_counts(cad_matrix* m);
+void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep);
+void lsmr_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep);
+double rel_normal_residual(cad_matrix* m, const double* b, const double* x, double* rnorm);
+
+}  // namespace cad

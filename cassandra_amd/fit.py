@@ -1,0 +1,232 @@
+"""Python mirror of the reference's fit API over the C-ABI, with torch tensors
+as device buffers.
+
+    Context            device + stream                    (no reference counterpart)
+    Patterns           GroupIndexer / CreateIndexer       (Pattern/Indexer.h:10-40)
+    Matrix             MatrixCSR, A*x, x*A, Jacobi        (Math/MatrixCSR.h:11-33)
+    Matrix.solve       PCG / LSQR / LSMR / CG             (Math/Solver.h:9-287)
+    fit_weights        FitWeights                         (PatternFit/PatternFit.cpp:7-39)
+
+Device tensors are CUDA (HIP) torch tensors; positions are int64 tensors of
+shape [n, 2] holding the bits of {P, O}.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import capi
+from .capi import call
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev(t, dtype):
+    if not (t.is_cuda and t.dtype == dtype and t.is_contiguous()):
+        raise ValueError(f"expected a contiguous {dtype} device tensor, got {t.dtype} on {t.device}")
+    return t
+
+
+class Context:
+    """A device and the stream the library enqueues on (its own by default)."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        h = C.c_void_p()
+        call("cad_context_create", device, C.byref(h))
+        self.h = h
+        self.torch_device = torch.device("cuda", device)
+
+    def close(self):
+        if getattr(self, "h", None):
+            capi.lib().cad_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream_handle(self) -> int:
+        s = C.c_void_p()
+        call("cad_context_get_stream", self.h, C.byref(s))
+        return s.value or 0
+
+    def torch_stream(self):
+        """The library stream as a torch stream (for events / ordering)."""
+        return torch.cuda.ExternalStream(self.stream_handle(), device=self.torch_device)
+
+    def synchronize(self):
+        call("cad_context_synchronize", self.h)
+
+    def random_play(self, seed: int, n: int, e_lo: int, e_hi: int | None = None) -> torch.Tensor:
+        """n positions of uniformly random legal play (int64 [n, 2])."""
+        out = torch.empty((n, 2), dtype=torch.int64, device=self.torch_device)
+        torch.cuda.synchronize(self.torch_device)
+        call("cad_random_play", self.h, seed, n, e_lo, e_lo if e_hi is None else e_hi, _ptr(out))
+        self.synchronize()
+        return out
+
+
+class Patterns:
+    """GroupIndexer over a list of pattern masks."""
+
+    def __init__(self, ctx: Context, masks):
+        self.ctx = ctx
+        arr = np.ascontiguousarray(np.array(masks, dtype=np.uint64))
+        h = C.c_void_p()
+        call("cad_patterns_create", ctx.h, arr.ctypes.data_as(C.c_void_p), len(arr), C.byref(h))
+        self.h = h
+        k, cols, npat = C.c_int(), C.c_int64(), C.c_int()
+        call("cad_patterns_info", h, C.byref(k), C.byref(cols), C.byref(npat))
+        self.k, self.cols, self.npatterns = k.value, cols.value, npat.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                capi.lib().cad_patterns_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def pattern_info(self, i):
+        cls, mask, size, order, off = C.c_int(), C.c_uint64(), C.c_int64(), C.c_int(), C.c_int64()
+        call("cad_patterns_pattern_info", self.h, i, C.byref(cls), C.byref(mask), C.byref(size), C.byref(order),
+             C.byref(off))
+        return dict(cls=cls.value, mask=mask.value, size=size.value, order=order.value, offset=off.value)
+
+    def variations(self):
+        out = np.zeros(self.k, dtype=np.uint64)
+        call("cad_patterns_variations", self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def indices_host(self, PO) -> np.ndarray:
+        PO = np.ascontiguousarray(np.asarray(PO, dtype=np.uint64).reshape(-1, 2))
+        out = np.zeros((PO.shape[0], self.k), dtype=np.uint32)
+        call("cad_patterns_indices_host", self.h, PO.ctypes.data_as(C.c_void_p), PO.shape[0],
+             out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def extract(self, pos: torch.Tensor, layout: int = capi.LAYOUT_ROW_MAJOR) -> torch.Tensor:
+        """Column indices of every row on the device (int32; [n, k] row-major
+        or [k, n] slot-major)."""
+        _dev(pos, torch.int64)
+        n = pos.shape[0]
+        shape = (n, self.k) if layout == capi.LAYOUT_ROW_MAJOR else (self.k, n)
+        out = torch.empty(shape, dtype=torch.int32, device=pos.device)
+        torch.cuda.synchronize(pos.device)
+        call("cad_index_extract", self.h, _ptr(pos), n, _ptr(out), layout)
+        self.ctx.synchronize()
+        return out
+
+    def matrix(self, pos: torch.Tensor) -> "Matrix":
+        _dev(pos, torch.int64)
+        torch.cuda.synchronize(pos.device)
+        h = C.c_void_p()
+        call("cad_matrix_from_positions", self.h, _ptr(pos), pos.shape[0], C.byref(h))
+        return Matrix(self.ctx, h)
+
+
+class Matrix:
+    """Device MatrixCSR (fixed k per row, implicit value 1, repeats allowed)."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self.h = h
+        rows, k, cols, ib = C.c_int64(), C.c_int(), C.c_int64(), C.c_int()
+        call("cad_matrix_info", h, C.byref(rows), C.byref(k), C.byref(cols), C.byref(ib))
+        self.rows, self.k, self.cols, self.index_bytes = rows.value, k.value, cols.value, ib.value
+
+    @classmethod
+    def from_host(cls, ctx: Context, idx: np.ndarray, cols: int) -> "Matrix":
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        h = C.c_void_p()
+        call("cad_matrix_create", ctx.h, idx.ctypes.data_as(C.c_void_p), idx.shape[0], idx.shape[1], cols,
+             C.byref(h))
+        return cls(ctx, h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                capi.lib().cad_matrix_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def download(self) -> np.ndarray:
+        out = np.zeros((self.rows, self.k), dtype=np.uint32)
+        call("cad_matrix_download", self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def _vec(self, n, dtype):
+        return torch.empty(n, dtype=dtype, device=self.ctx.torch_device)
+
+    def spmv(self, x: torch.Tensor) -> torch.Tensor:
+        """A x (Math/MatrixCSR.cpp:49-65)."""
+        if x.shape[0] != self.cols:
+            raise ValueError("Size mismatch.")
+        y = self._vec(self.rows, x.dtype)
+        torch.cuda.synchronize(x.device)
+        name = "cad_spmv_f32" if x.dtype == torch.float32 else "cad_spmv_f64"
+        call(name, self.h, _ptr(_dev(x, x.dtype)), _ptr(y))
+        self.ctx.synchronize()
+        return y
+
+    def spmtv(self, y: torch.Tensor) -> torch.Tensor:
+        """y A = A^T y (Math/MatrixCSR.cpp:67-87)."""
+        if y.shape[0] != self.rows:
+            raise ValueError("Size mismatch.")
+        z = self._vec(self.cols, y.dtype)
+        torch.cuda.synchronize(y.device)
+        name = "cad_spmtv_f32" if y.dtype == torch.float32 else "cad_spmtv_f64"
+        call(name, self.h, _ptr(_dev(y, y.dtype)), _ptr(z))
+        self.ctx.synchronize()
+        return z
+
+    def jacobi(self) -> torch.Tensor:
+        """JacobiPreconditionerOfATA (Math/MatrixCSR.cpp:21-47)."""
+        d = self._vec(self.cols, torch.float32)
+        call("cad_jacobi", self.h, _ptr(d))
+        self.ctx.synchronize()
+        return d
+
+    def make_targets(self, seed_x: int, seed_n: int, sx: float = 1.5, sn: float = 4.0):
+        xt = self._vec(self.cols, torch.float64)
+        b = self._vec(self.rows, torch.float64)
+        call("cad_make_targets", self.h, seed_x, seed_n, sx, sn, _ptr(xt), _ptr(b))
+        self.ctx.synchronize()
+        return xt, b
+
+    def solve(self, b: torch.Tensor, **options):
+        """Solve from x0 = 0; returns (x [cols] float64, report dict)."""
+        _dev(b, torch.float64)
+        if b.shape[0] != self.rows:
+            raise ValueError("Size mismatch.")
+        x = self._vec(self.cols, torch.float64)
+        o = capi.default_options(**options)
+        rep = capi.SolveReport()
+        torch.cuda.synchronize(b.device)
+        call("cad_solve", self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(rep))
+        return x, rep.as_dict()
+
+    def rel_normal_residual(self, b: torch.Tensor, x: torch.Tensor):
+        out, rn = C.c_double(), C.c_double()
+        torch.cuda.synchronize(b.device)
+        call("cad_rel_normal_residual", self.h, _ptr(_dev(b, torch.float64)), _ptr(_dev(x, torch.float64)),
+             C.byref(out), C.byref(rn))
+        return out.value, rn.value
+
+
+def fit_weights(patterns: Patterns, pos: torch.Tensor, scores: torch.Tensor, iterations: int,
+                precision: int = capi.PREC_FP32):
+    """FitWeights (PatternFit/PatternFit.cpp:7-39): Jacobi-PCG on A^T A from
+    x0 = 0 for `iterations` iterations; returns the weights (float64 tensor)."""
+    A = patterns.matrix(pos)
+    x, rep = A.solve(scores.to(torch.float64).contiguous(), solver=capi.SOLVER_PCG, precision=precision,
+                     max_iterations=iterations, tol=0.0, check_every=max(1, iterations))
+    return x, rep

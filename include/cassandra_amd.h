@@ -1,0 +1,189 @@
+/*
+ * cassandra_amd: C-ABI of the MI355X (gfx950) pattern-evaluator fit.
+ *
+ * The reference (dominichofer/Cassandra) has no FFI on this path; its boundary
+ * is its own C++ API (Math/MatrixCSR.h, Math/Solver.h, Pattern/Indexer.h,
+ * Pattern/Evaluator.h, PatternFit/PatternFit.cpp).  Each entry point below
+ * names the reference interface it replaces.  The C++ drop-in layer
+ * (cassandra_amd/cpp) and the Python torch.autograd.Function both bind here.
+ *
+ * Conventions
+ *  - Every function returns CAD_OK (0) or a negative error code; the message of
+ *    the last failure on the calling thread is cad_last_error().  No C++
+ *    exception crosses this boundary.
+ *  - Pointers named d_* are device pointers (caller-owned, on the context's
+ *    device); h_* are host pointers.  Handles own their device memory.
+ *  - Work is enqueued on the context's stream (its own non-blocking stream by
+ *    default, or one set with cad_context_set_stream).  Functions that return
+ *    results to the host synchronise that stream.
+ *  - Positions are 16 bytes {uint64 P, uint64 O} (Board/Position.h:11-33):
+ *    P = side to move; bit b = square (x, y) with b = x + 8y.
+ */
+#ifndef CASSANDRA_AMD_H
+#define CASSANDRA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAD_OK 0
+#define CAD_ERR_ARGUMENT (-1)       /* bad argument (reference: std::runtime_error) */
+#define CAD_ERR_SIZE_MISMATCH (-2)  /* reference: std::runtime_error("Size mismatch.") */
+#define CAD_ERR_PATTERN (-3)        /* reference: indexer precondition failures */
+#define CAD_ERR_DEVICE (-4)         /* HIP runtime error */
+#define CAD_ERR_NOMEM (-5)          /* device allocation failed */
+#define CAD_ERR_BREAKDOWN (-6)      /* solver breakdown (NaN / zero curvature) */
+
+typedef struct cad_context cad_context;
+typedef struct cad_patterns cad_patterns;
+typedef struct cad_matrix cad_matrix;
+
+/* ---- errors, context ---------------------------------------------------- */
+const char* cad_last_error(void);
+const char* cad_version(void);
+/* Creates a context on `device` with its own default-priority stream. */
+int cad_context_create(int device, cad_context** out);
+int cad_context_destroy(cad_context* ctx);
+/* Enqueue subsequent work on `hip_stream` (a hipStream_t, e.g. torch's
+ * current stream); NULL restores the context's own stream. */
+int cad_context_set_stream(cad_context* ctx, void* hip_stream);
+/* The stream work is currently enqueued on (a hipStream_t). */
+int cad_context_get_stream(cad_context* ctx, void** out);
+int cad_context_synchronize(cad_context* ctx);
+/* Device memory helpers for callers without HIP headers (the C++ layer). */
+int cad_malloc(cad_context* ctx, size_t bytes, void** d_out);
+int cad_free(cad_context* ctx, void* d_ptr);
+int cad_memcpy_h2d(cad_context* ctx, void* d_dst, const void* h_src, size_t bytes);
+int cad_memcpy_d2h(cad_context* ctx, void* h_dst, const void* d_src, size_t bytes);
+int cad_memcpy_d2d(cad_context* ctx, void* d_dst, const void* d_src, size_t bytes);
+
+/* ---- patterns (Pattern/Indexer.h:10-40, Pattern/Indexer.cpp:319-422) ----
+ * cad_patterns_create = GroupIndexer(masks): CreateIndexer per mask, columns
+ * laid out pattern after pattern.  Fails with CAD_ERR_PATTERN where the
+ * reference indexer constructors throw.  ctx may be NULL: a host-only group
+ * (info, variations and cad_patterns_indices_host work; device calls fail). */
+int cad_patterns_create(cad_context* ctx, const uint64_t* h_masks, int n, cad_patterns** out);
+int cad_patterns_destroy(cad_patterns* p);
+/* k = Variations().size() (elements per row), cols = index_space_size */
+int cad_patterns_info(const cad_patterns* p, int* k, int64_t* cols, int* npatterns);
+/* per pattern: class (0 A, 1 V, 2 D, 3 VH, 4 DC, 5 VHDC), transformed mask
+ * (Indexer::pattern), index_space_size, symmetry_group_order, column offset */
+int cad_patterns_pattern_info(const cad_patterns* p, int i, int* cls, uint64_t* mask,
+                              int64_t* size, int* order, int64_t* offset);
+/* GroupIndexer::Variations (Pattern/Indexer.cpp:394-403): k masks */
+int cad_patterns_variations(const cad_patterns* p, uint64_t* h_out);
+/* GroupIndexer::Indices on the host for n positions (row-major n x k). */
+int cad_patterns_indices_host(const cad_patterns* p, const uint64_t* h_positions, int64_t n,
+                              uint32_t* h_out);
+
+/* ---- index extraction (GroupIndexer::InsertIndices over rows,
+ * PatternFit/PatternFit.cpp:19-21) --------------------------------------- */
+#define CAD_LAYOUT_ROW_MAJOR 0  /* out[r*k + s]: MatrixCSR::Row(r) layout, int/uint32 */
+#define CAD_LAYOUT_SLOT_MAJOR 1 /* out[s*n + r] */
+/* d_out receives uint32 column indices (int32 for torch: same bits). */
+int cad_index_extract(cad_patterns* p, const uint64_t* d_positions, int64_t n, uint32_t* d_out,
+                      int layout);
+
+/* ---- matrix (Math/MatrixCSR.h:11-33) ------------------------------------- */
+/* Builds the device matrix of GroupIndexer rows for n device positions. */
+int cad_matrix_from_positions(cad_patterns* p, const uint64_t* d_positions, int64_t n,
+                              cad_matrix** out);
+/* MatrixCSR(k, rows, cols) with Row(i) filled: row-major host indices. */
+int cad_matrix_create(cad_context* ctx, const uint32_t* h_idx, int64_t rows, int k, int64_t cols,
+                      cad_matrix** out);
+int cad_matrix_destroy(cad_matrix* m);
+/* index_bytes: 2 or 4 (device storage of one entry) */
+int cad_matrix_info(const cad_matrix* m, int64_t* rows, int* k, int64_t* cols, int* index_bytes);
+/* MatrixCSR::Row(i) for all rows, row-major into h_out[rows*k]. */
+int cad_matrix_download(const cad_matrix* m, uint32_t* h_out);
+/* operator*(const MatrixCSR&, const Vector&) (Math/MatrixCSR.cpp:49-65):
+ * FP32 y = A x summed sequentially in row order (bit-exact with the reference). */
+int cad_spmv_f32(cad_matrix* m, const float* d_x, float* d_y);
+int cad_spmv_f64(cad_matrix* m, const double* d_x, double* d_y);
+/* operator*(const Vector&, const MatrixCSR&) (Math/MatrixCSR.cpp:67-87):
+ * z = A^T y (scatter-add; order of FP additions not deterministic). */
+int cad_spmtv_f32(cad_matrix* m, const float* d_y, float* d_z);
+int cad_spmtv_f64(cad_matrix* m, const double* d_y, double* d_z);
+/* JacobiPreconditionerOfATA (Math/MatrixCSR.cpp:21-47): 1/diag(A^T A) in float
+ * (inf for empty columns). */
+int cad_jacobi(cad_matrix* m, float* d_out);
+
+/* ---- solvers (Math/Solver.h:9-287, PatternFit/PatternFit.cpp:7-39) -------- */
+#define CAD_SOLVER_LSQR 0
+#define CAD_SOLVER_PCG 1   /* PCG on A^T A with Jacobi (FitWeights) */
+#define CAD_SOLVER_CG 2    /* CG on A^T A */
+#define CAD_SOLVER_LSMR 3
+
+#define CAD_PREC_FP64 0    /* FP64 products and vectors */
+#define CAD_PREC_FP32 1    /* FP32 products and vectors (the reference's precision) */
+#define CAD_PREC_MIXED 2   /* FP32 products, FP64 scalars/dots/norms/x and FP64
+                              refinement residual r = b - A x, A^T r (LSQR only) */
+
+typedef struct cad_solve_options {
+    int solver;           /* CAD_SOLVER_* */
+    int precision;        /* CAD_PREC_* */
+    int precondition;     /* LSQR: column scaling by 1/|a_c|; PCG: Jacobi (always) */
+    int max_iterations;   /* total (inner) iterations */
+    double tol;           /* stop when |A^T(b-Ax)| / |A^T b| <= tol (0: run max_iterations) */
+    double inner_tol;     /* MIXED: inner LSQR reduction per refinement step */
+    int max_refinements;  /* MIXED */
+    int check_every;      /* iterations between host convergence checks (graph size) */
+    int use_graph;        /* capture check_every iterations in a hipGraph */
+    int verbose;
+} cad_solve_options;
+
+typedef struct cad_solve_report {
+    int iterations;             /* inner iterations run */
+    int refinements;            /* outer refinement steps (MIXED) */
+    int converged;
+    int explicit_checks;
+    double rel_normal_residual; /* |A^T(b-Ax)| / |A^T b|, FP64, at return */
+    double residual_norm;       /* |b - A x| */
+    double seconds;             /* wall time of the solve (device-synchronised) */
+} cad_solve_report;
+
+void cad_solve_options_default(cad_solve_options* o);
+/* Solves min |A x - b| (LSQR/LSMR) or A^T A x = A^T b (PCG/CG) from x0 = 0.
+ * d_b: rows doubles; d_x: cols doubles (output). */
+int cad_solve(cad_matrix* m, const double* d_b, double* d_x, const cad_solve_options* opt,
+              cad_solve_report* rep);
+/* |A^T(b - A x)| / |A^T b| in FP64 on the device (the stopping metric). */
+int cad_rel_normal_residual(cad_matrix* m, const double* d_b, const double* d_x, double* out,
+                            double* residual_norm);
+
+/* ---- differentiable evaluator (Pattern/Evaluator.cpp:48-54) --------------
+ * Forward: out[r] = sum_s w[idx[r*k+s]] (row-major int32 indices, as built by
+ * cad_index_extract with CAD_LAYOUT_ROW_MAJOR).  Backward: grad_w = A^T g,
+ * the scatter-add adjoint (grad_w is overwritten).  The patterns handle gives
+ * the column block of every slot. */
+int cad_eval_forward_f32(cad_patterns* p, const int32_t* d_idx, int64_t B, const float* d_w,
+                         float* d_out);
+int cad_eval_forward_f64(cad_patterns* p, const int32_t* d_idx, int64_t B, const double* d_w,
+                         double* d_out);
+int cad_eval_backward_f32(cad_patterns* p, const int32_t* d_idx, int64_t B, const float* d_g,
+                          float* d_grad_w);
+int cad_eval_backward_f64(cad_patterns* p, const int32_t* d_idx, int64_t B, const double* d_g,
+                          double* d_grad_w);
+
+/* ---- synthetic inputs (Game/GameGenerator.cpp:3-28, Game/Player.cpp:3-16) */
+/* n positions from uniformly random legal play (empties drawn in [e_lo,e_hi]) */
+int cad_random_play(cad_context* ctx, uint64_t seed, int64_t n, int e_lo, int e_hi,
+                    uint64_t* d_out);
+/* Host versions of the same board code (Board/BitBoard.cpp:4-84,
+ * Board/PossibleMoves.cpp:86-156, Board/Flips.cpp:119-182): sym 1 vertical,
+ * 2 horizontal, 3 diagonal, 4 codiagonal. */
+uint64_t cad_flip(int sym, uint64_t b);
+uint64_t cad_possible_moves(uint64_t P, uint64_t O);
+uint64_t cad_flips(uint64_t P, uint64_t O, int move);
+int cad_random_play_host(uint64_t seed, int64_t n, int e_lo, int e_hi, uint64_t* h_out);
+/* x_true ~ sx * N(0,1) (cols) and b = clip(2 round((A x_true + sn N(0,1))/2), +-64) */
+int cad_make_targets(cad_matrix* m, uint64_t seed_x, uint64_t seed_n, double sx, double sn,
+                     double* d_xtrue, double* d_b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CASSANDRA_AMD_H */
