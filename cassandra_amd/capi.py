@@ -58,6 +58,7 @@ SIGNATURES = {
     "cad_context_destroy": (C.c_int, [vp]),
     "cad_context_set_stream": (C.c_int, [vp, vp]),
     "cad_context_get_stream": (C.c_int, [vp, PP]),
+    "cad_context_reset_stream": (C.c_int, [vp]),
     "cad_context_synchronize": (C.c_int, [vp]),
     "cad_malloc": (C.c_int, [vp, C.c_size_t, PP]),
     "cad_free": (C.c_int, [vp, vp]),

@@ -112,13 +112,18 @@ cad_matrix* new_matrix(cad_context* ctx, int64_t rows, int k, int64_t cols)
     return m;
 }
 
-// The scatter plan of the evaluator backward for B rows of a pattern group.
-void eval_plan(cad_patterns* p, int64_t B)
+// The scatter plan of the evaluator backward for B rows of a pattern group
+// (cached per B; a training loop sees one or two minibatch sizes).
+const ScatterPlan& eval_plan(cad_patterns* p, int64_t B)
 {
-    if (p->eval_plan_rows == B) return;
-    build_scatter_plan(p->eval_plan, p->slot_lo, p->slot_hi, std::vector<int64_t>{0, B}, std::vector<uint32_t>{0u},
-                       *p->ctx);
-    p->eval_plan_rows = B;
+    auto it = p->eval_plans.find(B);
+    if (it != p->eval_plans.end()) return *it->second;
+    if (p->eval_plans.size() >= 16) p->eval_plans.clear();
+    std::unique_ptr<ScatterPlan> sp(new ScatterPlan);
+    build_scatter_plan(*sp, p->slot_lo, p->slot_hi, std::vector<int64_t>{0, B}, std::vector<uint32_t>{0u}, *p->ctx);
+    const ScatterPlan& ref = *sp;
+    p->eval_plans[B] = std::move(sp);
+    return ref;
 }
 
 template <class VT>
@@ -128,8 +133,7 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
     const int64_t C = int64_t(p->group->cols());
     hipLaunchKernelGGL(k_fill<VT>, dim3(grid1(C, 256)), dim3(256), 0, s, grad, C, VT(0));
     if (B <= 0) return;
-    eval_plan(p, B);
-    const ScatterPlan& sp = p->eval_plan;
+    const ScatterPlan& sp = eval_plan(p, B);
     const bool f = sizeof(VT) == 4;
     const size_t n = f ? sp.work4.size() : sp.work8.size();
     const size_t lds = size_t(f ? sp.win4 : sp.win8) * sizeof(VT);
@@ -200,7 +204,15 @@ int cad_context_set_stream(cad_context* ctx, void* stream)
 {
     return guarded([&] {
         need(ctx != nullptr, "null context");
-        ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own;
+        ctx->stream = static_cast<hipStream_t>(stream);
+    });
+}
+
+int cad_context_reset_stream(cad_context* ctx)
+{
+    return guarded([&] {
+        need(ctx != nullptr, "null context");
+        ctx->stream = ctx->own;
     });
 }
 

@@ -91,8 +91,8 @@ struct cad_patterns {
     std::unique_ptr<cad::PatternGroup> group;
     cad::DBuf d_slots, d_sq, d_dense;
     std::vector<uint32_t> slot_lo, slot_hi;  // global column range of every slot
-    cad::ScatterPlan eval_plan;              // for evaluator backward, built per B
-    int64_t eval_plan_rows = -1;
+    // scatter plans of the evaluator backward, one per minibatch size
+    std::map<int64_t, std::unique_ptr<cad::ScatterPlan>> eval_plans;
     cad::DBuf d_base0, d_syscol0;            // zeros
 };
 

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "kernels.hpp"
 
@@ -35,11 +37,21 @@ void fill(T* p, int64_t n, T v, hipStream_t s)
     CAD_HIP(hipGetLastError());
 }
 
+// Raises the dynamic-LDS limit of a kernel (once per kernel and size; the
+// solvers call prepare_scatter before they capture a graph, so this never
+// runs inside a capture).
 template <class K>
 void allow_lds(K kernel, size_t bytes)
 {
-    if (bytes > 64 * 1024) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+    static std::mutex mu;
+    static std::map<const void*, size_t> done;
+    const void* key = reinterpret_cast<const void*>(kernel);
+    std::lock_guard<std::mutex> lock(mu);
+    if (bytes > 64 * 1024 && bytes > done[key]) {
+        (void)hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+        (void)hipGetLastError();
+        done[key] = bytes;
+    }
 }
 
 template <class VT, class YT, class View>
@@ -56,6 +68,14 @@ void launch_scatter(const ScatterPlan& sp, const View& view, const uint32_t* sys
     hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kScatterBlock), lds, s, view, wl,
                        sp.d_groups.as<SlotGroup>(), sp.d_slot_list.as<uint16_t>(), sys_col, y, div, z);
     CAD_HIP(hipGetLastError());
+}
+
+template <class VT, class YT>
+void prepare_scatter(cad_matrix* m)
+{
+    const ScatterPlan& sp = m->splan;
+    const size_t lds = size_t(sizeof(VT) == 4 ? sp.win4 : sp.win8) * sizeof(VT);
+    with_view(m, [&](auto view) { allow_lds(k_scatter<VT, YT, decltype(view)>, lds); });
 }
 
 template <class VT, class YT>
@@ -263,6 +283,7 @@ static void lsqr_run(cad_matrix* m, Workspace& ws, int n, int batch, bool graph,
             if (ws.graph) (void)hipGraphDestroy(ws.graph);
             ws.exec = nullptr;
             ws.graph = nullptr;
+            prepare_scatter<VT, VT>(m);
             CAD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < batch; i++) lsqr_iteration<VT>(m, ws, s);
             CAD_HIP(hipStreamEndCapture(s, &ws.graph));
@@ -393,6 +414,10 @@ static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solv
             }
         }
     }
+    if (nres.empty()) {  // no iteration ran: x = 0
+        fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
+        explicit_residual(m, ws, ws.x64.as<double>(), b, nres, rn, s);
+    }
     CAD_HIP(hipMemcpyAsync(x, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
     rep.iterations = it;
     double worst = 0, rsum = 0;
@@ -416,27 +441,31 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
     const std::vector<double> atb = atb_norms(m, ws, b, s);
     fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
     std::vector<int> act(m->nsys, 1);
+    std::vector<double> rel(m->nsys, 1.0), rn2(m->nsys, 0.0), target(m->nsys, o.inner_tol);
     const int batch = std::max(1, o.check_every);
     int it = 0;
-    double worst = 0, rsum = 0;
     for (int ref = 0;; ref++) {
         set_active(m, ws, ws.scal, act, s);
         lsqr_start_from_residual<float>(m, ws, b, s);
         const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
-        worst = 0;
-        rsum = 0;
         bool any = false;
         for (int i = 0; i < m->nsys; i++) {
             if (!act[i]) continue;
-            const double rel = atb[i] > 0 ? std::sqrt(h[i].reduce_b) / atb[i] : std::sqrt(h[i].reduce_b);
-            worst = std::max(worst, rel);
-            rsum += h[i].reduce_a;
-            if (o.tol > 0 && rel <= o.tol) act[i] = 0;
-            else any = true;
+            rel[i] = atb[i] > 0 ? std::sqrt(h[i].reduce_b) / atb[i] : std::sqrt(h[i].reduce_b);
+            rn2[i] = h[i].reduce_a;
+            if (o.tol > 0 && rel[i] <= o.tol) {
+                act[i] = 0;
+            } else {
+                any = true;
+                // the last refinement only needs to close the remaining gap
+                // (with a factor 3 of margin); FP32 products cannot go much
+                // below inner_tol in one solve
+                target[i] = o.tol > 0 ? std::max(o.inner_tol, o.tol / (3.0 * rel[i])) : o.inner_tol;
+            }
         }
         rep.explicit_checks++;
         if (o.verbose && m->nsys == 1)
-            std::fprintf(stderr, "[mixed] refinement %d it %d rel %.3e\n", ref, it, worst);
+            std::fprintf(stderr, "[mixed] refinement %d it %d rel %.3e\n", ref, it, rel[0]);
         if (!any || it >= o.max_iterations || ref >= o.max_refinements) {
             rep.refinements = ref;
             break;
@@ -447,7 +476,7 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
             for (int i = 0; i < m->nsys; i++) hv[i].active = act[i];
             CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * m->nsys, hipMemcpyHostToDevice, s));
         }
-        // inner FP32 LSQR until the estimated normal residual dropped by inner_tol
+        // inner FP32 LSQR until the estimated normal residual dropped by the target
         for (;;) {
             const int n = std::min(batch, o.max_iterations - it);
             if (n <= 0) break;
@@ -459,7 +488,7 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
                 if (!h[i].active) continue;
                 const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
                 if (!std::isfinite(est)) throw Error(CAD_ERR_BREAKDOWN, "LSQR breakdown (non-finite recurrence)");
-                if (est > o.inner_tol) inner_done = false;
+                if (est > target[i]) inner_done = false;
             }
             if (inner_done) break;
         }
@@ -469,6 +498,11 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
         CAD_HIP(hipGetLastError());
     }
     CAD_HIP(hipMemcpyAsync(x, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+    double worst = 0, rsum = 0;
+    for (int i = 0; i < m->nsys; i++) {
+        worst = std::max(worst, rel[i]);
+        rsum += rn2[i];
+    }
     rep.iterations = it;
     rep.rel_normal_residual = worst;
     rep.residual_norm = std::sqrt(rsum);

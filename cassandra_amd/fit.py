@@ -13,6 +13,7 @@ shape [n, 2] holding the bits of {P, O}.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 import torch
@@ -40,9 +41,19 @@ class Context:
         call("cad_context_create", device, C.byref(h))
         self.h = h
         self.torch_device = torch.device("cuda", device)
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        if not self.h:
+            raise RuntimeError("context is closed")
+        self._children.add(obj)
 
     def close(self):
+        """Destroys every live pattern group / matrix of the context, then the
+        context itself."""
         if getattr(self, "h", None):
+            for child in list(self._children):
+                child._destroy()
             capi.lib().cad_context_destroy(self.h)
             self.h = None
 
@@ -82,17 +93,21 @@ class Patterns:
         h = C.c_void_p()
         call("cad_patterns_create", ctx.h, arr.ctypes.data_as(C.c_void_p), len(arr), C.byref(h))
         self.h = h
+        ctx._adopt(self)
         k, cols, npat = C.c_int(), C.c_int64(), C.c_int()
         call("cad_patterns_info", h, C.byref(k), C.byref(cols), C.byref(npat))
         self.k, self.cols, self.npatterns = k.value, cols.value, npat.value
 
-    def __del__(self):
+    def _destroy(self):
         if getattr(self, "h", None):
-            try:
-                capi.lib().cad_patterns_destroy(self.h)
-            except Exception:
-                pass
+            capi.lib().cad_patterns_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass
 
     def pattern_info(self, i):
         cls, mask, size, order, off = C.c_int(), C.c_uint64(), C.c_int64(), C.c_int(), C.c_int64()
@@ -138,6 +153,7 @@ class Matrix:
     def __init__(self, ctx: Context, h):
         self.ctx = ctx
         self.h = h
+        ctx._adopt(self)
         rows, k, cols, ib = C.c_int64(), C.c_int(), C.c_int64(), C.c_int()
         call("cad_matrix_info", h, C.byref(rows), C.byref(k), C.byref(cols), C.byref(ib))
         self.rows, self.k, self.cols, self.index_bytes = rows.value, k.value, cols.value, ib.value
@@ -150,13 +166,16 @@ class Matrix:
              C.byref(h))
         return cls(ctx, h)
 
-    def __del__(self):
+    def _destroy(self):
         if getattr(self, "h", None):
-            try:
-                capi.lib().cad_matrix_destroy(self.h)
-            except Exception:
-                pass
+            capi.lib().cad_matrix_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass
 
     def download(self) -> np.ndarray:
         out = np.zeros((self.rows, self.k), dtype=np.uint32)

@@ -48,8 +48,11 @@ const char* cad_version(void);
 int cad_context_create(int device, cad_context** out);
 int cad_context_destroy(cad_context* ctx);
 /* Enqueue subsequent work on `hip_stream` (a hipStream_t, e.g. torch's
- * current stream); NULL restores the context's own stream. */
+ * current stream; NULL is the null stream).  Solvers capture graphs only on a
+ * non-null stream. */
 int cad_context_set_stream(cad_context* ctx, void* hip_stream);
+/* Go back to the context's own stream. */
+int cad_context_reset_stream(cad_context* ctx);
 /* The stream work is currently enqueued on (a hipStream_t). */
 int cad_context_get_stream(cad_context* ctx, void** out);
 int cad_context_synchronize(cad_context* ctx);

@@ -820,6 +820,19 @@ void or_random_positions_mt(uint32_t seed, int64_t n, uint64_t* out)
     }
 }
 
+void or_random_positions_mt_dist(uint32_t seed, int64_t n, uint64_t* out)
+{
+    // Board/PositionGenerator.h:9-10 verbatim in behaviour: the draws go
+    // through uniform_int_distribution<uint64_t>{0, 2^64-1}.
+    std::mt19937_64 eng(seed);
+    std::uniform_int_distribution<uint64_t> dist{0, 0xFFFFFFFFFFFFFFFFULL};
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t a = dist(eng), b = dist(eng);
+        out[2 * i] = a & ~b;
+        out[2 * i + 1] = ~a & b;
+    }
+}
+
 void or_random_play(uint64_t seed, int64_t n, int e_lo, int e_hi, uint64_t* out)
 {
 #pragma omp parallel for schedule(dynamic, 256)

@@ -114,6 +114,7 @@ uint64_t or_perft(int plies);
 /* BoardTest/PositionGenerator.cpp & Board/PositionGenerator.cpp:22-32:
  * RandomPositionGenerator(seed): n positions from std::mt19937_64 */
 void or_random_positions_mt(uint32_t seed, int64_t n, uint64_t* out_PO);
+void or_random_positions_mt_dist(uint32_t seed, int64_t n, uint64_t* out_PO);
 /* random legal play, per-game splitmix64 stream (the project's documented
  * generator; see cassandra_amd/csrc/board.hpp for the definition) */
 void or_random_play(uint64_t seed, int64_t n, int e_lo, int e_hi, uint64_t* out_PO);

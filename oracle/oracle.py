@@ -81,6 +81,7 @@ def lib():
         L.or_perft.restype = C.c_uint64
         L.or_perft.argtypes = [C.c_int]
         L.or_random_positions_mt.argtypes = [C.c_uint32, i64, u64p]
+        L.or_random_positions_mt_dist.argtypes = [C.c_uint32, i64, u64p]
         L.or_random_play.argtypes = [C.c_uint64, i64, C.c_int, C.c_int, u64p]
         L.or_make_targets.argtypes = [u32p, i64, C.c_int, i64, C.c_uint64, C.c_uint64,
                                       C.c_double, C.c_double, f64p, f64p]
@@ -283,6 +284,12 @@ def random_play(seed, n, e_lo, e_hi=None):
 def random_positions_mt(seed, n):
     out = np.zeros(2 * n, dtype=np.uint64)
     lib().or_random_positions_mt(seed, n, _p(out, u64p))
+    return out.reshape(-1, 2)
+
+
+def random_positions_mt_dist(seed, n):
+    out = np.zeros(2 * n, dtype=np.uint64)
+    lib().or_random_positions_mt_dist(seed, n, _p(out, u64p))
     return out.reshape(-1, 2)
 
 

@@ -60,6 +60,12 @@ void ref_jacobi(void* m, float* d)
 }
 
 // FitWeights' solve (PatternFit/PatternFit.cpp:53-56) on a prepared matrix.
+// The reference writes `PCG solver(transposed(matrix) * matrix, ...)`: the
+// TransposedMatrix and TwoMatrix proxies are temporaries that die at the end
+// of that statement while PCG (Math/Solver.h:83) and TwoMatrix
+// (Math/MatrixProxy.h:168-169) keep references to them, so Iterate() reads
+// dangling memory.  Here the proxies are named so that they outlive the solver;
+// the arithmetic is the reference's.
 void ref_fit_pcg(void* mp, const float* b, int64_t rows, int iterations, float* x,
                  float* resid_hist)
 {
@@ -67,7 +73,9 @@ void ref_fit_pcg(void* mp, const float* b, int64_t rows, int iterations, float* 
     Vector score(b, b + rows);
     Vector weights(m.Cols(), 0.0f);
     DiagonalPreconditioner P(JacobiPreconditionerOfATA(m));
-    PCG solver(transposed(m) * m, P, weights, transposed(m) * score);
+    auto At = transposed(m);
+    auto AtA = At * m;
+    PCG solver(AtA, P, weights, At * score);
     for (int i = 0; i < iterations; i++) {
         solver.Iterate(1);
         if (resid_hist) resid_hist[i] = solver.Residuum();
