@@ -13,9 +13,6 @@ thread_local std::string g_last_error;
 
 void set_error(const std::string& s) { g_last_error = s; }
 
-void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
-                        const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
-                        const Context& ctx);
 
 namespace {
 
@@ -120,7 +117,8 @@ const ScatterPlan& eval_plan(cad_patterns* p, int64_t B)
     if (it != p->eval_plans.end()) return *it->second;
     if (p->eval_plans.size() >= 16) p->eval_plans.clear();
     std::unique_ptr<ScatterPlan> sp(new ScatterPlan);
-    build_scatter_plan(*sp, p->slot_lo, p->slot_hi, std::vector<int64_t>{0, B}, std::vector<uint32_t>{0u}, *p->ctx);
+    build_scatter_plan(*sp, p->slot_lo, p->slot_hi, std::vector<int64_t>{0, B}, std::vector<uint32_t>{0u},
+                       int64_t(p->group->cols()), 4, *p->ctx);
     const ScatterPlan& ref = *sp;
     p->eval_plans[B] = std::move(sp);
     return ref;
@@ -131,20 +129,30 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
 {
     hipStream_t s = p->ctx->stream;
     const int64_t C = int64_t(p->group->cols());
-    hipLaunchKernelGGL(k_fill<VT>, dim3(grid1(C, 256)), dim3(256), 0, s, grad, C, VT(0));
-    if (B <= 0) return;
-    const ScatterPlan& sp = eval_plan(p, B);
-    const bool f = sizeof(VT) == 4;
-    const size_t n = f ? sp.work4.size() : sp.work8.size();
-    const size_t lds = size_t(f ? sp.win4 : sp.win8) * sizeof(VT);
-    auto kern = k_scatter<VT, VT, RowMajor32>;
-    if (lds > 64 * 1024)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  int(lds));
-    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kScatterBlock), lds, s, RowMajor32{idx, p->group->k()},
-                       (f ? sp.d_work4 : sp.d_work8).as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
-                       sp.d_slot_list.as<uint16_t>(), p->d_syscol0.as<uint32_t>(), g, (const LsqrScalars*)nullptr,
-                       grad);
+    if (B <= 0) {
+        hipLaunchKernelGGL(k_fill<VT>, dim3(grid1(C, 256)), dim3(256), 0, s, grad, C, VT(0));
+        CAD_HIP(hipGetLastError());
+        return;
+    }
+    ScatterPlan& sp = const_cast<ScatterPlan&>(eval_plan(p, B));
+    ScatterLists& L = sizeof(VT) == 4 ? sp.f4 : sp.f8;
+    const size_t need = sizeof(VT) * size_t(std::max<int64_t>(1, L.part_elems));
+    if (L.part.bytes < need) L.part = DBuf(need);
+    VT* part = L.part.as<VT>();
+    if (!L.work.empty()) {
+        const size_t lds = size_t(L.widest) * sizeof(VT);
+        auto kern = k_scatter_part<VT, VT, RowMajor32>;
+        if (lds > 64 * 1024)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s,
+                           RowMajor32{idx, p->group->k()}, L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
+                           sp.d_slot_list.as<uint16_t>(), p->d_syscol0.as<uint32_t>(), g,
+                           (const LsqrScalars*)nullptr, part);
+        CAD_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL((k_scatter_reduce<VT, VT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
+                       L.d_reduce.as<ReduceWork>(), (const VT*)part, grad, (const LsqrScalars*)nullptr);
     CAD_HIP(hipGetLastError());
 }
 
@@ -624,15 +632,7 @@ int cad_make_targets(cad_matrix* m, uint64_t seed_x, uint64_t seed_n, double sx,
         hipLaunchKernelGGL(k_targets_x, dim3(grid1(m->cols, 256)), dim3(256), 0, s, seed_x, m->cols, sx, xt);
         CAD_HIP(hipGetLastError());
         if (!m->rows) return;
-        const GridPlan& g = m->grid;
-        if (m->ibytes == 2)
-            hipLaunchKernelGGL((k_targets_b<SlotMajor<uint16_t>>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0,
-                               s, SlotMajor<uint16_t>{m->d_idx.as<uint16_t>(), m->rows, m->d_base.as<uint32_t>()},
-                               m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), xt, seed_n, sn, b);
-        else
-            hipLaunchKernelGGL((k_targets_b<SlotMajor<uint32_t>>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0,
-                               s, SlotMajor<uint32_t>{m->d_idx.as<uint32_t>(), m->rows, m->d_base.as<uint32_t>()},
-                               m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), xt, seed_n, sn, b);
+        matrix_targets(m, xt, seed_n, sn, b);
         CAD_HIP(hipGetLastError());
     });
 }

@@ -35,12 +35,34 @@ struct ColWork {
 };
 
 // A scatter tile of A^T y: rows [row_begin,row_end) x the slots of `group`,
-// accumulating the column window [win_lo, win_hi) in LDS.
+// accumulating the column window [win_lo, win_hi) in LDS and storing it to
+// partial[part_off ..] (one slice per row chunk of the window).
 struct ScatterWork {
     int64_t row_begin, row_end;  // global rows
+    int64_t part_off;            // this tile's slice of the partial buffer
     uint32_t win_lo, win_hi;     // global columns
     int32_t group;               // index into the slot-group table
     int32_t sys;
+};
+
+// A column range inside one scatter window (or outside every window, nchunk
+// = 0): z[c] = sum over the window's chunk slices.
+struct ReduceWork {
+    int64_t col_begin, col_end;  // global columns
+    int64_t part_off;            // slice offset of col_begin in chunk 0
+    int32_t nchunk;              // chunk slices of the window
+    uint32_t stride;             // slice length (window width)
+    int32_t sys;
+    int32_t pad;
+};
+
+// A run of consecutive slots whose columns lie in [col_lo, col_lo + width)
+// of a system (one pattern block), for the LDS-staged gather.  Blocks wider
+// than the LDS are staged in nwin windows of wmax columns.
+struct GatherSeg {
+    uint32_t col_lo, width;
+    uint16_t slot_first, nslots;
+    uint16_t nwin, pad;
 };
 
 // Slots whose column ranges overlap: a column index can repeat inside a row

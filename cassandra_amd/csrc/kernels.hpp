@@ -4,8 +4,8 @@
 //   slot-major: idx[s * ld + r], r < rows, s < k, column = sys_base + base[s] + idx
 //   (u16 when every slot's column range is narrower than 65536, else u32).
 // Every row-parallel kernel walks a RowWork list, every column-parallel kernel a
-// ColWork list and the scatter kernel a ScatterWork list, so one launch covers
-// any number of independent systems (block-diagonal batches).
+// ColWork list and the scatter kernels ScatterWork / ReduceWork lists, so one
+// launch covers any number of independent systems (block-diagonal batches).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,12 +18,13 @@ namespace cad {
 namespace {
 
 constexpr int kRowBlock = 256;   // threads of the row / column kernels
-constexpr int kRowsPerThread = 4;
+constexpr int kRowsPerThread = 4;  // rows per thread of the generic row work
 constexpr int kScatterBlock = 1024;
 
 // ------------------------------------------------------------ index views --
 template <class IdxT>
 struct SlotMajor {
+    using index_type = IdxT;
     const IdxT* __restrict__ idx;
     int64_t ld;
     const uint32_t* __restrict__ base;  // per slot
@@ -143,9 +144,113 @@ __global__ __launch_bounds__(kRowBlock) void k_index_extract(
     }
 }
 
-// ------------------------------------------------------------ SpMV ---------
+// ------------------------------------------------------------ gathers ------
 // y = A x with the in-row sum taken sequentially in slot order from zero, the
-// order of Math/MatrixCSR.cpp:59-62 (bit-exact with it in FP32).
+// order of Math/MatrixCSR.cpp:59-62 (bit-exact with it in FP32).  What is
+// done with each row sum is an epilogue:
+//   bool begin(sys)             per block; false skips the block (inactive system)
+//   double row(r, acc)          store the row; returns its share of a reduction
+//   void finish(sys, total)     last block of a system (kReduce only)
+template <class VT>
+struct EpiStore {
+    VT* __restrict__ y;
+    static constexpr bool kReduce = false;
+    __device__ bool begin(int) { return true; }
+    __device__ double row(int64_t r, VT a) { y[r] = a; return 0.0; }
+    __device__ void finish(int, double) {}
+};
+
+// LSQR step 1 (Math/Solver.h:161): u <- A v - alpha u.  u is kept unnormalised
+// as u_raw with u = u_raw / beta; the gather source is vd = D v.  The new beta
+// is |u_raw| (FP64 reduction).
+template <class VT>
+struct EpiLsqrU {
+    VT* __restrict__ u;
+    LsqrScalars* scal;
+    VT f;
+    static constexpr bool kReduce = true;
+    __device__ bool begin(int sys)
+    {
+        const LsqrScalars* st = &scal[sys];
+        if (!st->active) return false;
+        const double beta = st->beta;
+        f = VT(beta != 0.0 ? st->alpha / beta : 0.0);
+        return true;
+    }
+    __device__ double row(int64_t r, VT a)
+    {
+        const VT v = a - f * u[r];
+        u[r] = v;
+        return double(v) * double(v);
+    }
+    __device__ void finish(int sys, double s)
+    {
+        scal[sys].beta = sqrt(s);
+        scal[sys].anorm2 += s;
+    }
+};
+
+// FP64 residual of the original system: r = b - A x into r64, |r|^2 into
+// reduce_a and beta = |r|; when u is given it also receives r converted to VT
+// (the start vector of a refinement solve).
+template <class VT>
+struct EpiResidual {
+    const double* __restrict__ b;
+    double* __restrict__ r64;
+    VT* __restrict__ u;
+    LsqrScalars* scal;
+    static constexpr bool kReduce = true;
+    __device__ bool begin(int sys) { return scal[sys].active != 0; }
+    __device__ double row(int64_t r, double a)
+    {
+        const double v = b[r] - a;
+        r64[r] = v;
+        if (u) u[r] = VT(v);
+        return v * v;
+    }
+    __device__ void finish(int sys, double s)
+    {
+        scal[sys].reduce_a = s;
+        scal[sys].beta = sqrt(s);
+        scal[sys].anorm2 = 0.0;
+    }
+};
+
+// b = clip(2 * floor((A x_true + sn * N) / 2 + 1/2), -64, 64)
+struct EpiTargets {
+    double* __restrict__ b;
+    uint64_t seed;
+    double sn;
+    static constexpr bool kReduce = false;
+    __device__ bool begin(int) { return true; }
+    __device__ double row(int64_t r, double a)
+    {
+        Stream rng(seed, uint64_t(r));
+        const double v = __dadd_rn(a, __dmul_rn(sn, rng.ih12()));
+        const double e = 2.0 * floor(v * 0.5 + 0.5);
+        b[r] = fmin(64.0, fmax(-64.0, e));
+        return 0.0;
+    }
+    __device__ void finish(int, double) {}
+};
+
+// Block reduction of the epilogue shares, then the per-system grid finish.
+template <int BS, class Epi>
+__device__ __forceinline__ void reduce_tail(Epi& epi, double acc, int sys, const GridSys& g)
+{
+    if constexpr (Epi::kReduce) {
+        __shared__ double sh[BS / 64];
+        acc = block_sum<BS>(acc, sh);
+        if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
+        if (arrive_last(&g.counters[sys], g.blk_count[sys])) {
+            double s = 0.0;
+            for (int i = threadIdx.x; i < g.blk_count[sys]; i += BS) s += g.partials[2 * (g.blk_first[sys] + i)];
+            s = block_sum<BS>(s, sh);
+            if (threadIdx.x == 0) epi.finish(sys, s);
+        }
+    }
+}
+
 template <class VT, class View>
 __device__ __forceinline__ VT row_sum(const View& m, int k, int64_t r, uint32_t cb, const VT* __restrict__ x)
 {
@@ -155,84 +260,107 @@ __device__ __forceinline__ VT row_sum(const View& m, int k, int64_t r, uint32_t 
     return acc;
 }
 
-template <class VT, class View>
-__global__ __launch_bounds__(kRowBlock) void k_spmv(View m, int k, const RowWork* __restrict__ work,
-                                                    const uint32_t* __restrict__ sys_col, const VT* __restrict__ x,
-                                                    VT* __restrict__ y)
+// Generic gather: one row per thread, x read straight from global memory
+// (any matrix).
+template <class VT, class View, class Epi>
+__global__ __launch_bounds__(kRowBlock) void k_gather_rows(View m, int k, const RowWork* __restrict__ work,
+                                                           const uint32_t* __restrict__ sys_col,
+                                                           const VT* __restrict__ x, Epi epi, GridSys g)
 {
     const RowWork w = work[blockIdx.x];
-    const uint32_t cb = sys_col[w.sys];
-    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock) y[r] = row_sum<VT>(m, k, r, cb, x);
-}
-
-// FP64 residual of the original system: r = b - A x.  |r|^2 is reduced per
-// system into scal[sys].reduce_a and scal[sys].beta = |r|; when u is given it
-// also receives r converted to VT (the start vector of a refinement solve).
-template <class VT, class View>
-__global__ __launch_bounds__(kRowBlock) void k_residual(View m, int k, const RowWork* __restrict__ work,
-                                                        const uint32_t* __restrict__ sys_col,
-                                                        const double* __restrict__ x, const double* __restrict__ b,
-                                                        double* __restrict__ r, VT* __restrict__ u, GridSys g,
-                                                        LsqrScalars* scal)
-{
-    __shared__ double sh[kRowBlock / 64];
-    const RowWork w = work[blockIdx.x];
-    if (!scal[w.sys].active) return;
+    if (!epi.begin(w.sys)) return;
     const uint32_t cb = sys_col[w.sys];
     double acc = 0.0;
-    for (int64_t i = w.row_begin + threadIdx.x; i < w.row_end; i += kRowBlock) {
-        const double v = b[i] - row_sum<double>(m, k, i, cb, x);
-        r[i] = v;
-        if (u) u[i] = VT(v);
-        acc += v * v;
-    }
-    acc = block_sum<kRowBlock>(acc, sh);
-    if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
-    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
-        s = block_sum<kRowBlock>(s, sh);
-        if (threadIdx.x == 0) {
-            scal[w.sys].reduce_a = s;
-            scal[w.sys].beta = sqrt(s);
-            scal[w.sys].anorm2 = 0.0;
-        }
-    }
+    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock)
+        acc += epi.row(r, row_sum<VT>(m, k, r, cb, x));
+    reduce_tail<kRowBlock>(epi, acc, w.sys, g);
 }
 
-// LSQR step 1 (Math/Solver.h:161): u <- A v - alpha u.  u is kept unnormalised
-// as u_raw with u = u_raw / beta; vd = D v is the gather source.  The new beta
-// is |u_raw| (FP64 reduction).
-template <class VT, class View>
-__global__ __launch_bounds__(kRowBlock) void k_lsqr_u(View m, int k, const RowWork* __restrict__ work,
-                                                      const uint32_t* __restrict__ sys_col,
-                                                      const VT* __restrict__ vd, VT* __restrict__ u,
-                                                      GridSys g, LsqrScalars* scal)
+// LDS-staged gather for pattern matrices.  A block owns a tile of up to
+// kTileBlock * RPT rows (RPT rows per thread, accumulators in registers) and
+// walks the slot segments in slot order; each segment's column block of x is
+// staged in LDS (coalesced) and gathered from there, so the random reads hit
+// LDS instead of pulling a cache line from L2 per element.  Segments wider than
+// the LDS (wmax columns) are staged window by window: each slot's value is
+// kept in registers until every window was seen, then the segment's slots are
+// added in slot order, so the sum order is the sequential one.
+constexpr int kTileBlock = 1024;
+constexpr int kMaxSegSlots = 8;
+
+template <class VT, int RPT, class IdxT, class Epi>
+__global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, const RowWork* __restrict__ work,
+                                                            const uint32_t* __restrict__ sys_col,
+                                                            const GatherSeg* __restrict__ segs, int nseg,
+                                                            uint32_t wmax, const VT* __restrict__ x, Epi epi,
+                                                            GridSys g)
 {
-    __shared__ double sh[kRowBlock / 64];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    VT* xs = reinterpret_cast<VT*>(smem_raw);
     const RowWork w = work[blockIdx.x];
-    LsqrScalars* st = &scal[w.sys];
-    if (!st->active) return;
+    if (!epi.begin(w.sys)) return;
     const uint32_t cb = sys_col[w.sys];
-    const double beta = st->beta;
-    const VT f = VT(beta != 0.0 ? st->alpha / beta : 0.0);
-    double acc = 0.0;
-    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock) {
-        const VT v = row_sum<VT>(m, k, r, cb, vd) - f * u[r];
-        u[r] = v;
-        acc += double(v) * double(v);
-    }
-    acc = block_sum<kRowBlock>(acc, sh);
-    if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
-    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
-        s = block_sum<kRowBlock>(s, sh);
-        if (threadIdx.x == 0) {
-            st->beta = sqrt(s);
-            st->anorm2 += s;
+    // rows of this thread: r0 + i * kTileBlock for i * kTileBlock < nval
+    const int64_t r0 = w.row_begin + threadIdx.x;
+    const int nval = int(max<int64_t>(0, min<int64_t>(w.row_end - r0, int64_t(RPT) * kTileBlock)));
+    VT acc[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; i++) acc[i] = VT(0);
+    for (int sg = 0; sg < nseg; sg++) {
+        const GatherSeg seg = segs[sg];
+        const VT* __restrict__ xg = x + cb + seg.col_lo;
+        if (seg.nwin == 1) {
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < seg.width; i += kTileBlock) xs[i] = xg[i];
+            __syncthreads();
+            for (int s = seg.slot_first; s < seg.slot_first + seg.nslots; s++) {
+                const IdxT* __restrict__ p = m.idx + int64_t(s) * m.ld + r0;
+                // local offset in 32-bit arithmetic: base - col_lo wraps for
+                // u32 storage (base 0) and the sum with the index unwraps it
+                const uint32_t off = m.base[s] - seg.col_lo;
+#pragma unroll
+                for (int i = 0; i < RPT; i++)
+                    if (i * kTileBlock < nval) acc[i] += xs[off + uint32_t(p[i * kTileBlock])];
+            }
+        } else {
+            VT v[RPT][kMaxSegSlots];
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++)
+#pragma unroll
+                for (int i = 0; i < RPT; i++) v[i][j] = VT(0);
+            for (uint32_t wl = 0; wl < seg.width; wl += wmax) {
+                const uint32_t W = min(wmax, seg.width - wl);
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < W; i += kTileBlock) xs[i] = xg[wl + i];
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < kMaxSegSlots; j++) {
+                    if (j < seg.nslots) {
+                        const int s = seg.slot_first + j;
+                        const IdxT* __restrict__ p = m.idx + int64_t(s) * m.ld + r0;
+                        const uint32_t off = m.base[s] - seg.col_lo - wl;
+#pragma unroll
+                        for (int i = 0; i < RPT; i++) {
+                            if (i * kTileBlock < nval) {
+                                const uint32_t c = off + uint32_t(p[i * kTileBlock]);
+                                if (c < W) v[i][j] = xs[c];
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++)
+                if (j < seg.nslots) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) acc[i] += v[i][j];
+                }
         }
     }
+    double red = 0.0;
+#pragma unroll
+    for (int i = 0; i < RPT; i++)
+        if (i * kTileBlock < nval) red += epi.row(r0 + int64_t(i) * kTileBlock, acc[i]);
+    reduce_tail<kTileBlock>(epi, red, w.sys, g);
 }
 
 // Rows init of LSQR: u_raw = src (converted), beta = |src| (FP64).
@@ -265,17 +393,17 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_u(const RowWork* __rest
 }
 
 // ------------------------------------------------------------ scatter ------
-// z += A^T (y * scale): each block owns a column window of one slot group in
-// LDS, scatters a range of rows into it (ds_add) and flushes the window to
-// global memory with one atomic per touched column.
-// scale = 1 / div[sys] when div is given (0 when div[sys] == 0), else 1.
+// A^T (y * scale), stage 1: each block owns a column window of one slot group
+// in LDS, scatters a chunk of rows into it (ds_add) and stores the window to
+// its slice of the partial buffer -- no global atomics.
+// scale = 1 / div[sys].beta when div is given (0 when beta == 0), else 1.
 template <class VT, class YT, class View>
-__global__ __launch_bounds__(kScatterBlock) void k_scatter(View m, const ScatterWork* __restrict__ work,
-                                                           const SlotGroup* __restrict__ groups,
-                                                           const uint16_t* __restrict__ slot_list,
-                                                           const uint32_t* __restrict__ sys_col,
-                                                           const YT* __restrict__ y, const LsqrScalars* div,
-                                                           VT* __restrict__ z)
+__global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const ScatterWork* __restrict__ work,
+                                                                const SlotGroup* __restrict__ groups,
+                                                                const uint16_t* __restrict__ slot_list,
+                                                                const uint32_t* __restrict__ sys_col,
+                                                                const YT* __restrict__ y, const LsqrScalars* div,
+                                                                VT* __restrict__ part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     VT* win = reinterpret_cast<VT*>(smem_raw);
@@ -300,9 +428,24 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter(View m, const Scatter
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) {
-        const VT v = win[i];
-        if (v != VT(0)) atomicAdd(&z[w.win_lo + i], v);
+    VT* __restrict__ out = part + w.part_off;
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = win[i];
+}
+
+// Stage 2: z[c] = sum of the window's chunk slices, in chunk order
+// (deterministic); columns outside every window get 0.
+template <class T, class ZT>
+__global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* __restrict__ work,
+                                                              const T* __restrict__ part, ZT* __restrict__ z,
+                                                              const LsqrScalars* div)
+{
+    const ReduceWork w = work[blockIdx.x];
+    if (div && !div[w.sys].active) return;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const T* __restrict__ p = part + w.part_off + (c - w.col_begin);
+        T s = T(0);
+        for (int j = 0; j < w.nchunk; j++) s += p[int64_t(j) * w.stride];
+        z[c] = ZT(s);
     }
 }
 
@@ -311,11 +454,11 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter(View m, const Scatter
 // of its column inside the slot group (repeats can only occur within a group),
 // which sums to mult^2 per row.  Counts are exact 32-bit integers.
 template <class View>
-__global__ __launch_bounds__(kScatterBlock) void k_jacobi_count(View m, const ScatterWork* __restrict__ work,
-                                                                const SlotGroup* __restrict__ groups,
-                                                                const uint16_t* __restrict__ slot_list,
-                                                                const uint32_t* __restrict__ sys_col,
-                                                                uint32_t* __restrict__ cnt)
+__global__ __launch_bounds__(kScatterBlock) void k_jacobi_part(View m, const ScatterWork* __restrict__ work,
+                                                               const SlotGroup* __restrict__ groups,
+                                                               const uint16_t* __restrict__ slot_list,
+                                                               const uint32_t* __restrict__ sys_col,
+                                                               uint32_t* __restrict__ part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     uint32_t* win = reinterpret_cast<uint32_t*>(smem_raw);
@@ -336,10 +479,8 @@ __global__ __launch_bounds__(kScatterBlock) void k_jacobi_count(View m, const Sc
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) {
-        const uint32_t v = win[i];
-        if (v) atomicAdd(&cnt[w.win_lo + i], v);
-    }
+    uint32_t* __restrict__ out = part + w.part_off;
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = win[i];
 }
 
 // 1 / diag in float, exactly as inv() of the reference (1/0 = inf).
@@ -359,10 +500,9 @@ __global__ void k_colscale(const uint32_t* __restrict__ cnt, int64_t n, double* 
 // ------------------------------------------------------------ LSQR cols ----
 // LSQR step 2 (Math/Solver.h:162) and the rotation (:165-171):
 //   vbar = D z - beta v, alpha = |vbar|, then rho, c, s, theta, phi, phibar.
-// z is cleared for the next scatter.
 template <class VT>
 __global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict__ work,
-                                                      const double* __restrict__ D, VT* __restrict__ z,
+                                                      const double* __restrict__ D, const VT* __restrict__ z,
                                                       double* __restrict__ v, GridSys g, LsqrScalars* scal)
 {
     __shared__ double sh[kRowBlock / 64];
@@ -373,7 +513,6 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict_
     double acc = 0.0;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         const double vb = D[c] * double(z[c]) - beta * v[c];
-        z[c] = VT(0);
         v[c] = vb;
         acc += vb * vb;
     }
@@ -430,7 +569,7 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_xw(const ColWork* __restrict
 //   vbar = D z / beta, alpha = |vbar|; also |z|^2 = |A^T src|^2 into reduce_b.
 template <class ZT>
 __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_v(const ColWork* __restrict__ work,
-                                                           const double* __restrict__ D, ZT* __restrict__ z,
+                                                           const double* __restrict__ D, const ZT* __restrict__ z,
                                                            double* __restrict__ v, GridSys g, LsqrScalars* scal)
 {
     __shared__ double sh[kRowBlock / 64];
@@ -442,7 +581,6 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_v(const ColWork* __rest
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         const double zc = double(z[c]);
         const double vb = D[c] * zc * ib;
-        z[c] = ZT(0);
         v[c] = vb;
         a0 += vb * vb;
         a1 += zc * zc;
@@ -509,9 +647,9 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_apply(const ColWork* __restr
     }
 }
 
-// |z|^2 per system into reduce_b (used for |A^T b| and |A^T r|); z is cleared.
+// |z|^2 per system into reduce_b (used for |A^T b| and |A^T r|).
 template <class ZT>
-__global__ __launch_bounds__(kRowBlock) void k_cols_norm2(const ColWork* __restrict__ work, ZT* __restrict__ z,
+__global__ __launch_bounds__(kRowBlock) void k_cols_norm2(const ColWork* __restrict__ work, const ZT* __restrict__ z,
                                                           GridSys g, LsqrScalars* scal)
 {
     __shared__ double sh[kRowBlock / 64];
@@ -521,7 +659,6 @@ __global__ __launch_bounds__(kRowBlock) void k_cols_norm2(const ColWork* __restr
     double acc = 0.0;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         const double zc = double(z[c]);
-        z[c] = ZT(0);
         acc += zc * zc;
     }
     acc = block_sum<kRowBlock>(acc, sh);
@@ -717,24 +854,6 @@ __global__ void k_targets_x(uint64_t seed, int64_t cols, double sx, double* __re
     for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < cols; c += int64_t(gridDim.x) * blockDim.x) {
         Stream s(seed, uint64_t(c));
         xt[c] = __dmul_rn(sx, s.ih12());
-    }
-}
-
-// b = clip(2 * floor((A x_true + sn * N) / 2 + 1/2), -64, 64)
-template <class View>
-__global__ __launch_bounds__(kRowBlock) void k_targets_b(View m, int k, const RowWork* __restrict__ work,
-                                                         const uint32_t* __restrict__ sys_col,
-                                                         const double* __restrict__ xt, uint64_t seed, double sn,
-                                                         double* __restrict__ b)
-{
-    const RowWork w = work[blockIdx.x];
-    const uint32_t cb = sys_col[w.sys];
-    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock) {
-        const double s = row_sum<double>(m, k, r, cb, xt);
-        Stream rng(seed, uint64_t(r));
-        const double v = __dadd_rn(s, __dmul_rn(sn, rng.ih12()));
-        const double e = 2.0 * floor(v * 0.5 + 0.5);
-        b[r] = fmin(64.0, fmax(-64.0, e));
     }
 }
 

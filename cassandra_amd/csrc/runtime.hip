@@ -102,15 +102,17 @@ void make_groups(const std::vector<uint32_t>& lo, const std::vector<uint32_t>& h
     }
 }
 
-// Tiles of (system rows) x (group window).  Row chunks are sized so that the
-// LDS accumulation of a tile outweighs its window flush and the grid holds
-// several tiles per CU.
-std::vector<ScatterWork> make_scatter_work(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
-                                           const std::vector<uint32_t>& sys_col, uint32_t wmax, int cus,
-                                           uint32_t& widest)
+// Scatter tiles of (system rows) x (group window).  The number of row chunks
+// of a window is set by its share of the work (about two tiles per CU in
+// total), capped so that the partial slices (written and read once) stay near
+// 10% of the window's index traffic.
+void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
+                        const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, uint32_t wmax, int vbytes,
+                        int ibytes, int cus, ScatterLists& L)
 {
-    std::vector<ScatterWork> out;
-    widest = 0;
+    L.work.clear();
+    L.reduce.clear();
+    L.widest = 0;
     const int nsys = int(sys_col.size());
     double total = 0;
     for (int s = 0; s < nsys; s++)
@@ -119,55 +121,159 @@ std::vector<ScatterWork> make_scatter_work(const ScatterPlan& sp, const std::vec
             const uint32_t nwin = (g.col_hi - g.col_lo + wmax - 1) / wmax;
             total += double(R) * g.nslots * nwin;
         }
-    const double target_items = 8.0 * std::max(cus, 1);
-    const double per_item = std::max(1.0, total / target_items);
+    const double target = 2.0 * std::max(cus, 1);
+    int64_t off = 0;
     for (int s = 0; s < nsys; s++) {
         const int64_t R = sys_rows[s + 1] - sys_rows[s];
-        if (R <= 0) continue;
+        std::vector<std::pair<uint32_t, uint32_t>> covered;
         for (size_t gi = 0; gi < sp.groups.size(); gi++) {
             const SlotGroup& g = sp.groups[gi];
             for (uint32_t wl = g.col_lo; wl < g.col_hi; wl += wmax) {
                 const uint32_t wh = std::min(g.col_hi, wl + wmax);
                 const uint32_t W = wh - wl;
-                int64_t chunk = int64_t(per_item / g.nslots);
-                chunk = std::max<int64_t>(chunk, 4 * int64_t(W) / std::max<int>(1, g.nslots));
-                chunk = std::max<int64_t>(chunk, kScatterBlock);
-                chunk = (chunk + kScatterBlock - 1) / kScatterBlock * kScatterBlock;
-                for (int64_t r0 = 0; r0 < R; r0 += chunk) {
-                    ScatterWork w;
-                    w.row_begin = sys_rows[s] + r0;
-                    w.row_end = sys_rows[s] + std::min(R, r0 + chunk);
-                    w.win_lo = sys_col[s] + wl;
-                    w.win_hi = sys_col[s] + wh;
-                    w.group = int32_t(gi);
-                    w.sys = s;
-                    out.push_back(w);
+                covered.push_back({wl, wh});
+                L.widest = std::max(L.widest, W);
+                int64_t C = 0;
+                if (R > 0) {
+                    const double share = total > 0 ? double(R) * g.nslots / total : 1.0;
+                    C = std::max<int64_t>(1, int64_t(std::llround(target * share)));
+                    const double cap = 0.1 * double(R) * g.nslots * ibytes / (2.0 * W * vbytes);
+                    C = std::max<int64_t>(1, std::min<int64_t>(C, int64_t(cap)));
+                    C = std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock);
+                    const int64_t chunk = (R + C - 1) / C;
+                    C = (R + chunk - 1) / chunk;
+                    for (int64_t c = 0; c < C; c++) {
+                        ScatterWork w;
+                        w.row_begin = sys_rows[s] + c * chunk;
+                        w.row_end = sys_rows[s] + std::min(R, (c + 1) * chunk);
+                        w.part_off = off + c * int64_t(W);
+                        w.win_lo = sys_col[s] + wl;
+                        w.win_hi = sys_col[s] + wh;
+                        w.group = int32_t(gi);
+                        w.sys = s;
+                        L.work.push_back(w);
+                    }
                 }
-                widest = std::max(widest, W);
+                for (uint32_t c0 = wl; c0 < wh; c0 += 1024) {
+                    ReduceWork r;
+                    r.col_begin = int64_t(sys_col[s]) + c0;
+                    r.col_end = int64_t(sys_col[s]) + std::min(wh, c0 + 1024);
+                    r.part_off = off + (c0 - wl);
+                    r.nchunk = int32_t(C);
+                    r.stride = W;
+                    r.sys = s;
+                    r.pad = 0;
+                    L.reduce.push_back(r);
+                }
+                off += C * int64_t(W);
             }
         }
+        // columns outside every window are written as zeros
+        std::sort(covered.begin(), covered.end());
+        covered.push_back({uint32_t(cols_per_sys), uint32_t(cols_per_sys)});
+        uint32_t at = 0;
+        for (auto& iv : covered) {
+            for (uint32_t c0 = at; c0 < iv.first; c0 += 1024) {
+                ReduceWork r;
+                r.col_begin = int64_t(sys_col[s]) + c0;
+                r.col_end = int64_t(sys_col[s]) + std::min(iv.first, c0 + 1024);
+                r.part_off = 0;
+                r.nchunk = 0;
+                r.stride = 0;
+                r.sys = s;
+                r.pad = 0;
+                L.reduce.push_back(r);
+            }
+            at = std::max(at, iv.second);
+        }
     }
-    return out;
+    L.part_elems = off;
 }
 
 }  // namespace
 
+GridSys GatherPlan::sys() const
+{
+    return GridSys{d_first.as<int32_t>(), d_count.as<int32_t>(), d_counters.as<unsigned>(), d_partials.as<double>()};
+}
+
 void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
                         const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
-                        const Context& ctx)
+                        int64_t cols_per_sys, int ibytes, const Context& ctx)
 {
     make_groups(lo, hi, sp);
-    const size_t lds = std::min<size_t>(ctx.max_lds, 128 * 1024);
-    const uint32_t w4 = uint32_t(lds / 4), w8 = uint32_t(lds / 8);
-    sp.work4 = make_scatter_work(sp, sys_rows, sys_col, w4, ctx.cus, sp.win4);
-    sp.work8 = make_scatter_work(sp, sys_rows, sys_col, w8, ctx.cus, sp.win8);
-    sp.work_cnt = sp.work4;
-    sp.win_cnt = sp.win4;
+    const size_t lds = std::min<size_t>(ctx.max_lds, 160 * 1024);
+    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 4), 4, ibytes, ctx.cus, sp.f4);
+    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 8), 8, ibytes, ctx.cus, sp.f8);
     sp.d_groups = to_device(sp.groups, ctx.stream);
     sp.d_slot_list = to_device(sp.slot_list, ctx.stream);
-    sp.d_work4 = to_device(sp.work4, ctx.stream);
-    sp.d_work8 = to_device(sp.work8, ctx.stream);
-    sp.d_work_cnt = to_device(sp.work_cnt, ctx.stream);
+    for (ScatterLists* L : {&sp.f4, &sp.f8}) {
+        L->d_work = to_device(L->work, ctx.stream);
+        L->d_reduce = to_device(L->reduce, ctx.stream);
+        L->part = DBuf();
+    }
+}
+
+// Segments of the tiled gather: the slot groups must be runs of consecutive
+// slots that follow each other in slot order (true for pattern matrices, whose
+// groups are the patterns); groups wider than the LDS may hold at most
+// kMaxSegSlots slots.
+void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
+{
+    const Context& ctx = *m->ctx;
+    const ScatterPlan& sp = m->splan;
+    gp.ok = false;
+    gp.rpt = rpt;
+    const size_t lds = std::min<size_t>(ctx.max_lds, 160 * 1024);
+    gp.wmax = uint32_t(lds / vbytes);
+    std::vector<const SlotGroup*> order;
+    for (auto& g : sp.groups) order.push_back(&g);
+    std::sort(order.begin(), order.end(),
+              [&](const SlotGroup* a, const SlotGroup* b) { return sp.slot_list[a->first] < sp.slot_list[b->first]; });
+    int next = 0;
+    gp.segs.clear();
+    gp.lds_elems = 0;
+    for (const SlotGroup* g : order) {
+        const int first = sp.slot_list[g->first];
+        if (first != next) return;
+        for (int j = 0; j < g->nslots; j++)
+            if (sp.slot_list[g->first + j] != first + j) return;
+        GatherSeg seg;
+        seg.col_lo = g->col_lo;
+        seg.width = g->col_hi - g->col_lo;
+        seg.slot_first = uint16_t(first);
+        seg.nslots = g->nslots;
+        seg.nwin = uint16_t((seg.width + gp.wmax - 1) / gp.wmax);
+        seg.pad = 0;
+        if (seg.nwin > 1 && seg.nslots > kMaxSegSlots) return;
+        gp.lds_elems = std::max(gp.lds_elems, std::min(seg.width, gp.wmax));
+        gp.segs.push_back(seg);
+        next = first + g->nslots;
+    }
+    if (next != m->k || gp.segs.empty()) return;
+    // row tiles of at most kTileBlock * rpt rows; a single system is cut into a
+    // multiple of the CU count so that every CU gets the same number of tiles
+    const int64_t cap = int64_t(kTileBlock) * rpt;
+    gp.tiles.clear();
+    gp.first.assign(m->nsys, 0);
+    gp.count.assign(m->nsys, 0);
+    for (int s = 0; s < m->nsys; s++) {
+        const int64_t r0 = m->sys_row_begin[s], R = m->sys_row_begin[s + 1] - r0;
+        gp.first[s] = int32_t(gp.tiles.size());
+        int64_t n = (R + cap - 1) / cap;
+        if (m->nsys == 1 && n > 0) n = (n + ctx.cus - 1) / ctx.cus * ctx.cus;
+        const int64_t per = n > 0 ? (R + n - 1) / n : 0;
+        for (int64_t t = 0; t < n && t * per < R; t++)
+            gp.tiles.push_back(RowWork{r0 + t * per, r0 + std::min(R, (t + 1) * per), s, 0});
+        gp.count[s] = int32_t(gp.tiles.size()) - gp.first[s];
+    }
+    gp.d_segs = to_device(gp.segs, ctx.stream);
+    gp.d_tiles = to_device(gp.tiles, ctx.stream);
+    gp.d_first = to_device(gp.first, ctx.stream);
+    gp.d_count = to_device(gp.count, ctx.stream);
+    gp.d_counters = to_device(std::vector<unsigned>(std::max(1, m->nsys), 0u), ctx.stream);
+    gp.d_partials = DBuf(2 * sizeof(double) * std::max<size_t>(1, gp.tiles.size()));
+    gp.ok = true;
 }
 
 void build_plans(cad_matrix* m)
@@ -181,7 +287,10 @@ void build_plans(cad_matrix* m)
         m->cols_per_sys = m->cols;
     }
     m->d_sys_col = to_device(m->sys_col, ctx.stream);
-    build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, ctx);
+    build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys, m->ibytes,
+                       ctx);
+    build_gather_plan(m, m->gplan4, 4, 4);
+    build_gather_plan(m, m->gplan8, 8, 2);
 
     GridPlan& g = m->grid;
     g.rows.clear();

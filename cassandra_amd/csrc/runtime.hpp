@@ -47,15 +47,36 @@ DBuf to_device(const std::vector<T>& v, hipStream_t s)
     return b;
 }
 
+// The two-stage scatter of one value size: tiles (window x row chunk) that
+// write partial slices, and the column reduction over the slices.
+struct ScatterLists {
+    std::vector<ScatterWork> work;
+    std::vector<ReduceWork> reduce;
+    DBuf d_work, d_reduce;
+    uint32_t widest = 0;     // widest window (LDS of the tile kernel)
+    int64_t part_elems = 0;  // partial buffer length
+    DBuf part;               // partial buffer (allocated on first use)
+};
+
 // Slot groups + per-slot column ranges -> scatter work lists.
 struct ScatterPlan {
     std::vector<SlotGroup> groups;
     std::vector<uint16_t> slot_list;
     DBuf d_groups, d_slot_list;
-    // one work list per value size (4: float windows, 8: double windows)
-    std::vector<ScatterWork> work4, work8, work_cnt;
-    DBuf d_work4, d_work8, d_work_cnt;
-    uint32_t win4 = 0, win8 = 0, win_cnt = 0;  // widest window of each list
+    ScatterLists f4, f8;  // float and double (and uint32 counts) windows
+};
+
+// LDS-staged gather of one value size: row tiles and the slot segments.
+struct GatherPlan {
+    bool ok = false;  // the matrix has the pattern structure the tile kernel needs
+    int rpt = 0;      // rows per thread
+    uint32_t wmax = 0;
+    uint32_t lds_elems = 0;
+    std::vector<GatherSeg> segs;
+    std::vector<RowWork> tiles;
+    std::vector<int32_t> first, count;
+    DBuf d_segs, d_tiles, d_first, d_count, d_counters, d_partials;
+    GridSys sys() const;
 };
 
 // Blocks of rows / columns of the systems, with the per-system grid
@@ -110,6 +131,7 @@ struct cad_matrix {
     cad::DBuf d_sys_col;
     cad::ScatterPlan splan;
     cad::GridPlan grid;
+    cad::GatherPlan gplan4, gplan8;  // float / double tiles
     cad::DBuf d_jacobi_cnt;         // cached exact multiplicity counts (u32)
     bool have_cnt = false;
     std::map<int, std::unique_ptr<cad::Workspace>> ws;

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -54,45 +55,101 @@ void allow_lds(K kernel, size_t bytes)
     }
 }
 
-template <class VT, class YT, class View>
-void launch_scatter(const ScatterPlan& sp, const View& view, const uint32_t* sys_col, const YT* y,
-                    const LsqrScalars* div, VT* z, hipStream_t s)
+template <class VT>
+ScatterLists& lists(ScatterPlan& sp)
 {
-    const bool f = sizeof(VT) == 4;
-    const size_t n = f ? sp.work4.size() : sp.work8.size();
-    if (!n) return;
-    const ScatterWork* wl = (f ? sp.d_work4 : sp.d_work8).as<ScatterWork>();
-    const size_t lds = size_t(f ? sp.win4 : sp.win8) * sizeof(VT);
-    auto kern = k_scatter<VT, YT, View>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kScatterBlock), lds, s, view, wl,
-                       sp.d_groups.as<SlotGroup>(), sp.d_slot_list.as<uint16_t>(), sys_col, y, div, z);
-    CAD_HIP(hipGetLastError());
+    return sizeof(VT) == 4 ? sp.f4 : sp.f8;
+}
+
+// The partial buffer of a scatter list (allocated outside graph capture: every
+// solver calls prepare_scatter first).
+template <class PT>
+PT* partials(ScatterLists& L)
+{
+    const size_t bytes = sizeof(PT) * size_t(std::max<int64_t>(1, L.part_elems));
+    if (L.part.bytes < bytes) L.part = DBuf(bytes);
+    return L.part.as<PT>();
+}
+
+// A^T y in two launches: LDS window tiles into partial slices, then the
+// deterministic column reduction (which writes every column of z).
+template <class VT, class YT, class ZT, class View>
+void launch_scatter(ScatterPlan& sp, const View& view, const uint32_t* sys_col, const YT* y, const LsqrScalars* div,
+                    ZT* z, hipStream_t s)
+{
+    ScatterLists& L = lists<VT>(sp);
+    VT* part = partials<VT>(L);
+    if (!L.work.empty()) {
+        const size_t lds = size_t(L.widest) * sizeof(VT);
+        auto kern = k_scatter_part<VT, YT, View>;
+        allow_lds(kern, lds);
+        hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
+                           L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(), sp.d_slot_list.as<uint16_t>(),
+                           sys_col, y, div, part);
+        CAD_HIP(hipGetLastError());
+    }
+    if (!L.reduce.empty()) {
+        hipLaunchKernelGGL((k_scatter_reduce<VT, ZT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
+                           L.d_reduce.as<ReduceWork>(), (const VT*)part, z, div);
+        CAD_HIP(hipGetLastError());
+    }
 }
 
 template <class VT, class YT>
 void prepare_scatter(cad_matrix* m)
 {
-    const ScatterPlan& sp = m->splan;
-    const size_t lds = size_t(sizeof(VT) == 4 ? sp.win4 : sp.win8) * sizeof(VT);
-    with_view(m, [&](auto view) { allow_lds(k_scatter<VT, YT, decltype(view)>, lds); });
+    ScatterLists& L = lists<VT>(m->splan);
+    partials<VT>(L);
+    const size_t lds = size_t(L.widest) * sizeof(VT);
+    with_view(m, [&](auto view) { allow_lds(k_scatter_part<VT, YT, decltype(view)>, lds); });
 }
 
-template <class VT, class YT>
-void scatter(cad_matrix* m, const YT* y, const LsqrScalars* div, VT* z, hipStream_t s)
+template <class VT, class YT, class ZT>
+void scatter(cad_matrix* m, const YT* y, const LsqrScalars* div, ZT* z, hipStream_t s)
 {
-    with_view(m, [&](auto view) { launch_scatter<VT, YT>(m->splan, view, m->d_sys_col.as<uint32_t>(), y, div, z, s); });
+    with_view(m, [&](auto view) {
+        launch_scatter<VT, YT, ZT>(m->splan, view, m->d_sys_col.as<uint32_t>(), y, div, z, s);
+    });
+}
+
+template <class VT>
+GatherPlan& gather_plan(cad_matrix* m)
+{
+    return sizeof(VT) == 4 ? m->gplan4 : m->gplan8;
+}
+
+// Row gather with an epilogue: the LDS-staged tile kernel for pattern
+// matrices, the generic row kernel otherwise.
+template <class VT, class Epi>
+void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
+{
+    GatherPlan& gp = gather_plan<VT>(m);
+    constexpr int RPT = sizeof(VT) == 4 ? 4 : 2;
+    with_view(m, [&](auto view) {
+        using IdxT = typename decltype(view)::index_type;
+        if (gp.ok && gp.rpt == RPT) {
+            if (gp.tiles.empty()) return;
+            const size_t lds = size_t(gp.lds_elems) * sizeof(VT);
+            auto kern = k_gather_tile<VT, RPT, IdxT, Epi>;
+            allow_lds(kern, lds);
+            hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
+                               gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(), gp.d_segs.as<GatherSeg>(),
+                               int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
+        } else {
+            const GridPlan& g = m->grid;
+            if (g.rows.empty()) return;
+            hipLaunchKernelGGL((k_gather_rows<VT, decltype(view), Epi>), dim3(unsigned(g.rows.size())),
+                               dim3(kRowBlock), 0, s, view, m->k, g.d_rows.as<RowWork>(),
+                               m->d_sys_col.as<uint32_t>(), x, epi, g.row_sys());
+        }
+    });
+    CAD_HIP(hipGetLastError());
 }
 
 template <class VT>
 void spmv(cad_matrix* m, const VT* x, VT* y, hipStream_t s)
 {
-    const GridPlan& g = m->grid;
-    with_view(m, [&](auto view) {
-        hipLaunchKernelGGL((k_spmv<VT, decltype(view)>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s, view,
-                           m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), x, y);
-    });
-    CAD_HIP(hipGetLastError());
+    gather<VT>(m, x, EpiStore<VT>{y}, s);
 }
 
 // ------------------------------------------------------------ products ----
@@ -101,14 +158,17 @@ void matrix_spmv_f64(cad_matrix* m, const double* x, double* y) { spmv<double>(m
 
 void matrix_spmtv_f32(cad_matrix* m, const float* y, float* z)
 {
-    fill<float>(z, m->cols, 0.0f, m->ctx->stream);
-    scatter<float, float>(m, y, nullptr, z, m->ctx->stream);
+    scatter<float, float, float>(m, y, nullptr, z, m->ctx->stream);
 }
 
 void matrix_spmtv_f64(cad_matrix* m, const double* y, double* z)
 {
-    fill<double>(z, m->cols, 0.0, m->ctx->stream);
-    scatter<double, double>(m, y, nullptr, z, m->ctx->stream);
+    scatter<double, double, double>(m, y, nullptr, z, m->ctx->stream);
+}
+
+void matrix_targets(cad_matrix* m, const double* xt, uint64_t seed, double sn, double* b)
+{
+    gather<double>(m, xt, EpiTargets{b, seed, sn}, m->ctx->stream);
 }
 
 const uint32_t* jacobi_counts(cad_matrix* m)
@@ -116,20 +176,24 @@ const uint32_t* jacobi_counts(cad_matrix* m)
     if (m->have_cnt) return m->d_jacobi_cnt.as<uint32_t>();
     hipStream_t s = m->ctx->stream;
     m->d_jacobi_cnt = DBuf(sizeof(uint32_t) * std::max<int64_t>(1, m->cols));
-    fill<uint32_t>(m->d_jacobi_cnt.as<uint32_t>(), m->cols, 0u, s);
-    const ScatterPlan& sp = m->splan;
-    if (!sp.work_cnt.empty()) {
-        const size_t lds = size_t(sp.win_cnt) * 4;
+    ScatterPlan& sp = m->splan;
+    ScatterLists& L = sp.f4;
+    uint32_t* part = partials<uint32_t>(L);
+    if (!L.work.empty()) {
+        const size_t lds = size_t(L.widest) * 4;
         with_view(m, [&](auto view) {
-            auto kern = k_jacobi_count<decltype(view)>;
+            auto kern = k_jacobi_part<decltype(view)>;
             allow_lds(kern, lds);
-            hipLaunchKernelGGL(kern, dim3(unsigned(sp.work_cnt.size())), dim3(kScatterBlock), lds, s, view,
-                               sp.d_work_cnt.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
-                               sp.d_slot_list.as<uint16_t>(), m->d_sys_col.as<uint32_t>(),
-                               m->d_jacobi_cnt.as<uint32_t>());
+            hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
+                               L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
+                               sp.d_slot_list.as<uint16_t>(), m->d_sys_col.as<uint32_t>(), part);
         });
         CAD_HIP(hipGetLastError());
     }
+    hipLaunchKernelGGL((k_scatter_reduce<uint32_t, uint32_t>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0,
+                       s, L.d_reduce.as<ReduceWork>(), (const uint32_t*)part, m->d_jacobi_cnt.as<uint32_t>(),
+                       (const LsqrScalars*)nullptr);
+    CAD_HIP(hipGetLastError());
     m->have_cnt = true;
     return m->d_jacobi_cnt.as<uint32_t>();
 }
@@ -208,18 +272,9 @@ static void explicit_residual(cad_matrix* m, Workspace& ws, const double* x, con
     ensure(ws.scal_chk, sizeof(LsqrScalars) * m->nsys);
     std::vector<int> act(m->nsys, 1);
     set_active(m, ws, ws.scal_chk, act, s);
-    if (!ws.z64_clear) {
-        fill<double>(ws.z64.as<double>(), m->cols, 0.0, s);
-        ws.z64_clear = true;
-    }
     LsqrScalars* sc = ws.scal_chk.as<LsqrScalars>();
-    with_view(m, [&](auto view) {
-        hipLaunchKernelGGL((k_residual<double, decltype(view)>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s,
-                           view, m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), x, b,
-                           ws.r64.as<double>(), (double*)nullptr, g.row_sys(), sc);
-    });
-    CAD_HIP(hipGetLastError());
-    scatter<double, double>(m, ws.r64.as<double>(), nullptr, ws.z64.as<double>(), s);
+    gather<double>(m, x, EpiResidual<double>{b, ws.r64.as<double>(), nullptr, sc}, s);
+    scatter<double, double, double>(m, ws.r64.as<double>(), nullptr, ws.z64.as<double>(), s);
     hipLaunchKernelGGL(k_cols_norm2<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
                        g.d_cols.as<ColWork>(), ws.z64.as<double>(), g.col_sys(), sc);
     CAD_HIP(hipGetLastError());
@@ -240,9 +295,7 @@ static std::vector<double> atb_norms(cad_matrix* m, Workspace& ws, const double*
     ensure(ws.scal_chk, sizeof(LsqrScalars) * m->nsys);
     std::vector<int> act(m->nsys, 1);
     set_active(m, ws, ws.scal_chk, act, s);
-    fill<double>(ws.z64.as<double>(), m->cols, 0.0, s);
-    ws.z64_clear = true;
-    scatter<double, double>(m, b, nullptr, ws.z64.as<double>(), s);
+    scatter<double, double, double>(m, b, nullptr, ws.z64.as<double>(), s);
     hipLaunchKernelGGL(k_cols_norm2<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
                        g.d_cols.as<ColWork>(), ws.z64.as<double>(), g.col_sys(), ws.scal_chk.as<LsqrScalars>());
     CAD_HIP(hipGetLastError());
@@ -258,18 +311,27 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
 {
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
-    with_view(m, [&](auto view) {
-        hipLaunchKernelGGL((k_lsqr_u<VT, decltype(view)>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s, view,
-                           m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), ws.vd.as<VT>(), ws.u.as<VT>(),
-                           g.row_sys(), sc);
-        launch_scatter<VT, VT>(m->splan, view, m->d_sys_col.as<uint32_t>(), ws.u.as<VT>(), sc, ws.z.as<VT>(), s);
-    });
+    gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s);
+    scatter<VT, VT, VT>(m, ws.u.as<VT>(), sc, ws.z.as<VT>(), s);
     hipLaunchKernelGGL(k_lsqr_v<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.D.as<double>(), ws.z.as<VT>(), ws.v.as<double>(), g.col_sys(), sc);
     hipLaunchKernelGGL(k_lsqr_xw<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.D.as<double>(), ws.v.as<double>(), ws.vd.as<VT>(), ws.w.as<double>(), ws.y.as<double>(),
                        (const LsqrScalars*)sc);
     CAD_HIP(hipGetLastError());
+}
+
+// Kernel attributes of the iteration's gather (set outside graph capture).
+template <class VT>
+static void lsqr_prepare_gather(cad_matrix* m)
+{
+    GatherPlan& gp = gather_plan<VT>(m);
+    constexpr int RPT = sizeof(VT) == 4 ? 4 : 2;
+    if (!gp.ok) return;
+    with_view(m, [&](auto view) {
+        using IdxT = typename decltype(view)::index_type;
+        allow_lds(k_gather_tile<VT, RPT, IdxT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
+    });
 }
 
 // Runs n LSQR iterations, replaying a captured graph of `batch` iterations
@@ -284,6 +346,7 @@ static void lsqr_run(cad_matrix* m, Workspace& ws, int n, int batch, bool graph,
             ws.exec = nullptr;
             ws.graph = nullptr;
             prepare_scatter<VT, VT>(m);
+            lsqr_prepare_gather<VT>(m);
             CAD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < batch; i++) lsqr_iteration<VT>(m, ws, s);
             CAD_HIP(hipStreamEndCapture(s, &ws.graph));
@@ -307,17 +370,8 @@ static void lsqr_start_from_residual(cad_matrix* m, Workspace& ws, const double*
 {
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
-    if (!ws.z64_clear) {
-        fill<double>(ws.z64.as<double>(), m->cols, 0.0, s);
-        ws.z64_clear = true;
-    }
-    with_view(m, [&](auto view) {
-        hipLaunchKernelGGL((k_residual<VT, decltype(view)>), dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s, view,
-                           m->k, g.d_rows.as<RowWork>(), m->d_sys_col.as<uint32_t>(), ws.x64.as<double>(), b,
-                           ws.r64.as<double>(), ws.u.as<VT>(), g.row_sys(), sc);
-        launch_scatter<double, double>(m->splan, view, m->d_sys_col.as<uint32_t>(), ws.r64.as<double>(), nullptr,
-                                       ws.z64.as<double>(), s);
-    });
+    gather<double>(m, ws.x64.as<double>(), EpiResidual<VT>{b, ws.r64.as<double>(), ws.u.as<VT>(), sc}, s);
+    scatter<double, double, double>(m, ws.r64.as<double>(), nullptr, ws.z64.as<double>(), s);
     hipLaunchKernelGGL(k_lsqr_init_v<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
                        g.d_cols.as<ColWork>(), ws.D.as<double>(), ws.z64.as<double>(), ws.v.as<double>(), g.col_sys(),
                        sc);
@@ -516,7 +570,7 @@ static void pcg_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     const GridPlan& g = m->grid;
     PcgScalars* sc = ws.pscal.as<PcgScalars>();
     spmv<VT>(m, ws.p.as<VT>(), ws.t.as<VT>(), s);
-    scatter<VT, VT>(m, ws.t.as<VT>(), nullptr, ws.q.as<VT>(), s);
+    scatter<VT, VT, VT>(m, ws.t.as<VT>(), nullptr, ws.q.as<VT>(), s);
     hipLaunchKernelGGL(k_pcg_alpha<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.p.as<VT>(), ws.q.as<VT>(), g.col_sys(), sc);
     hipLaunchKernelGGL(k_pcg_update<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
@@ -560,8 +614,7 @@ static void pcg_solve(cad_matrix* m, const double* b, double* x, const cad_solve
     }
     hipLaunchKernelGGL((k_convert<VT, double>), dim3(grid_for(R)), dim3(256), 0, s, ws.bvt.as<VT>(), b, R);
     // rhs = A^T b into rr (x0 = 0 so r = rhs)
-    fill<VT>(ws.rr.as<VT>(), C, VT(0), s);
-    scatter<VT, VT>(m, ws.bvt.as<VT>(), nullptr, ws.rr.as<VT>(), s);
+    scatter<VT, VT, VT>(m, ws.bvt.as<VT>(), nullptr, ws.rr.as<VT>(), s);
     hipLaunchKernelGGL(k_pcg_init<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.P.as<VT>(), ws.rr.as<VT>(), ws.zz.as<VT>(), ws.p.as<VT>(), ws.xr.as<VT>(), ws.q.as<VT>(),
                        g.col_sys(), ws.pscal.as<PcgScalars>());

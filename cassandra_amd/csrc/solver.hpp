@@ -12,7 +12,6 @@ struct Workspace {
     DBuf u, r64, v, w, y, x64, D, z64, vd, z, scal, scal_chk;
     DBuf P, p, q, zz, xr, rr, t, bvt, pscal, jac;
     int vd_prec = 0;
-    bool z64_clear = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_iters = 0, graph_prec = 0;
@@ -29,5 +28,9 @@ const uint32_t* jacobi_counts(cad_matrix* m);
 void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep);
 void lsmr_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep);
 double rel_normal_residual(cad_matrix* m, const double* b, const double* x, double* rnorm);
+void matrix_targets(cad_matrix* m, const double* xt, uint64_t seed, double sn, double* b);
+void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
+                        const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
+                        int64_t cols_per_sys, int ibytes, const Context& ctx);
 
 }  // namespace cad
