@@ -192,6 +192,13 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
 
 }  // namespace
 
+// Dynamic LDS a window may use: the device limit minus 1 KB kept for the
+// kernels' static LDS (block-reduction scratch).
+size_t lds_budget(const Context& ctx)
+{
+    return std::min<size_t>(ctx.max_lds, 160 * 1024) - 1024;
+}
+
 GridSys GatherPlan::sys() const
 {
     return GridSys{d_first.as<int32_t>(), d_count.as<int32_t>(), d_counters.as<unsigned>(), d_partials.as<double>()};
@@ -202,7 +209,7 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
                         int64_t cols_per_sys, int ibytes, const Context& ctx)
 {
     make_groups(lo, hi, sp);
-    const size_t lds = std::min<size_t>(ctx.max_lds, 160 * 1024);
+    const size_t lds = lds_budget(ctx);
     make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 4), 4, ibytes, ctx.cus, sp.f4);
     make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 8), 8, ibytes, ctx.cus, sp.f8);
     sp.d_groups = to_device(sp.groups, ctx.stream);
@@ -224,7 +231,7 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
     const ScatterPlan& sp = m->splan;
     gp.ok = false;
     gp.rpt = rpt;
-    const size_t lds = std::min<size_t>(ctx.max_lds, 160 * 1024);
+    const size_t lds = lds_budget(ctx);
     gp.wmax = uint32_t(lds / vbytes);
     std::vector<const SlotGroup*> order;
     for (auto& g : sp.groups) order.push_back(&g);

@@ -1,0 +1,294 @@
+// The reference's own MathTest / PatternTest cases, run against the C++
+// drop-in layer (cassandra_amd/cpp/include, the reference's API) -- plus fit
+// checks that need the GPU.  `dropin_test host` runs the cases that need no
+// device, `dropin_test gpu` the rest.  Exit status 0 when every case passed.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <random>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "Math/Math.h"
+#include "Pattern/Evaluator.h"
+#include "Pattern/Helpers.h"
+#include "Pattern/Indexer.h"
+#include "Pattern/Names.h"
+#include "PatternFit/PatternFit.h"
+
+static int failures = 0;
+#define CHECK(cond)                                                                 \
+    do {                                                                            \
+        if (!(cond)) {                                                              \
+            std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);         \
+            failures++;                                                             \
+            return;                                                                 \
+        }                                                                           \
+    } while (0)
+
+struct Case {
+    const char* name;
+    bool gpu;
+    std::function<void()> fn;
+};
+
+static std::vector<Case>& cases()
+{
+    static std::vector<Case> c;
+    return c;
+}
+#define TEST_CASE(name, gpu) \
+    static void name();      \
+    static bool reg_##name = (cases().push_back({#name, gpu, name}), true); \
+    static void name()
+
+// ---- PatternTest/Pattern.h:4-107 test masks and symmetric variants
+static const std::vector<uint64_t> test_masks{0x3, 0x8040, 0x102, 0x81, 0x0100000000000001ULL,
+                                              0x4200000000000042ULL, 0x4080000000000102ULL, 0x8100000000000081ULL};
+
+static std::vector<Position> variants(Position p)
+{
+    return {p,
+            FlippedCodiagonal(p),
+            FlippedDiagonal(p),
+            FlippedHorizontal(p),
+            FlippedVertical(p),
+            FlippedCodiagonal(FlippedHorizontal(p)),
+            FlippedDiagonal(FlippedHorizontal(p)),
+            FlippedVertical(FlippedHorizontal(p))};
+}
+
+static std::vector<Position> random_positions(unsigned seed, int n)
+{
+    std::mt19937_64 eng(seed);
+    std::vector<Position> out;
+    for (int i = 0; i < n; i++) {
+        uint64_t a = eng(), b = eng();
+        out.emplace_back(a & ~b, ~a & b);
+    }
+    return out;
+}
+
+// ---- MathTest/Vector.cpp
+TEST_CASE(vector_known_answers, false)
+{
+    CHECK((Vector(3, 1.0f) + Vector(3, 2.0f)) == Vector(3, 3.0f));
+    CHECK((Vector(3, 2.0f) - Vector(3, 1.0f)) == Vector(3, 1.0f));
+    CHECK((Vector(3, 2.0f) * 3.0f) == Vector(3, 6.0f));
+    CHECK((Vector(3, 6.0f) / 2.0f) == Vector(3, 3.0f));
+    CHECK(elementwise_multiplication(Vector(3, 2.0f), Vector(3, 3.0f)) == Vector(3, 6.0f));
+    CHECK(elementwise_division(Vector(3, 6.0f), Vector(3, 2.0f)) == Vector(3, 3.0f));
+    CHECK(inv(Vector{1.0f, 2.0f, 4.0f}) == (Vector{1.0f, 0.5f, 0.25f}));
+    CHECK(dot(Vector(3, 2.0f), Vector(3, 3.0f)) == 18.0f);
+    CHECK(L1_norm(Vector(3, -2.0f)) == 6.0f);
+    CHECK(L2_norm(Vector{3.0f, 4.0f}) == 5.0f);
+    CHECK(sum(Vector(3, 2.0f)) == 6.0f);
+    bool threw = false;
+    try {
+        Vector v(3, 2.0f);
+        v += Vector(4, 3.0f);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+// ---- BoardTest/BitBoard.cpp:8-23
+TEST_CASE(flip_known_answers, false)
+{
+    CHECK(FlippedCodiagonal(0xFULL) == 0x8080808000000000ULL);
+    CHECK(FlippedDiagonal(0xFULL) == 0x01010101ULL);
+    CHECK(FlippedHorizontal(0xFULL) == 0xF0ULL);
+    CHECK(FlippedVertical(0xFULL) == 0x0F00000000000000ULL);
+    CHECK(PossibleMoves(Position::Start()) == 0x0000102004080000ULL);
+}
+
+// ---- PatternTest/Indexer.cpp:8-47
+TEST_CASE(indexer_symmetry_independence, false)
+{
+    auto check = [](auto& ix) {
+        for (Position pos : random_positions(1337, 10000)) {
+            auto base = ix.Indices(pos);
+            std::sort(base.begin(), base.end());
+            for (Position v : variants(pos)) {
+                auto vi = ix.Indices(v);
+                std::sort(vi.begin(), vi.end());
+                if (vi != base) return false;
+            }
+        }
+        return true;
+    };
+    for (uint64_t m : test_masks) {
+        auto ix = CreateIndexer(m);
+        CHECK(check(*ix));
+    }
+    GroupIndexer g(test_masks);
+    CHECK(check(g));
+}
+
+// ---- PatternTest/Indexer.cpp:51-92
+TEST_CASE(indexer_covers_index_space, false)
+{
+    for (uint64_t m : test_masks) {
+        auto ix = CreateIndexer(m);
+        std::set<uint32_t> seen;
+        for (Position c : Configurations(m))
+            for (uint32_t i : ix->Indices(c)) seen.insert(i);
+        for (int i = 0; i < ix->index_space_size; i++) CHECK(seen.count(uint32_t(i)) == 1);
+    }
+    CHECK(ConfigurationCount(pattern::edax) == 114363);
+    GroupIndexer g(pattern::edax);
+    CHECK(g.index_space_size == 114363 && g.Variations().size() == 46);
+}
+
+// ---- PatternTest/Evaluator.cpp:7-31
+TEST_CASE(score_estimator_round_trip_and_symmetry, false)
+{
+    for (uint64_t m : test_masks) {
+        std::vector<float> w(ConfigurationCount(m));
+        for (std::size_t i = 0; i < w.size(); i++) w[i] = float(i + 1);
+        ScoreEstimator est({m}, w);
+        CHECK(est.Weights() == w);
+        for (Position c : Configurations(m)) {
+            const float base = est.Eval(c);
+            for (Position v : variants(c)) CHECK(est.Eval(v) == base);
+        }
+    }
+}
+
+// ---- MathTest/MatrixCSR.cpp:3-54 (products on the GPU)
+static MatrixCSR kat_matrix()
+{
+    MatrixCSR m(2, 3, 4);
+    m.Row(0)[0] = 0;
+    m.Row(0)[1] = 1;
+    m.Row(1)[0] = 1;
+    m.Row(1)[1] = 2;
+    m.Row(2)[0] = 3;
+    m.Row(2)[1] = 3;
+    return m;
+}
+
+TEST_CASE(matrix_csr_known_answers, true)
+{
+    MatrixCSR m = kat_matrix();
+    CHECK(m.Rows() == 3 && m.Cols() == 4 && m.Row(0).size() == 2);
+    CHECK((m * Vector{1, 2, 3, 4}) == (Vector{3, 5, 8}));
+    CHECK((Vector{1, 2, 3} * m) == (Vector{1, 3, 2, 6}));
+    CHECK(JacobiPreconditionerOfATA(m) == (Vector{1, 0.5f, 1, 0.25f}));
+    // rows changed after a product: the device copy follows
+    m.Row(2)[1] = 0;
+    CHECK((m * Vector{1, 2, 3, 4}) == (Vector{3, 5, 5}));
+    bool threw = false;
+    try {
+        m* Vector(3, 1.0f);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+static std::vector<ScoredPosition> scored(int n, uint64_t seed)
+{
+    std::mt19937_64 eng(seed);
+    std::vector<ScoredPosition> out;
+    // random-play like positions: use the start position plus random legal moves
+    for (int i = 0; i < n; i++) {
+        Position p = Position::Start();
+        for (int ply = 0; ply < 30; ply++) {
+            uint64_t mv = PossibleMoves(p);
+            if (!mv) {
+                Position q(p.Opponent(), p.Player());
+                if (!PossibleMoves(q)) break;
+                p = q;
+                continue;
+            }
+            int r = int(eng() % uint64_t(std::popcount(mv)));
+            for (int j = 0; j < r; j++) mv &= mv - 1;
+            const int m = std::countr_zero(mv);
+            const uint64_t f = Flips(p, m);
+            p = Position(p.Opponent() ^ f, p.Player() ^ f ^ (1ULL << m));
+        }
+        out.push_back({p, Score(int(eng() % 65) - 32)});
+    }
+    return out;
+}
+
+// FitWeights on the GPU against the reference-structured PCG of Solver.h run
+// on host vectors (each product on the GPU): same algorithm, same float type.
+TEST_CASE(fit_weights_against_host_pcg, true)
+{
+    const std::vector<uint64_t> pat{pattern::L02X, pattern::Q3, pattern::D8};
+    auto sp = scored(20000, 7);
+    Vector w = FitWeights(pat, sp, 10);
+    GroupIndexer g(pat);
+    MatrixCSR A(g.Variations().size(), sp.size(), std::size_t(g.index_space_size));
+    for (std::size_t i = 0; i < sp.size(); i++) g.InsertIndices(sp[i].pos, A.Row(i));
+    Vector b(sp.size());
+    for (std::size_t i = 0; i < sp.size(); i++) b[i] = float(int8_t(sp[i].score));
+    DiagonalPreconditioner P(JacobiPreconditionerOfATA(A));
+    PCG solver(transposed(A) * A, P, Vector(A.Cols(), 0.0f), transposed(A) * b);
+    solver.Iterate(10);
+    const Vector pg = A * w, ph = A * solver.X();
+    double num = 0, den = 0;
+    for (std::size_t i = 0; i < pg.size(); i++) {
+        num += double(pg[i] - ph[i]) * (pg[i] - ph[i]);
+        den += double(ph[i]) * ph[i];
+    }
+    CHECK(std::sqrt(num / den) < 1e-4);
+}
+
+TEST_CASE(eval_batch_matches_eval, true)
+{
+    const std::vector<uint64_t> pat{pattern::L02X, pattern::B5, pattern::Q3, pattern::D8};
+    std::vector<float> w(ConfigurationCount(pat));
+    std::mt19937 eng(3);
+    std::normal_distribution<float> nd;
+    for (auto& v : w) v = nd(eng);
+    ScoreEstimator est(pat, w);
+    auto sp = scored(3000, 11);
+    std::vector<Position> pos;
+    for (auto& s : sp) pos.push_back(s.pos);
+    const auto batch = est.EvalBatch(pos);
+    for (std::size_t i = 0; i < pos.size(); i++) CHECK(batch[i] == est.Eval(pos[i]));
+}
+
+TEST_CASE(device_lsqr_reaches_tolerance, true)
+{
+    const std::vector<uint64_t> pat{pattern::L02X, pattern::B5, pattern::Q3, pattern::D8};
+    auto sp = scored(30000, 5);
+    GroupIndexer g(pat);
+    MatrixCSR A(g.Variations().size(), sp.size(), std::size_t(g.index_space_size));
+    for (std::size_t i = 0; i < sp.size(); i++) g.InsertIndices(sp[i].pos, A.Row(i));
+    std::vector<double> b(sp.size());
+    for (std::size_t i = 0; i < sp.size(); i++) b[i] = double(int8_t(sp[i].score));
+    DeviceSolveReport rep;
+    auto x = SolveOnDevice(A, b, {}, &rep);
+    CHECK(rep.converged && rep.rel_normal_residual <= 1e-10 && x.size() == A.Cols());
+    // the FitStages loop over empty-count stages
+    PatternBasedEstimator est(10, pat);
+    FitStages(est, sp, 5);
+    CHECK(est.Stages() == 7);
+}
+
+int main(int argc, char** argv)
+{
+    const std::string mode = argc > 1 ? argv[1] : "host";
+    int run = 0;
+    for (auto& c : cases()) {
+        if (c.gpu != (mode == "gpu")) continue;
+        const int before = failures;
+        try {
+            c.fn();
+        } catch (const std::exception& e) {
+            std::printf("  FAILED %s: exception %s\n", c.name, e.what());
+            failures++;
+        }
+        std::printf("%s %s\n", failures == before ? "ok  " : "FAIL", c.name);
+        std::fflush(stdout);
+        run++;
+    }
+    std::printf("%d cases, %d failures\n", run, failures);
+    return failures ? 1 : 0;
+}
