@@ -74,6 +74,8 @@ SIGNATURES = {
     "cad_patterns_indices_host": (C.c_int, [vp, vp, i64, vp]),
     "cad_index_extract": (C.c_int, [vp, vp, i64, vp, C.c_int]),
     "cad_matrix_from_positions": (C.c_int, [vp, vp, i64, PP]),
+    "cad_matrix_from_positions_systems": (C.c_int, [vp, vp, vp, C.c_int, PP]),
+    "cad_matrix_systems": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(i64), vp]),
     "cad_matrix_create": (C.c_int, [vp, vp, i64, C.c_int, i64, PP]),
     "cad_matrix_destroy": (C.c_int, [vp]),
     "cad_matrix_info": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_int), C.POINTER(i64), C.POINTER(C.c_int)]),
@@ -85,6 +87,7 @@ SIGNATURES = {
     "cad_jacobi": (C.c_int, [vp, vp]),
     "cad_solve_options_default": (None, [C.POINTER(SolveOptions)]),
     "cad_solve": (C.c_int, [vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveReport)]),
+    "cad_solve_systems": (C.c_int, [vp, vp, vp, C.POINTER(SolveOptions), C.POINTER(SolveReport), vp, vp]),
     "cad_rel_normal_residual": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "cad_eval_forward_f32": (C.c_int, [vp, vp, i64, vp, vp]),
     "cad_eval_forward_f64": (C.c_int, [vp, vp, i64, vp, vp]),

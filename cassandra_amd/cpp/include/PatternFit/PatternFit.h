@@ -1,7 +1,8 @@
 // Drop-in for PatternFit/PatternFit.cpp:7-39 (FitWeights, which the reference
 // does not declare in its header) plus the per-stage fit of its stage loop
-// (PatternFit/PatternFit.cpp:77-107) without the search that rescores the
-// positions.  Both run on the GPU.
+// EvaluateIteratively (PatternFit/PatternFit.cpp:40-84) without the search
+// that rescores the positions (with every position already scored and
+// reevaluate = false the loop is exactly FitStages).  Both run on the GPU.
 #pragma once
 #include <span>
 #include <vector>
@@ -15,5 +16,8 @@
 Vector FitWeights(const std::vector<uint64_t>& pattern, std::span<ScoredPosition> scored_pos, int iterations);
 
 // Fits every empty-count stage of a MultiStageScoreEstimator (positions with
-// EmptyCount in [stage*stage_size, stage*stage_size + stage_size - 1]).
+// EmptyCount in [stage*stage_size, stage*stage_size + stage_size - 1]) with
+// FitWeights' solver, all stages in one batched solve (per-stage scalars, so
+// each stage gets the fit FitWeights gives it alone).  A stage without
+// positions gets zero weights.
 void FitStages(PatternBasedEstimator& estimator, std::span<const ScoredPosition> scored_pos, int iterations);

@@ -368,26 +368,71 @@ int cad_index_extract(cad_patterns* p, const uint64_t* d_pos, int64_t n, uint32_
 }
 
 // ------------------------------------------------------------ matrices ----
+namespace {
+
+// Index matrix of positions straight from the device; row_begin (nsys + 1
+// entries) splits the rows into independent systems, each with its own copy of
+// the pattern group's columns (system s owns columns [s * cols, (s + 1) * cols)).
+cad_matrix* matrix_from_positions(cad_patterns* p, const uint64_t* d_pos, const std::vector<int64_t>& row_begin)
+{
+    need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+    const PatternGroup& g = *p->group;
+    const int nsys = int(row_begin.size()) - 1;
+    const int64_t n = row_begin.back();
+    const int64_t cols_per = int64_t(g.cols());
+    need(cols_per * nsys <= int64_t(0xFFFFFFFFu), "more than 2^32 columns over all systems");
+    std::unique_ptr<cad_matrix> m(new_matrix(p->ctx, n, g.k(), cols_per * nsys));
+    m->slot_lo = p->slot_lo;
+    m->slot_hi = p->slot_hi;
+    uint32_t widest = 0;
+    for (size_t s = 0; s < m->slot_lo.size(); s++) widest = std::max(widest, m->slot_hi[s] - m->slot_lo[s]);
+    m->ibytes = widest <= 65536 ? 2 : 4;
+    m->d_idx = DBuf(size_t(m->ibytes) * size_t(n) * size_t(g.k()));
+    std::vector<uint32_t> base(g.k());
+    for (int s = 0; s < g.k(); s++) base[s] = m->ibytes == 2 ? m->slot_lo[s] : 0u;
+    m->d_base = to_device(base, p->ctx->stream);
+    extract(p, d_pos, n, m->d_idx.p, m->ibytes, n, 1, m->ibytes == 2 ? 1 : 0);
+    m->nsys = nsys;
+    m->sys_row_begin = row_begin;
+    m->cols_per_sys = cols_per;
+    m->sys_col.resize(nsys);
+    for (int s = 0; s < nsys; s++) m->sys_col[s] = uint32_t(cols_per * s);
+    build_plans(m.get());
+    CAD_HIP(hipStreamSynchronize(p->ctx->stream));
+    return m.release();
+}
+
+}  // namespace
+
 int cad_matrix_from_positions(cad_patterns* p, const uint64_t* d_pos, int64_t n, cad_matrix** out)
 {
     return guarded([&] {
         need(p && out && n > 0 && d_pos, "invalid argument");
-        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
-        const PatternGroup& g = *p->group;
-        std::unique_ptr<cad_matrix> m(new_matrix(p->ctx, n, g.k(), int64_t(g.cols())));
-        m->slot_lo = p->slot_lo;
-        m->slot_hi = p->slot_hi;
-        uint32_t widest = 0;
-        for (size_t s = 0; s < m->slot_lo.size(); s++) widest = std::max(widest, m->slot_hi[s] - m->slot_lo[s]);
-        m->ibytes = widest <= 65536 ? 2 : 4;
-        m->d_idx = DBuf(size_t(m->ibytes) * size_t(n) * size_t(g.k()));
-        std::vector<uint32_t> base(g.k());
-        for (int s = 0; s < g.k(); s++) base[s] = m->ibytes == 2 ? m->slot_lo[s] : 0u;
-        m->d_base = to_device(base, p->ctx->stream);
-        extract(p, d_pos, n, m->d_idx.p, m->ibytes, n, 1, m->ibytes == 2 ? 1 : 0);
-        build_plans(m.get());
-        CAD_HIP(hipStreamSynchronize(p->ctx->stream));
-        *out = m.release();
+        *out = matrix_from_positions(p, d_pos, {0, n});
+    });
+}
+
+int cad_matrix_from_positions_systems(cad_patterns* p, const uint64_t* d_pos, const int64_t* row_begin, int nsys,
+                                      cad_matrix** out)
+{
+    return guarded([&] {
+        need(p && out && d_pos && row_begin && nsys >= 1, "invalid argument");
+        need(row_begin[0] == 0, "row_begin[0] must be 0");
+        std::vector<int64_t> rb(row_begin, row_begin + nsys + 1);
+        for (int s = 0; s < nsys; s++)
+            need(rb[s + 1] > rb[s], "every system needs at least one row", CAD_ERR_SIZE_MISMATCH);
+        *out = matrix_from_positions(p, d_pos, rb);
+    });
+}
+
+int cad_matrix_systems(const cad_matrix* m, int* nsys, int64_t* cols_per_system, int64_t* row_begin)
+{
+    return guarded([&] {
+        need(m != nullptr, "null matrix");
+        if (nsys) *nsys = m->nsys;
+        if (cols_per_system) *cols_per_system = m->cols_per_sys;
+        if (row_begin)
+            for (int s = 0; s <= m->nsys; s++) row_begin[s] = m->sys_row_begin[s];
     });
 }
 
@@ -461,18 +506,22 @@ int cad_matrix_download(const cad_matrix* m, uint32_t* out)
         std::vector<uint32_t> base(k);
         hipStream_t s = m->ctx->stream;
         CAD_HIP(hipMemcpyAsync(base.data(), m->d_base.p, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, s));
+        // global column = system base + slot base + stored entry
+        std::vector<uint32_t> rowbase(size_t(std::max<int64_t>(R, 0)), 0u);
+        for (int sy = 0; sy < m->nsys; sy++)
+            for (int64_t r = m->sys_row_begin[sy]; r < m->sys_row_begin[sy + 1]; r++) rowbase[r] = m->sys_col[sy];
         if (m->ibytes == 2) {
             std::vector<uint16_t> t(size_t(R) * k);
             CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 2, hipMemcpyDeviceToHost, s));
             CAD_HIP(hipStreamSynchronize(s));
             for (int64_t r = 0; r < R; r++)
-                for (int j = 0; j < k; j++) out[r * k + j] = base[j] + t[size_t(j) * R + r];
+                for (int j = 0; j < k; j++) out[r * k + j] = rowbase[r] + base[j] + t[size_t(j) * R + r];
         } else {
             std::vector<uint32_t> t(size_t(R) * k);
             CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 4, hipMemcpyDeviceToHost, s));
             CAD_HIP(hipStreamSynchronize(s));
             for (int64_t r = 0; r < R; r++)
-                for (int j = 0; j < k; j++) out[r * k + j] = base[j] + t[size_t(j) * R + r];
+                for (int j = 0; j < k; j++) out[r * k + j] = rowbase[r] + base[j] + t[size_t(j) * R + r];
         }
     });
 }
@@ -534,18 +583,45 @@ void cad_solve_options_default(cad_solve_options* o)
     o->verbose = 0;
 }
 
+namespace {
+
+cad_solve_options checked_options(const cad_solve_options* opt)
+{
+    cad_solve_options o;
+    if (opt) o = *opt;
+    else cad_solve_options_default(&o);
+    need(o.precision >= 0 && o.precision <= 2, "unknown precision");
+    need(!(o.precision == CAD_PREC_MIXED && o.solver != CAD_SOLVER_LSQR), "mixed precision is LSQR only");
+    return o;
+}
+
+}  // namespace
+
 int cad_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options* opt, cad_solve_report* rep)
 {
     return guarded([&] {
         need(m && b && x, "null argument");
-        cad_solve_options o;
-        if (opt) o = *opt;
-        else cad_solve_options_default(&o);
-        need(o.precision >= 0 && o.precision <= 2, "unknown precision");
-        need(!(o.precision == CAD_PREC_MIXED && o.solver != CAD_SOLVER_LSQR), "mixed precision is LSQR only");
+        const cad_solve_options o = checked_options(opt);
         cad_solve_report r;
         solve(m, b, x, o, r);
         if (rep) *rep = r;
+    });
+}
+
+int cad_solve_systems(cad_matrix* m, const double* b, double* x, const cad_solve_options* opt,
+                      cad_solve_report* rep, double* rel_per_system, int* iterations_per_system)
+{
+    return guarded([&] {
+        need(m && b && x, "null argument");
+        const cad_solve_options o = checked_options(opt);
+        cad_solve_report r;
+        SystemDetail det;
+        solve(m, b, x, o, r, &det);
+        if (rep) *rep = r;
+        for (int s = 0; s < m->nsys; s++) {
+            if (rel_per_system) rel_per_system[s] = det.rel[s];
+            if (iterations_per_system) iterations_per_system[s] = det.iters[s];
+        }
     });
 }
 

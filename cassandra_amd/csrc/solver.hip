@@ -404,7 +404,8 @@ static void lsqr_alloc(cad_matrix* m, Workspace& ws, hipStream_t s)
 // LSQR in one precision (FP64 or FP32 products and vectors) with column
 // scaling; stops on the explicit FP64 metric.
 template <class VT>
-static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep)
+static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
+                       SystemDetail& det)
 {
     hipStream_t s = m->ctx->stream;
     Workspace& ws = workspace(m, 100 + int(sizeof(VT)));
@@ -419,6 +420,7 @@ static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solv
 
     const int batch = std::max(1, o.check_every);
     std::vector<double> trigger(m->nsys, 1.0), nres, rn;
+    std::vector<int> frozen_at(m->nsys, -1);
     int it = 0;
     bool done = (o.max_iterations <= 0);
     while (!done) {
@@ -460,6 +462,7 @@ static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solv
                 // freeze converged systems
                 for (int i = 0; i < m->nsys; i++)
                     if (!still[i] && h[i].active) {
+                        frozen_at[i] = it;
                         LsqrScalars t = h[i];
                         t.active = 0;
                         CAD_HIP(hipMemcpyAsync(ws.scal.as<LsqrScalars>() + i, &t, sizeof(t), hipMemcpyHostToDevice, s));
@@ -475,8 +478,12 @@ static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solv
     CAD_HIP(hipMemcpyAsync(x, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
     rep.iterations = it;
     double worst = 0, rsum = 0;
+    det.rel.assign(m->nsys, 0.0);
+    det.iters.assign(m->nsys, it);
     for (int i = 0; i < m->nsys; i++) {
-        worst = std::max(worst, atb[i] > 0 ? nres[i] / atb[i] : nres[i]);
+        det.rel[i] = atb[i] > 0 ? nres[i] / atb[i] : nres[i];
+        if (frozen_at[i] >= 0) det.iters[i] = frozen_at[i];
+        worst = std::max(worst, det.rel[i]);
         rsum += rn[i] * rn[i];
     }
     rep.rel_normal_residual = worst;
@@ -486,7 +493,8 @@ static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solv
 
 // Mixed precision with iterative refinement: FP32 products inside LSQR,
 // FP64 residual r = b - A x and A^T r outside.
-static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep)
+static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
+                       SystemDetail& det)
 {
     hipStream_t s = m->ctx->stream;
     Workspace& ws = workspace(m, 200);
@@ -496,6 +504,7 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
     fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
     std::vector<int> act(m->nsys, 1);
     std::vector<double> rel(m->nsys, 1.0), rn2(m->nsys, 0.0), target(m->nsys, o.inner_tol);
+    det.iters.assign(m->nsys, -1);
     const int batch = std::max(1, o.check_every);
     int it = 0;
     for (int ref = 0;; ref++) {
@@ -509,6 +518,7 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
             rn2[i] = h[i].reduce_a;
             if (o.tol > 0 && rel[i] <= o.tol) {
                 act[i] = 0;
+                det.iters[i] = it;
             } else {
                 any = true;
                 // the last refinement only needs to close the remaining gap
@@ -556,7 +566,9 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
     for (int i = 0; i < m->nsys; i++) {
         worst = std::max(worst, rel[i]);
         rsum += rn2[i];
+        if (det.iters[i] < 0) det.iters[i] = it;
     }
+    det.rel = rel;
     rep.iterations = it;
     rep.rel_normal_residual = worst;
     rep.residual_norm = std::sqrt(rsum);
@@ -583,7 +595,7 @@ static void pcg_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
 
 template <class VT>
 static void pcg_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
-                      bool jacobi)
+                      bool jacobi, SystemDetail& det)
 {
     hipStream_t s = m->ctx->stream;
     Workspace& ws = workspace(m, 300 + int(sizeof(VT)) + (jacobi ? 0 : 50));
@@ -636,8 +648,11 @@ static void pcg_solve(cad_matrix* m, const double* b, double* x, const cad_solve
     std::vector<double> atb = atb_norms(m, ws, b, s), nres, rn;
     explicit_residual(m, ws, x, b, nres, rn, s);
     double worst = 0, rsum = 0;
+    det.rel.assign(m->nsys, 0.0);
+    det.iters.assign(m->nsys, it);
     for (int i = 0; i < m->nsys; i++) {
-        worst = std::max(worst, atb[i] > 0 ? nres[i] / atb[i] : nres[i]);
+        det.rel[i] = atb[i] > 0 ? nres[i] / atb[i] : nres[i];
+        worst = std::max(worst, det.rel[i]);
         rsum += rn[i] * rn[i];
     }
     rep.iterations = it;
@@ -647,25 +662,30 @@ static void pcg_solve(cad_matrix* m, const double* b, double* x, const cad_solve
 }
 
 // ------------------------------------------------------------ entry -------
-void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep)
+void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
+           SystemDetail* detail)
 {
     std::memset(&rep, 0, sizeof(rep));
+    SystemDetail det;
     auto t0 = std::chrono::steady_clock::now();
     if (o.solver == CAD_SOLVER_LSQR) {
-        if (o.precision == CAD_PREC_MIXED) lsqr_mixed(m, b, x, o, rep);
-        else if (o.precision == CAD_PREC_FP32) lsqr_plain<float>(m, b, x, o, rep);
-        else lsqr_plain<double>(m, b, x, o, rep);
+        if (o.precision == CAD_PREC_MIXED) lsqr_mixed(m, b, x, o, rep, det);
+        else if (o.precision == CAD_PREC_FP32) lsqr_plain<float>(m, b, x, o, rep, det);
+        else lsqr_plain<double>(m, b, x, o, rep, det);
     } else if (o.solver == CAD_SOLVER_PCG || o.solver == CAD_SOLVER_CG) {
         const bool jac = o.solver == CAD_SOLVER_PCG;
-        if (o.precision == CAD_PREC_FP32) pcg_solve<float>(m, b, x, o, rep, jac);
-        else pcg_solve<double>(m, b, x, o, rep, jac);
+        if (o.precision == CAD_PREC_FP32) pcg_solve<float>(m, b, x, o, rep, jac, det);
+        else pcg_solve<double>(m, b, x, o, rep, jac, det);
     } else if (o.solver == CAD_SOLVER_LSMR) {
         lsmr_solve(m, b, x, o, rep);
+        det.rel.assign(1, rep.rel_normal_residual);
+        det.iters.assign(1, rep.iterations);
     } else {
         throw Error(CAD_ERR_ARGUMENT, "unknown solver");
     }
     CAD_HIP(hipStreamSynchronize(m->ctx->stream));
     rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (detail) *detail = std::move(det);
 }
 
 double rel_normal_residual(cad_matrix* m, const double* b, const double* x, double* rnorm)

@@ -25,7 +25,16 @@ void matrix_spmtv_f32(cad_matrix* m, const float* y, float* z);
 void matrix_spmtv_f64(cad_matrix* m, const double* y, double* z);
 void matrix_jacobi(cad_matrix* m, float* d);
 const uint32_t* jacobi_counts(cad_matrix* m);
-void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep);
+// Per-system outcome of a solve on a block-diagonal batch: the FP64 metric
+// |A_s^T(b_s - A_s x_s)| / |A_s^T b_s| at return and the iteration count at
+// which the system was frozen (converged) or the total.
+struct SystemDetail {
+    std::vector<double> rel;
+    std::vector<int> iters;
+};
+
+void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
+           SystemDetail* detail = nullptr);
 void lsmr_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep);
 double rel_normal_residual(cad_matrix* m, const double* b, const double* x, double* rnorm);
 void matrix_targets(cad_matrix* m, const double* xt, uint64_t seed, double sn, double* b);

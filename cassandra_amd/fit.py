@@ -146,6 +146,19 @@ class Patterns:
         call("cad_matrix_from_positions", self.h, _ptr(pos), pos.shape[0], C.byref(h))
         return Matrix(self.ctx, h)
 
+    def matrix_systems(self, pos: torch.Tensor, row_begin) -> "Matrix":
+        """Block-diagonal batch: rows [row_begin[s], row_begin[s+1]) of pos form
+        system s, which owns columns [s * cols, (s + 1) * cols) of x."""
+        _dev(pos, torch.int64)
+        rb = np.ascontiguousarray(np.asarray(row_begin, dtype=np.int64))
+        if rb.ndim != 1 or rb.shape[0] < 2 or int(rb[-1]) != pos.shape[0]:
+            raise ValueError("row_begin must hold nsys + 1 boundaries ending at the row count")
+        torch.cuda.synchronize(pos.device)
+        h = C.c_void_p()
+        call("cad_matrix_from_positions_systems", self.h, _ptr(pos), rb.ctypes.data_as(C.c_void_p),
+             rb.shape[0] - 1, C.byref(h))
+        return Matrix(self.ctx, h)
+
 
 class Matrix:
     """Device MatrixCSR (fixed k per row, implicit value 1, repeats allowed)."""
@@ -157,6 +170,12 @@ class Matrix:
         rows, k, cols, ib = C.c_int64(), C.c_int(), C.c_int64(), C.c_int()
         call("cad_matrix_info", h, C.byref(rows), C.byref(k), C.byref(cols), C.byref(ib))
         self.rows, self.k, self.cols, self.index_bytes = rows.value, k.value, cols.value, ib.value
+        ns, cps = C.c_int(), C.c_int64()
+        call("cad_matrix_systems", h, C.byref(ns), C.byref(cps), None)
+        self.nsys, self.cols_per_system = ns.value, cps.value
+        rb = np.zeros(self.nsys + 1, dtype=np.int64)
+        call("cad_matrix_systems", h, None, None, rb.ctypes.data_as(C.c_void_p))
+        self.row_begin = rb
 
     @classmethod
     def from_host(cls, ctx: Context, idx: np.ndarray, cols: int) -> "Matrix":
@@ -232,6 +251,22 @@ class Matrix:
         torch.cuda.synchronize(b.device)
         call("cad_solve", self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(rep))
         return x, rep.as_dict()
+
+    def solve_systems(self, b: torch.Tensor, **options):
+        """Solve every system of a batch in the same launches; returns (x,
+        report dict, per-system metric, per-system iterations)."""
+        _dev(b, torch.float64)
+        if b.shape[0] != self.rows:
+            raise ValueError("Size mismatch.")
+        x = self._vec(self.cols, torch.float64)
+        o = capi.default_options(**options)
+        rep = capi.SolveReport()
+        rel = np.zeros(self.nsys, dtype=np.float64)
+        its = np.zeros(self.nsys, dtype=np.int32)
+        torch.cuda.synchronize(b.device)
+        call("cad_solve_systems", self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(rep), rel.ctypes.data_as(C.c_void_p),
+             its.ctypes.data_as(C.c_void_p))
+        return x, rep.as_dict(), rel, its
 
     def rel_normal_residual(self, b: torch.Tensor, x: torch.Tensor):
         out, rn = C.c_double(), C.c_double()

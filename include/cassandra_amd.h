@@ -94,6 +94,19 @@ int cad_index_extract(cad_patterns* p, const uint64_t* d_positions, int64_t n, u
 /* Builds the device matrix of GroupIndexer rows for n device positions. */
 int cad_matrix_from_positions(cad_patterns* p, const uint64_t* d_positions, int64_t n,
                               cad_matrix** out);
+/* A block-diagonal batch of independent systems, one per empty-count stage
+ * (PatternFit/PatternFit.cpp:41-84 fits the stages one FitWeights call at a
+ * time; Pattern/Evaluator.cpp:77-88 selects a stage's weights by EmptyCount).
+ * Rows [row_begin[s], row_begin[s+1]) of the device positions form system s
+ * (host array of nsys + 1 entries, row_begin[0] = 0, every system >= 1 row);
+ * system s owns columns [s * cols, (s + 1) * cols) of x, cols being the pattern
+ * group's column count.  Every product and solver runs all systems in the same
+ * launches; solvers keep per-system scalars and stop each system on its own. */
+int cad_matrix_from_positions_systems(cad_patterns* p, const uint64_t* d_positions,
+                                      const int64_t* row_begin, int nsys, cad_matrix** out);
+/* nsys (1 for an ordinary matrix), columns per system and, when row_begin is
+ * not NULL, the nsys + 1 row boundaries. */
+int cad_matrix_systems(const cad_matrix* m, int* nsys, int64_t* cols_per_system, int64_t* row_begin);
 /* MatrixCSR(k, rows, cols) with Row(i) filled: row-major host indices. */
 int cad_matrix_create(cad_context* ctx, const uint32_t* h_idx, int64_t rows, int k, int64_t cols,
                       cad_matrix** out);
@@ -153,6 +166,12 @@ void cad_solve_options_default(cad_solve_options* o);
  * d_b: rows doubles; d_x: cols doubles (output). */
 int cad_solve(cad_matrix* m, const double* d_b, double* d_x, const cad_solve_options* opt,
               cad_solve_report* rep);
+/* cad_solve with the per-system outcome of a batch (host arrays of nsys
+ * entries, either may be NULL): the FP64 metric of each system at return and
+ * the iteration at which it converged (or the total).  rep holds the worst
+ * system's metric. */
+int cad_solve_systems(cad_matrix* m, const double* d_b, double* d_x, const cad_solve_options* opt,
+                      cad_solve_report* rep, double* rel_per_system, int* iterations_per_system);
 /* |A^T(b - A x)| / |A^T b| in FP64 on the device (the stopping metric). */
 int cad_rel_normal_residual(cad_matrix* m, const double* d_b, const double* d_x, double* out,
                             double* residual_norm);

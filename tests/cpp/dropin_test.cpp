@@ -287,6 +287,46 @@ TEST_CASE(device_lsqr_reaches_tolerance, true)
     CHECK(est.Stages() == 7);
 }
 
+// FitStages fits all stages in one batch; each stage must get the weights
+// FitWeights gives its positions alone (PatternFit/PatternFit.cpp:72-81).
+TEST_CASE(fit_stages_equals_stagewise_fit_weights, true)
+{
+    const std::vector<uint64_t> pat{pattern::L02X, pattern::Q3, pattern::D8};
+    auto sp = scored(12000, 9);
+    // spread the empty counts: play on from some of the positions
+    for (std::size_t i = 0; i < sp.size(); i += 3) {
+        Position p = sp[i].pos;
+        for (int ply = 0; ply < int(i % 17); ply++) {
+            uint64_t mv = PossibleMoves(p);
+            if (!mv) break;
+            const int m = std::countr_zero(mv);
+            const uint64_t f = Flips(p, m);
+            p = Position(p.Opponent() ^ f, p.Player() ^ f ^ (1ULL << m));
+        }
+        sp[i].pos = p;
+    }
+    PatternBasedEstimator est(5, pat);
+    FitStages(est, sp, 8);
+    int checked = 0;
+    for (int stage = 0; stage < est.Stages(); stage++) {
+        std::vector<ScoredPosition> part;
+        for (auto& s : sp)
+            if (s.EmptyCount() / 5 == stage) part.push_back(s);
+        if (part.size() < 200) continue;
+        const Vector w = FitWeights(pat, part, 8);
+        ScoreEstimator single(pat, w);
+        double num = 0, den = 0;
+        for (auto& s : part) {
+            const double a = est.score.estimators[stage].Eval(s.pos), b = single.Eval(s.pos);
+            num += (a - b) * (a - b);
+            den += b * b;
+        }
+        CHECK(std::sqrt(num / den) < 1e-4);
+        checked++;
+    }
+    CHECK(checked >= 2);
+}
+
 int main(int argc, char** argv)
 {
     const std::string mode = argc > 1 ? argv[1] : "host";
