@@ -14,6 +14,19 @@ time-to-fit on the same GPU, the SpMV / scatter kernel bandwidth against the
 8 TB/s HBM roofline (HIP events on the library's stream), and the CPU
 baseline (the oracle's FP64 LSQR on the host cores, a bounded sample of
 iterations scaled to the iteration count the fit needs).
+
+Three more workloads ride along in the same line (--legs):
+  buckets    50 independent systems, one per empty count 10..59, rows drawn
+             log-uniform in [20k, 800k] (seed 1003), pattern set {L02X, B5, Q3,
+             D8}; all fitted in the same launches (mixed LSQR to 1e-10 per
+             system) against the same 50 fits run one after another.
+  evaluator  the differentiable evaluator: 5M positions at 30 empties, pattern
+             set {L02X, B5, Q3, D8}, Huber(4) Adam on minibatches of 65,536;
+             positions/s of a training epoch and the gather / scatter kernel
+             bandwidth of one minibatch.
+  matrix_free 20M positions spread over 15..45 empties, 12 patterns (k = 54):
+             the mixed LSQR with indices recomputed from the positions in every
+             product against the stored-index FP64 LSQR, both to 1e-10.
 """
 from __future__ import annotations
 
@@ -91,6 +104,190 @@ def cpu_baseline(A, b, fp64_iterations, budget_s):
     return out
 
 
+def leg_buckets(ctx):
+    """50 empty-count systems in one batch against 50 separate fits."""
+    import numpy as np
+    import torch
+
+    import cassandra_amd.capi as capi
+    from cassandra_amd import names as N
+    from cassandra_amd.fit import Patterns
+
+    rng = np.random.default_rng(1003)
+    empties = list(range(10, 60))
+    rows = np.rint(np.exp(rng.uniform(np.log(20000), np.log(800000), size=len(empties)))).astype(np.int64)
+    P = Patterns(ctx, N.FIT4)
+    parts = [ctx.random_play(1003 + 7919 * i, int(n), e) for i, (e, n) in enumerate(zip(empties, rows))]
+    rb = np.concatenate([[0], np.cumsum(rows)])
+    A = P.matrix_systems(torch.cat(parts), rb)
+    _, b = A.make_targets(2003, 3003)
+    log(f"buckets: {A.nsys} systems, {A.rows} rows, k={A.k}, {A.cols} columns")
+    opts = dict(precision=capi.PREC_MIXED, tol=1e-10)
+    A.solve_systems(b, **opts)  # warm: workspace, graph
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    x, rep, rel, its = A.solve_systems(b, **opts)
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t0
+    log(f"buckets batched: {t_batch:.4f} s, {rep}")
+    # the same fits one system at a time (each warmed first)
+    mats = [P.matrix(p_) for p_ in parts]
+    del parts
+    bs = [b[int(rb[i]):int(rb[i + 1])].contiguous() for i in range(len(mats))]
+    for M, bi in zip(mats, bs):
+        M.solve(bi, **opts)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    seq_rel = []
+    for M, bi in zip(mats, bs):
+        _, r = M.solve(bi, **opts)
+        seq_rel.append(r["rel_normal_residual"])
+    ctx.synchronize()
+    t_seq = time.perf_counter() - t0
+    log(f"buckets one by one: {t_seq:.4f} s")
+    return {"systems": int(A.nsys), "rows": int(A.rows), "k": int(A.k), "cols_per_system": int(A.cols_per_system),
+            "seconds_batched": t_batch, "seconds_one_by_one": t_seq, "speedup": t_seq / t_batch,
+            "iterations": rep["iterations"], "converged": rep["converged"],
+            "max_rel_normal_residual": float(np.max(rel)), "median_iterations": float(np.median(its)),
+            "max_rel_one_by_one": float(np.max(seq_rel))}
+
+
+def leg_matrix_free(ctx):
+    """20M positions over 15..45 empties, 12 patterns: the matrix-free mixed
+    LSQR (indices recomputed in every product) against the stored-index FP64
+    LSQR on the same GPU."""
+    import ctypes as C_
+
+    import torch
+
+    import cassandra_amd.capi as capi
+    from cassandra_amd import names as N
+    from cassandra_amd.fit import Patterns
+
+    n = 20_000_000
+    P = Patterns(ctx, N.FIT12)
+    pos = ctx.random_play(1005, n, 15, 45)
+    F = P.matrix_free(pos)
+    _, b = F.make_targets(2005, 3005)
+    log(f"matrix-free: {F.rows} rows, k={F.k}, cols={F.cols}")
+    F.solve(b, precision=capi.PREC_MIXED, tol=1e-10)  # warm
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, rep_f = F.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
+    ctx.synchronize()
+    t_mf = time.perf_counter() - t0
+    log(f"matrix-free mixed: {t_mf:.3f} s {rep_f}")
+    L = capi.lib()
+    hp = lambda t_: C_.c_void_p(t_.data_ptr())  # noqa: E731
+    x32 = torch.randn(F.cols, dtype=torch.float32, device="cuda")
+    y32 = torch.empty(F.rows, dtype=torch.float32, device="cuda")
+    z32 = torch.empty(F.cols, dtype=torch.float32, device="cuda")
+    mf_spmv = event_time_ms(ctx, lambda: L.cad_spmv_f32(F.h, hp(x32), hp(y32)), 10)
+    mf_spmtv = event_time_ms(ctx, lambda: L.cad_spmtv_f32(F.h, hp(y32), hp(z32)), 10)
+    del F
+    S = P.matrix(pos)
+    del pos
+    S.solve(b, precision=capi.PREC_FP64, tol=1e-10)  # warm
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, rep_s = S.solve(b, precision=capi.PREC_FP64, tol=1e-10)
+    ctx.synchronize()
+    t_s = time.perf_counter() - t0
+    log(f"stored fp64: {t_s:.3f} s {rep_s}")
+    st_spmv = event_time_ms(ctx, lambda: L.cad_spmv_f32(S.h, hp(x32), hp(y32)), 10)
+    csr_u32 = S.rows * (4 * S.k + 4) + 4 * S.cols
+    return {"rows": S.rows, "k": S.k, "cols": S.cols, "seconds_matrix_free_mixed": t_mf,
+            "iterations_matrix_free": rep_f["iterations"], "rel_matrix_free": rep_f["rel_normal_residual"],
+            "seconds_stored_fp64": t_s, "iterations_stored_fp64": rep_s["iterations"],
+            "rel_stored_fp64": rep_s["rel_normal_residual"], "speedup": t_s / t_mf,
+            "mf_spmv_us": mf_spmv * 1e3, "mf_spmtv_us": mf_spmtv * 1e3, "stored_spmv_us": st_spmv * 1e3,
+            "mf_spmv_equiv_u32_csr_GBps": csr_u32 / (mf_spmv * 1e-3) / 1e9,
+            "index_bytes_stored": S.index_bytes}
+
+
+def leg_evaluator(ctx):
+    """Huber/Adam training throughput and the gather / scatter bandwidth."""
+    import torch
+
+    import cassandra_amd.capi as capi
+    from cassandra_amd import names as N
+    from cassandra_amd.evaluator import huber_fit, pattern_eval
+    from cassandra_amd.fit import Patterns
+
+    P = Patterns(ctx, N.FIT4)
+    n, B = 5_000_000, 65536
+    pos = ctx.random_play(1004, n, 30)
+    A = P.matrix(pos)
+    _, b = A.make_targets(2004, 3004)
+    del A
+    idx = P.extract(pos, capi.LAYOUT_ROW_MAJOR)
+    del pos
+    target = b.float()
+    huber_fit(P, idx[: 4 * B], target[: 4 * B], epochs=1, batch=B)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, losses = huber_fit(P, idx, target, epochs=1, batch=B)
+    torch.cuda.synchronize()
+    t_epoch = time.perf_counter() - t0
+    log(f"evaluator: epoch {t_epoch:.3f} s, loss {losses}")
+    # one minibatch: forward gather and backward scatter on torch's stream
+    w = torch.randn(P.cols, dtype=torch.float32, device="cuda", requires_grad=True)
+    sub = idx[:B].contiguous()
+    g = torch.randn(B, dtype=torch.float32, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+
+    def timed(fn):
+        fn()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    with torch.no_grad():
+        fwd_ms = timed(lambda: pattern_eval(w, sub, P))
+    out = pattern_eval(w, sub, P)
+    bwd_ms = timed(lambda: torch.autograd.grad(out, w, g, retain_graph=True))
+    nbytes = B * (4 * P.k + 4) + 4 * P.cols
+    return {"positions": n, "minibatch": B, "epoch_seconds": t_epoch, "positions_per_s": n / t_epoch,
+            "loss_epoch": losses[0], "forward_us": fwd_ms * 1e3, "backward_us": bwd_ms * 1e3,
+            "forward_GBps": nbytes / (fwd_ms * 1e-3) / 1e9, "backward_GBps": nbytes / (bwd_ms * 1e-3) / 1e9,
+            "bytes_per_minibatch": nbytes}
+
+
+def traffic_launches(ctx, A):
+    """Three launches each of the FP32 A x and A^T y products, then three
+    elementwise copies of 1 GiB (the scale for the FETCH_SIZE / WRITE_SIZE
+    counters)."""
+    import ctypes as C_
+
+    import torch
+
+    import cassandra_amd.capi as capi
+
+    L = capi.lib()
+    x32 = torch.randn(A.cols, dtype=torch.float32, device="cuda")
+    y32 = torch.empty(A.rows, dtype=torch.float32, device="cuda")
+    u32 = torch.randn(A.rows, dtype=torch.float32, device="cuda")
+    z32 = torch.empty(A.cols, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    hp = lambda t_: C_.c_void_p(t_.data_ptr())  # noqa: E731
+    for _ in range(3):
+        capi.check(L.cad_spmv_f32(A.h, hp(x32), hp(y32)))
+    for _ in range(3):
+        capi.check(L.cad_spmtv_f32(A.h, hp(u32), hp(z32)))
+    ctx.synchronize()
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads and writes exactly 1 GiB
+    torch.cuda.synchronize()
+    log(f"traffic launches done: A x and A^T y on {A.rows}x{A.cols} (k={A.k}, {A.index_bytes} B/index), "
+        f"copy of {src.numel() * 4} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +299,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--legs", default="buckets,evaluator,matrix_free",
+                    help="extra workloads: buckets,evaluator,matrix_free (or empty)")
+    ap.add_argument("--traffic", action="store_true",
+                    help="only launch the measured kernels a few times and a 1 GiB device copy (for counter runs)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -124,6 +325,10 @@ def main():
     xt, b = A.make_targets(2002, 3002)
     del pos
     log(f"data: {A.rows} rows, k={A.k}, cols={A.cols}, index bytes {A.index_bytes} ({time.perf_counter() - t:.1f} s)")
+    if args.traffic:
+        traffic_launches(ctx, A)
+        ctx.close()
+        return
 
     opts = dict(precision=capi.PREC_MIXED, tol=1e-10, verbose=int(args.verbose))
     for i in range(args.warmup):
@@ -165,6 +370,7 @@ def main():
     spmtv_ms = event_time_ms(ctx, lambda: L.cad_spmtv_f32(A.h, hp(u32), hp(z32)), args.reps)
     ib = A.index_bytes
     bytes_pass = A.rows * (ib * A.k + 4) + 4 * A.cols  # index stream + row vector + column vector
+    bytes_u32 = A.rows * (4 * A.k + 4) + 4 * A.cols
     spmv_gbps = bytes_pass / (spmv_ms * 1e-3) / 1e9
     spmtv_gbps = bytes_pass / (spmtv_ms * 1e-3) / 1e9
     log(f"spmv {spmv_ms * 1e3:.1f} us {spmv_gbps:.0f} GB/s; spmtv {spmtv_ms * 1e3:.1f} us {spmtv_gbps:.0f} GB/s")
@@ -183,24 +389,36 @@ def main():
     if not args.no_cpu and fp64 is not None:
         cpu = cpu_baseline(A, b, fp64["iterations"], args.cpu_budget)
         log(f"cpu: {cpu}")
+    shape = dict(rows=A.rows, k=A.k, cols=A.cols)
+    del A, b, x
+    legs = set(args.legs.split(",")) if args.legs else set()
+    buckets = leg_buckets(ctx) if "buckets" in legs else None
+    evaluator = leg_evaluator(ctx) if "evaluator" in legs else None
+    matrix_free = leg_matrix_free(ctx) if "matrix_free" in legs else None
 
     out = {
         "metric": METRIC, "value": t_mixed, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": t_mixed * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 products, fp64 recurrences", "data": "synthetic (random legal play, seeded)",
-        "config": {"workload": f"LSQR fit, {A.rows} random-play positions at {args.empties} empties, "
-                               f"8 patterns (k={A.k}, cols={A.cols}), tol 1e-10", "rows": A.rows, "k": A.k,
-                   "cols": A.cols, "parallelism": "single GPU"},
+        "config": {"workload": f"LSQR fit, {shape['rows']} random-play positions at {args.empties} empties, "
+                               f"8 patterns (k={shape['k']}, cols={shape['cols']}), tol 1e-10", **shape,
+                   "parallelism": "single GPU"},
         "mixed": {"iterations": reps[-1]["iterations"], "refinements": reps[-1]["refinements"],
                   "rel_normal_residual": reps[-1]["rel_normal_residual"]},
         "fp64_lsqr": None if fp64 is None else {"seconds": fp64["seconds"], "iterations": fp64["iterations"],
                                                  "rel_normal_residual": fp64["rel_normal_residual"]},
         "speedup_vs_fp64": None if fp64 is None else fp64["seconds"] / t_mixed,
         "kernels": {"spmv_us": spmv_ms * 1e3, "spmv_GBps": spmv_gbps, "spmtv_us": spmtv_ms * 1e3,
-                    "spmtv_GBps": spmtv_gbps, "bytes_per_pass": bytes_pass, "index_bytes": ib},
+                    "spmtv_GBps": spmtv_gbps, "bytes_per_pass": bytes_pass, "index_bytes": ib,
+                    # the same pass in the reference's u32 CSR layout, over the same time
+                    "spmv_equiv_u32_GBps": bytes_u32 / (spmv_ms * 1e-3) / 1e9,
+                    "spmtv_equiv_u32_GBps": bytes_u32 / (spmtv_ms * 1e-3) / 1e9},
         "roofline": {"kernel": dominant[0], "bound": "hbm", "achieved": dominant[2], "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": dominant[2] / HBM_PEAK_GBPS, "traffic": traffic},
         "cpu_baseline": cpu,
+        "buckets": buckets,
+        "evaluator": evaluator,
+        "matrix_free": matrix_free,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -425,6 +425,34 @@ int cad_matrix_from_positions_systems(cad_patterns* p, const uint64_t* d_pos, co
     });
 }
 
+int cad_matrix_from_positions_mf(cad_patterns* p, const uint64_t* d_pos, int64_t n, cad_matrix** out)
+{
+    return guarded([&] {
+        need(p && out && n > 0 && d_pos, "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        const PatternGroup& g = *p->group;
+        std::unique_ptr<cad_matrix> m(new_matrix(p->ctx, n, g.k(), int64_t(g.cols())));
+        hipStream_t s = p->ctx->stream;
+        m->mf = true;
+        m->ibytes = 0;
+        m->slot_lo = p->slot_lo;
+        m->slot_hi = p->slot_hi;
+        m->d_base = to_device(std::vector<uint32_t>(g.k(), 0u), s);
+        auto copy = [&](DBuf& dst, const DBuf& src) {
+            dst = DBuf(std::max<size_t>(src.bytes, 16));
+            if (src.bytes) CAD_HIP(hipMemcpyAsync(dst.p, src.p, src.bytes, hipMemcpyDeviceToDevice, s));
+        };
+        copy(m->d_slots, p->d_slots);
+        copy(m->d_sq, p->d_sq);
+        copy(m->d_dense, p->d_dense);
+        m->d_pos = DBuf(sizeof(uint64_t) * 2 * size_t(n));
+        CAD_HIP(hipMemcpyAsync(m->d_pos.p, d_pos, m->d_pos.bytes, hipMemcpyDeviceToDevice, s));
+        build_plans(m.get());
+        CAD_HIP(hipStreamSynchronize(s));
+        *out = m.release();
+    });
+}
+
 int cad_matrix_systems(const cad_matrix* m, int* nsys, int64_t* cols_per_system, int64_t* row_begin)
 {
     return guarded([&] {
@@ -510,7 +538,21 @@ int cad_matrix_download(const cad_matrix* m, uint32_t* out)
         std::vector<uint32_t> rowbase(size_t(std::max<int64_t>(R, 0)), 0u);
         for (int sy = 0; sy < m->nsys; sy++)
             for (int64_t r = m->sys_row_begin[sy]; r < m->sys_row_begin[sy + 1]; r++) rowbase[r] = m->sys_col[sy];
-        if (m->ibytes == 2) {
+        if (m->mf) {
+            // recompute the indices (global columns, slot-major) from the positions
+            std::vector<uint32_t> t(size_t(R) * k);
+            if (R > 0) {
+                DBuf tmp(t.size() * 4);
+                hipLaunchKernelGGL(k_index_extract<uint32_t>, dim3(grid1(R, kRowBlock)), dim3(kRowBlock), 0, s,
+                                   m->d_pos.as<uint64_t>(), R, m->d_slots.as<DevSlot>(), k, m->d_sq.as<uint8_t>(),
+                                   m->d_dense.as<uint32_t>(), tmp.as<uint32_t>(), R, int64_t(1), 0);
+                CAD_HIP(hipGetLastError());
+                CAD_HIP(hipMemcpyAsync(t.data(), tmp.p, t.size() * 4, hipMemcpyDeviceToHost, s));
+                CAD_HIP(hipStreamSynchronize(s));
+            }
+            for (int64_t r = 0; r < R; r++)
+                for (int j = 0; j < k; j++) out[r * k + j] = rowbase[r] + t[size_t(j) * R + r];
+        } else if (m->ibytes == 2) {
             std::vector<uint16_t> t(size_t(R) * k);
             CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 2, hipMemcpyDeviceToHost, s));
             CAD_HIP(hipStreamSynchronize(s));

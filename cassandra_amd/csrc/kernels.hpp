@@ -144,6 +144,22 @@ __global__ __launch_bounds__(kRowBlock) void k_index_extract(
     }
 }
 
+// Matrix-free view of a pattern matrix: the column of slot s in row r is
+// recomputed from the row's position (16 bytes) with the slot's program
+// instead of being read from a stored index matrix.
+struct PositionsView {
+    using index_type = void;  // no stored indices
+    const uint64_t* __restrict__ pos;
+    const DevSlot* __restrict__ slots;
+    const uint8_t* __restrict__ sq;
+    const uint32_t* __restrict__ dense;
+    __device__ __forceinline__ uint32_t col(int s, int64_t r) const
+    {
+        const DevSlot sl = slots[s];
+        return sl.col_offset + slot_index(sl, sq, dense, pos[2 * r], pos[2 * r + 1]);
+    }
+};
+
 // ------------------------------------------------------------ gathers ------
 // y = A x with the in-row sum taken sequentially in slot order from zero, the
 // order of Math/MatrixCSR.cpp:59-62 (bit-exact with it in FP32).  What is
@@ -345,6 +361,96 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                                 if (c < W) v[i][j] = xs[c];
                             }
                         }
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++)
+                if (j < seg.nslots) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) acc[i] += v[i][j];
+                }
+        }
+    }
+    double red = 0.0;
+#pragma unroll
+    for (int i = 0; i < RPT; i++)
+        if (i * kTileBlock < nval) red += epi.row(r0 + int64_t(i) * kTileBlock, acc[i]);
+    reduce_tail<kTileBlock>(epi, red, w.sys, g);
+}
+
+// The LDS-staged gather of a matrix-free pattern matrix: as k_gather_tile,
+// with each thread's RPT positions held in registers and every slot's column
+// computed by its program where the stored kernel loads it.  Multi-window
+// segments compute the slot columns once, before the window loop.
+template <class VT, int RPT, class Epi>
+__global__ __launch_bounds__(kTileBlock) void k_gather_tile_mf(PositionsView m, const RowWork* __restrict__ work,
+                                                               const uint32_t* __restrict__ sys_col,
+                                                               const GatherSeg* __restrict__ segs, int nseg,
+                                                               uint32_t wmax, const VT* __restrict__ x, Epi epi,
+                                                               GridSys g)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    VT* xs = reinterpret_cast<VT*>(smem_raw);
+    const RowWork w = work[blockIdx.x];
+    if (!epi.begin(w.sys)) return;
+    const uint32_t cb = sys_col[w.sys];
+    const int64_t r0 = w.row_begin + threadIdx.x;
+    const int nval = int(max<int64_t>(0, min<int64_t>(w.row_end - r0, int64_t(RPT) * kTileBlock)));
+    uint64_t P[RPT], O[RPT];
+    VT acc[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        acc[i] = VT(0);
+        P[i] = O[i] = 0;
+        if (i * kTileBlock < nval) {
+            P[i] = m.pos[2 * (r0 + int64_t(i) * kTileBlock)];
+            O[i] = m.pos[2 * (r0 + int64_t(i) * kTileBlock) + 1];
+        }
+    }
+    for (int sg = 0; sg < nseg; sg++) {
+        const GatherSeg seg = segs[sg];
+        const VT* __restrict__ xg = x + cb + seg.col_lo;
+        if (seg.nwin == 1) {
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < seg.width; i += kTileBlock) xs[i] = xg[i];
+            __syncthreads();
+            for (int s = seg.slot_first; s < seg.slot_first + seg.nslots; s++) {
+                const DevSlot sl = m.slots[s];
+                const uint32_t off = sl.col_offset - seg.col_lo;
+#pragma unroll
+                for (int i = 0; i < RPT; i++)
+                    if (i * kTileBlock < nval) acc[i] += xs[off + slot_index(sl, m.sq, m.dense, P[i], O[i])];
+            }
+        } else {
+            uint32_t c[RPT][kMaxSegSlots];
+            VT v[RPT][kMaxSegSlots];
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++) {
+#pragma unroll
+                for (int i = 0; i < RPT; i++) {
+                    v[i][j] = VT(0);
+                    c[i][j] = 0xFFFFFFFFu;
+                }
+                if (j < seg.nslots) {
+                    const DevSlot sl = m.slots[seg.slot_first + j];
+                    const uint32_t off = sl.col_offset - seg.col_lo;
+#pragma unroll
+                    for (int i = 0; i < RPT; i++)
+                        if (i * kTileBlock < nval) c[i][j] = off + slot_index(sl, m.sq, m.dense, P[i], O[i]);
+                }
+            }
+            for (uint32_t wl = 0; wl < seg.width; wl += wmax) {
+                const uint32_t W = min(wmax, seg.width - wl);
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < W; i += kTileBlock) xs[i] = xg[wl + i];
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < kMaxSegSlots; j++) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) {
+                        const uint32_t cl = c[i][j] - wl;
+                        if (cl < W) v[i][j] = xs[cl];
                     }
                 }
             }

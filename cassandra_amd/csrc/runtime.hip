@@ -294,7 +294,9 @@ void build_plans(cad_matrix* m)
         m->cols_per_sys = m->cols;
     }
     m->d_sys_col = to_device(m->sys_col, ctx.stream);
-    build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys, m->ibytes,
+    // (a matrix-free matrix sizes its scatter partials as a u16 matrix would)
+    build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
+                       m->mf ? 2 : m->ibytes,
                        ctx);
     build_gather_plan(m, m->gplan4, 4, 4);
     build_gather_plan(m, m->gplan8, 8, 2);

@@ -121,8 +121,11 @@ struct cad_matrix {
     cad_context* ctx = nullptr;
     int64_t rows = 0, cols = 0;
     int k = 0;
-    int ibytes = 4;                 // 2: u16 local + slot base, 4: u32 with base 0
+    int ibytes = 4;                 // 2: u16 local + slot base, 4: u32 with base 0, 0: matrix-free
     cad::DBuf d_idx, d_base;        // slot-major
+    // matrix-free: the positions and a copy of the pattern group's slot programs
+    bool mf = false;
+    cad::DBuf d_pos, d_slots, d_sq, d_dense;
     std::vector<uint32_t> slot_lo, slot_hi;
     int nsys = 1;
     std::vector<int64_t> sys_row_begin;  // nsys + 1

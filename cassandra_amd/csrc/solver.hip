@@ -18,7 +18,10 @@ namespace cad {
 template <class F>
 void with_view(cad_matrix* m, F&& f)
 {
-    if (m->ibytes == 2)
+    if (m->mf)
+        f(PositionsView{m->d_pos.as<uint64_t>(), m->d_slots.as<DevSlot>(), m->d_sq.as<uint8_t>(),
+                        m->d_dense.as<uint32_t>()});
+    else if (m->ibytes == 2)
         f(SlotMajor<uint16_t>{m->d_idx.as<uint16_t>(), m->rows, m->d_base.as<uint32_t>()});
     else
         f(SlotMajor<uint32_t>{m->d_idx.as<uint32_t>(), m->rows, m->d_base.as<uint32_t>()});
@@ -130,11 +133,19 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
         if (gp.ok && gp.rpt == RPT) {
             if (gp.tiles.empty()) return;
             const size_t lds = size_t(gp.lds_elems) * sizeof(VT);
-            auto kern = k_gather_tile<VT, RPT, IdxT, Epi>;
-            allow_lds(kern, lds);
-            hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
-                               gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(), gp.d_segs.as<GatherSeg>(),
-                               int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
+            if constexpr (std::is_void_v<IdxT>) {
+                auto kern = k_gather_tile_mf<VT, RPT, Epi>;
+                allow_lds(kern, lds);
+                hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
+                                   gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(), gp.d_segs.as<GatherSeg>(),
+                                   int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
+            } else {
+                auto kern = k_gather_tile<VT, RPT, IdxT, Epi>;
+                allow_lds(kern, lds);
+                hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
+                                   gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(), gp.d_segs.as<GatherSeg>(),
+                                   int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
+            }
         } else {
             const GridPlan& g = m->grid;
             if (g.rows.empty()) return;
@@ -330,7 +341,10 @@ static void lsqr_prepare_gather(cad_matrix* m)
     if (!gp.ok) return;
     with_view(m, [&](auto view) {
         using IdxT = typename decltype(view)::index_type;
-        allow_lds(k_gather_tile<VT, RPT, IdxT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
+        if constexpr (std::is_void_v<IdxT>)
+            allow_lds(k_gather_tile_mf<VT, RPT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
+        else
+            allow_lds(k_gather_tile<VT, RPT, IdxT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
     });
 }
 

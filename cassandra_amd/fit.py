@@ -146,6 +146,15 @@ class Patterns:
         call("cad_matrix_from_positions", self.h, _ptr(pos), pos.shape[0], C.byref(h))
         return Matrix(self.ctx, h)
 
+    def matrix_free(self, pos: torch.Tensor) -> "Matrix":
+        """The matrix of pos without stored indices: every product recomputes
+        the rows' indices from a library-held copy of the positions."""
+        _dev(pos, torch.int64)
+        torch.cuda.synchronize(pos.device)
+        h = C.c_void_p()
+        call("cad_matrix_from_positions_mf", self.h, _ptr(pos), pos.shape[0], C.byref(h))
+        return Matrix(self.ctx, h)
+
     def matrix_systems(self, pos: torch.Tensor, row_begin) -> "Matrix":
         """Block-diagonal batch: rows [row_begin[s], row_begin[s+1]) of pos form
         system s, which owns columns [s * cols, (s + 1) * cols) of x."""

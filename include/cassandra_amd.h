@@ -104,6 +104,14 @@ int cad_matrix_from_positions(cad_patterns* p, const uint64_t* d_positions, int6
  * launches; solvers keep per-system scalars and stop each system on its own. */
 int cad_matrix_from_positions_systems(cad_patterns* p, const uint64_t* d_positions,
                                       const int64_t* row_begin, int nsys, cad_matrix** out);
+/* The same matrix without stored indices (matrix-free): the library keeps a
+ * copy of the n device positions (16 bytes a row) and recomputes every row's
+ * indices with the pattern programs inside each product, where
+ * GroupIndexer::InsertIndices (Pattern/Indexer.cpp:413-422) would have filled
+ * MatrixCSR::Row(i) once.  Every product, solver and cad_matrix_download works
+ * on it; cad_matrix_info reports index_bytes 0. */
+int cad_matrix_from_positions_mf(cad_patterns* p, const uint64_t* d_positions, int64_t n,
+                                 cad_matrix** out);
 /* nsys (1 for an ordinary matrix), columns per system and, when row_begin is
  * not NULL, the nsys + 1 row boundaries. */
 int cad_matrix_systems(const cad_matrix* m, int* nsys, int64_t* cols_per_system, int64_t* row_begin);

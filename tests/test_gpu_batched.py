@@ -15,10 +15,11 @@ from cassandra_amd.fit import Patterns
 pytestmark = pytest.mark.gpu
 
 # (empties, rows): uneven, including a tiny system
-SYSTEMS = [(12, 3000), (20, 777), (33, 12000), (47, 5), (58, 5000), (24, 2500)]
-# Truncated iterates are compared on the systems with more rows than
-# iterations: the 5-row system is solved exactly within 5 steps, after which
-# both solvers iterate on rounding noise.
+SYSTEMS = [(12, 3000), (20, 777), (33, 12000), (47, 5), (50, 5000), (24, 2500)]
+# Truncated iterates are compared on the systems of many distinct rows: the
+# 5-row system is solved exactly within 5 steps, after which both solvers
+# iterate on rounding noise (so would a system near the start position, which
+# has only a few distinct positions).
 LARGE = [s for s, (_, n) in enumerate(SYSTEMS) if n >= 100]
 
 
@@ -90,11 +91,12 @@ def test_mixed_reaches_tolerance_per_system(ctx, oracle, batch):
     cols, idx, rb, A, b = batch
     say("batched mixed")
     x, rep, rel, its = A.solve_systems(b, precision=capi.PREC_MIXED, tol=1e-10)
+    say(rep, rel, its)
     assert rep["converged"] == 1
     xh, bh = x.cpu().numpy(), b.cpu().numpy()
     for s in range(len(SYSTEMS)):
         bs = bh[rb[s]:rb[s + 1]]
-        r, _ = oracle.rel_normal_residual(idx[s], cols, bs, xh[s * cols:(s + 1) * cols])
+        r = oracle.rel_normal_residual(idx[s], cols, bs, xh[s * cols:(s + 1) * cols])[0]
         assert r <= 1.5e-10, (s, r)
         assert rel[s] <= 1e-10
     assert np.all(its <= rep["iterations"]) and its.max() == rep["iterations"]

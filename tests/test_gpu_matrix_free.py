@@ -1,0 +1,64 @@
+"""Matrix-free pattern matrices: the indices are recomputed from the positions
+inside every product.  Same plan and arithmetic as the stored matrix, so the
+products and solver iterates must be bit-identical to it; the oracle pins both
+(GroupIndexer rows, Pattern/Indexer.cpp:413-422; Math/MatrixCSR.cpp:49-87)."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import cassandra_amd.capi as capi
+from cassandra_amd import names as N
+from cassandra_amd.fit import Patterns
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name,masks", [("fit12", N.FIT12), ("edax", N.edax), ("mix", N.test_mix)])
+def test_products_bit_identical(ctx, oracle, name, masks):
+    say("matrix-free products", name)
+    pos = oracle.random_play(77, 6001, 15, 45)
+    idx = oracle.Group(masks).indices(pos)
+    P = Patterns(ctx, masks)
+    d = torch.from_numpy(pos.view(np.int64)).cuda()
+    S, F = P.matrix(d), P.matrix_free(d)
+    assert F.index_bytes == 0 and (F.rows, F.k, F.cols) == (S.rows, S.k, S.cols)
+    assert np.array_equal(F.download(), idx)
+    x = torch.randn(S.cols, dtype=torch.float32, device="cuda")
+    yf = F.spmv(x)
+    assert torch.equal(yf, S.spmv(x))
+    assert np.array_equal(yf.cpu().numpy(), oracle.spmv(idx, x.cpu().numpy()))
+    x64 = x.double()
+    assert torch.equal(F.spmv(x64), S.spmv(x64))
+    g = torch.randint(-64, 65, (S.rows,), device="cuda").float()
+    zf = F.spmtv(g)
+    assert np.array_equal(zf.cpu().numpy(), oracle.spmtv(idx, g.cpu().numpy(), S.cols))
+    g64 = torch.randn(S.rows, dtype=torch.float64, device="cuda")
+    assert torch.equal(F.spmtv(g64), S.spmtv(g64))
+    assert torch.equal(F.jacobi(), S.jacobi())
+
+
+def test_solver_iterates_identical_and_converged(ctx, oracle):
+    say("matrix-free solvers")
+    pos = oracle.random_play(78, 20000, 15, 45)
+    idx = oracle.Group(N.FIT12).indices(pos)
+    P = Patterns(ctx, N.FIT12)
+    d = torch.from_numpy(pos.view(np.int64)).cuda()
+    S, F = P.matrix(d), P.matrix_free(d)
+    _, b = S.make_targets(2005, 3005)
+    _, bf = F.make_targets(2005, 3005)
+    assert torch.equal(b, bf)
+    opts = dict(precision=capi.PREC_FP64, tol=0.0, max_iterations=10, check_every=5)
+    xs, _ = S.solve(b, **opts)
+    xf, _ = F.solve(b, **opts)
+    assert torch.equal(xs, xf)
+    xm, rep = F.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
+    assert rep["converged"] == 1
+    r = oracle.rel_normal_residual(idx, S.cols, b.cpu().numpy(), xm.cpu().numpy())[0]
+    assert r <= 1.5e-10
