@@ -37,7 +37,7 @@ def _compile(src: str) -> str:
 
 CPP_DIR = os.path.join(HERE, "cpp")
 CPP_SOURCES = ["device.cpp", "Vector.cpp", "MatrixCSR.cpp", "Board.cpp", "Helpers.cpp", "Indexer.cpp",
-               "Evaluator.cpp", "PatternFit.cpp"]
+               "Evaluator.cpp", "PatternFit.cpp", "Binary.cpp"]
 CPP_OUT = os.path.join(HERE, "libcassandra_fit.so")
 TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "dropin_test.cpp")
 TEST_OUT = os.path.join(OBJ, "dropin_test")

@@ -5,12 +5,15 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <filesystem>
+#include <fstream>
 #include <functional>
 #include <random>
 #include <set>
 #include <string>
 #include <vector>
 
+#include "IO/Binary.h"
 #include "Math/Math.h"
 #include "Pattern/Evaluator.h"
 #include "Pattern/Helpers.h"
@@ -156,6 +159,37 @@ TEST_CASE(score_estimator_round_trip_and_symmetry, false)
             for (Position v : variants(c)) CHECK(est.Eval(v) == base);
         }
     }
+}
+
+// ---- IO/Binary.cpp:3-68: the weight file round trip and its byte layout
+TEST_CASE(model_file_round_trip, false)
+{
+    const std::vector<uint64_t> pat{pattern::L02X, pattern::D8};
+    PatternBasedEstimator est(20, pat);
+    CHECK(est.Stages() == 4);
+    const std::size_t m = ConfigurationCount(pat);
+    for (int s = 0; s < est.Stages(); s++) {
+        std::vector<float> w(m);
+        for (std::size_t i = 0; i < m; i++) w[i] = float(s) + 0.25f * float(i % 97);
+        est.score.estimators[s] = ScoreEstimator(pat, w);
+    }
+    const auto file = std::filesystem::temp_directory_path() / "cad_model_round_trip.bin";
+    Save(est, file);
+    PatternBasedEstimator back = LoadPatternBasedEstimator(file);
+    CHECK(back.StageSize() == 20 && back.Stages() == 4 && back.Pattern() == est.Pattern());
+    for (int s = 0; s < 4; s++) CHECK(back.score.estimators[s].Weights() == est.score.estimators[s].Weights());
+    CHECK(back.accuracy.ParameterValues() == est.accuracy.ParameterValues());
+    // i32 stage size, then vectors as u64 count + elements
+    const std::size_t n = est.Pattern().size();
+    const std::size_t want = 4 + (8 + 8 * n) + 8 + 4 * ((8 + 8 * n) + (8 + 4 * m)) + (8 + 4 * 5);
+    CHECK(std::filesystem::file_size(file) == want);
+    std::ifstream in(file, std::ios::binary);
+    int32_t stage = 0;
+    uint64_t count = 0;
+    in.read(reinterpret_cast<char*>(&stage), 4);
+    in.read(reinterpret_cast<char*>(&count), 8);
+    CHECK(stage == 20 && count == n);
+    std::filesystem::remove(file);
 }
 
 // ---- MathTest/MatrixCSR.cpp:3-54 (products on the GPU)
@@ -330,6 +364,15 @@ TEST_CASE(fit_stages_equals_stagewise_fit_weights, true)
 int main(int argc, char** argv)
 {
     const std::string mode = argc > 1 ? argv[1] : "host";
+    if (mode == "copy-model" && argc == 4) {  // load a weight file and save it again
+        try {
+            Save(LoadPatternBasedEstimator(argv[2]), argv[3]);
+        } catch (const std::exception& e) {
+            std::printf("copy-model failed: %s\n", e.what());
+            return 1;
+        }
+        return 0;
+    }
     int run = 0;
     for (auto& c : cases()) {
         if (c.gpu != (mode == "gpu")) continue;
