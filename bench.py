@@ -303,6 +303,8 @@ def main():
                     help="extra workloads: buckets,evaluator,matrix_free (or empty)")
     ap.add_argument("--traffic", action="store_true",
                     help="only launch the measured kernels a few times and a 1 GiB device copy (for counter runs)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default product schedules instead of timing the candidates")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -325,11 +327,13 @@ def main():
     xt, b = A.make_targets(2002, 3002)
     del pos
     log(f"data: {A.rows} rows, k={A.k}, cols={A.cols}, index bytes {A.index_bytes} ({time.perf_counter() - t:.1f} s)")
+    t = time.perf_counter()
+    tune_log = A.autotune() if args.autotune else ""
+    log(f"autotune ({time.perf_counter() - t:.2f} s):\n{tune_log}")
     if args.traffic:
         traffic_launches(ctx, A)
         ctx.close()
         return
-
     opts = dict(precision=capi.PREC_MIXED, tol=1e-10, verbose=int(args.verbose))
     for i in range(args.warmup):
         _, rep = A.solve(b, **opts)
@@ -415,6 +419,7 @@ def main():
                     "spmtv_equiv_u32_GBps": bytes_u32 / (spmtv_ms * 1e-3) / 1e9},
         "roofline": {"kernel": dominant[0], "bound": "hbm", "achieved": dominant[2], "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": dominant[2] / HBM_PEAK_GBPS, "traffic": traffic},
+        "schedule": tune_log.strip().splitlines()[-1] if tune_log else "default",
         "cpu_baseline": cpu,
         "buckets": buckets,
         "evaluator": evaluator,

@@ -77,6 +77,7 @@ SIGNATURES = {
     "cad_matrix_from_positions_systems": (C.c_int, [vp, vp, vp, C.c_int, PP]),
     "cad_matrix_systems": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(i64), vp]),
     "cad_matrix_from_positions_mf": (C.c_int, [vp, vp, i64, PP]),
+    "cad_matrix_autotune": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
     "cad_matrix_create": (C.c_int, [vp, vp, i64, C.c_int, i64, PP]),
     "cad_matrix_destroy": (C.c_int, [vp]),
     "cad_matrix_info": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_int), C.POINTER(i64), C.POINTER(C.c_int)]),

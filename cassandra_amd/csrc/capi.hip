@@ -118,7 +118,7 @@ const ScatterPlan& eval_plan(cad_patterns* p, int64_t B)
     if (p->eval_plans.size() >= 16) p->eval_plans.clear();
     std::unique_ptr<ScatterPlan> sp(new ScatterPlan);
     build_scatter_plan(*sp, p->slot_lo, p->slot_hi, std::vector<int64_t>{0, B}, std::vector<uint32_t>{0u},
-                       int64_t(p->group->cols()), 4, *p->ctx);
+                       int64_t(p->group->cols()), 4, *p->ctx, TuneParams{});
     const ScatterPlan& ref = *sp;
     p->eval_plans[B] = std::move(sp);
     return ref;
@@ -450,6 +450,19 @@ int cad_matrix_from_positions_mf(cad_patterns* p, const uint64_t* d_pos, int64_t
         build_plans(m.get());
         CAD_HIP(hipStreamSynchronize(s));
         *out = m.release();
+    });
+}
+
+int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size)
+{
+    return guarded([&] {
+        need(m != nullptr, "null matrix");
+        const std::string l = autotune(m);
+        if (log && log_size) {
+            const size_t n = std::min(log_size - 1, l.size());
+            std::memcpy(log, l.data(), n);
+            log[n] = 0;
+        }
     });
 }
 

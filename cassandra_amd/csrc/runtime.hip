@@ -108,7 +108,7 @@ void make_groups(const std::vector<uint32_t>& lo, const std::vector<uint32_t>& h
 // 10% of the window's index traffic.
 void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
                         const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, uint32_t wmax, int vbytes,
-                        int ibytes, int cus, ScatterLists& L)
+                        int ibytes, int cus, const TuneParams& tune, ScatterLists& L)
 {
     L.work.clear();
     L.reduce.clear();
@@ -121,7 +121,7 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
             const uint32_t nwin = (g.col_hi - g.col_lo + wmax - 1) / wmax;
             total += double(R) * g.nslots * nwin;
         }
-    const double target = 2.0 * std::max(cus, 1);
+    const double target = tune.blocks_per_cu * std::max(cus, 1);
     int64_t off = 0;
     for (int s = 0; s < nsys; s++) {
         const int64_t R = sys_rows[s + 1] - sys_rows[s];
@@ -137,7 +137,7 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
                 if (R > 0) {
                     const double share = total > 0 ? double(R) * g.nslots / total : 1.0;
                     C = std::max<int64_t>(1, int64_t(std::llround(target * share)));
-                    const double cap = 0.1 * double(R) * g.nslots * ibytes / (2.0 * W * vbytes);
+                    const double cap = tune.partial_cap * double(R) * g.nslots * ibytes / (2.0 * W * vbytes);
                     C = std::max<int64_t>(1, std::min<int64_t>(C, int64_t(cap)));
                     C = std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock);
                     const int64_t chunk = (R + C - 1) / C;
@@ -206,12 +206,12 @@ GridSys GatherPlan::sys() const
 
 void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
                         const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
-                        int64_t cols_per_sys, int ibytes, const Context& ctx)
+                        int64_t cols_per_sys, int ibytes, const Context& ctx, const TuneParams& tune)
 {
     make_groups(lo, hi, sp);
     const size_t lds = lds_budget(ctx);
-    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 4), 4, ibytes, ctx.cus, sp.f4);
-    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 8), 8, ibytes, ctx.cus, sp.f8);
+    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 4), 4, ibytes, ctx.cus, tune, sp.f4);
+    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 8), 8, ibytes, ctx.cus, tune, sp.f8);
     sp.d_groups = to_device(sp.groups, ctx.stream);
     sp.d_slot_list = to_device(sp.slot_list, ctx.stream);
     for (ScatterLists* L : {&sp.f4, &sp.f8}) {
@@ -231,6 +231,7 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
     const ScatterPlan& sp = m->splan;
     gp.ok = false;
     gp.rpt = rpt;
+    if (rpt <= 0) return;  // the row kernel
     const size_t lds = lds_budget(ctx);
     gp.wmax = uint32_t(lds / vbytes);
     std::vector<const SlotGroup*> order;
@@ -296,10 +297,9 @@ void build_plans(cad_matrix* m)
     m->d_sys_col = to_device(m->sys_col, ctx.stream);
     // (a matrix-free matrix sizes its scatter partials as a u16 matrix would)
     build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
-                       m->mf ? 2 : m->ibytes,
-                       ctx);
-    build_gather_plan(m, m->gplan4, 4, 4);
-    build_gather_plan(m, m->gplan8, 8, 2);
+                       m->mf ? 2 : m->ibytes, ctx, m->tune);
+    build_gather_plan(m, m->gplan4, 4, m->tune.rpt4);
+    build_gather_plan(m, m->gplan8, 8, m->tune.rpt8);
 
     GridPlan& g = m->grid;
     g.rows.clear();

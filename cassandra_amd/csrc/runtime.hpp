@@ -92,6 +92,14 @@ struct GridPlan {
     GridSys col_sys() const;
 };
 
+// Schedule knobs of a matrix's plans, chosen by the autotuner (or defaults).
+struct TuneParams {
+    int rpt4 = 4;               // rows per thread of the FP32 tile gather (0: row kernel)
+    int rpt8 = 2;               // rows per thread of the FP64 tile gather (0: row kernel)
+    double blocks_per_cu = 2.0; // scatter tiles per CU the row chunks aim at
+    double partial_cap = 0.1;   // partial-slice traffic cap as a fraction of index traffic
+};
+
 struct Workspace;
 
 struct Context {
@@ -135,6 +143,8 @@ struct cad_matrix {
     cad::ScatterPlan splan;
     cad::GridPlan grid;
     cad::GatherPlan gplan4, gplan8;  // float / double tiles
+    cad::TuneParams tune;           // schedule of the plans below
+    bool tuned = false;
     cad::DBuf d_jacobi_cnt;         // cached exact multiplicity counts (u32)
     bool have_cnt = false;
     std::map<int, std::unique_ptr<cad::Workspace>> ws;
@@ -143,5 +153,9 @@ struct cad_matrix {
 
 namespace cad {
 void build_plans(cad_matrix* m);
+void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt);
+void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
+                        const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
+                        int64_t cols_per_sys, int ibytes, const Context& ctx, const TuneParams& tune);
 void set_error(const std::string& s);
 }  // namespace cad

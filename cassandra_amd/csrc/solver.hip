@@ -121,31 +121,51 @@ GatherPlan& gather_plan(cad_matrix* m)
     return sizeof(VT) == 4 ? m->gplan4 : m->gplan8;
 }
 
+// The tile gather's rows per thread is a schedule knob: the instantiated
+// values are 2, 4 (FP32) and 1, 2 (FP64), the ones whose registers fit a
+// 1024-thread block without spilling; f receives the value as a compile-time
+// constant.
+template <class VT, class View, class F>
+void with_rpt(int rpt, F&& f)
+{
+    if constexpr (sizeof(VT) == 4) {
+        if (rpt == 2) return f(std::integral_constant<int, 2>{});
+        if (rpt == 4) return f(std::integral_constant<int, 4>{});
+    } else {
+        if (rpt == 1) return f(std::integral_constant<int, 1>{});
+        if (rpt == 2) return f(std::integral_constant<int, 2>{});
+    }
+    throw Error(CAD_ERR_ARGUMENT, "unsupported rows per thread of the tile gather");
+}
+
 // Row gather with an epilogue: the LDS-staged tile kernel for pattern
 // matrices, the generic row kernel otherwise.
 template <class VT, class Epi>
 void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
 {
     GatherPlan& gp = gather_plan<VT>(m);
-    constexpr int RPT = sizeof(VT) == 4 ? 4 : 2;
     with_view(m, [&](auto view) {
-        using IdxT = typename decltype(view)::index_type;
-        if (gp.ok && gp.rpt == RPT) {
+        using View = decltype(view);
+        using IdxT = typename View::index_type;
+        if (gp.ok) {
             if (gp.tiles.empty()) return;
             const size_t lds = size_t(gp.lds_elems) * sizeof(VT);
-            if constexpr (std::is_void_v<IdxT>) {
-                auto kern = k_gather_tile_mf<VT, RPT, Epi>;
-                allow_lds(kern, lds);
-                hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
-                                   gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(), gp.d_segs.as<GatherSeg>(),
-                                   int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
-            } else {
-                auto kern = k_gather_tile<VT, RPT, IdxT, Epi>;
-                allow_lds(kern, lds);
-                hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
-                                   gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(), gp.d_segs.as<GatherSeg>(),
-                                   int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
-            }
+            with_rpt<VT, View>(gp.rpt, [&](auto rc) {
+                constexpr int RPT = decltype(rc)::value;
+                if constexpr (std::is_void_v<IdxT>) {
+                    auto kern = k_gather_tile_mf<VT, RPT, Epi>;
+                    allow_lds(kern, lds);
+                    hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
+                                       gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
+                                       gp.d_segs.as<GatherSeg>(), int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
+                } else {
+                    auto kern = k_gather_tile<VT, RPT, IdxT, Epi>;
+                    allow_lds(kern, lds);
+                    hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
+                                       gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
+                                       gp.d_segs.as<GatherSeg>(), int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
+                }
+            });
         } else {
             const GridPlan& g = m->grid;
             if (g.rows.empty()) return;
@@ -337,14 +357,17 @@ template <class VT>
 static void lsqr_prepare_gather(cad_matrix* m)
 {
     GatherPlan& gp = gather_plan<VT>(m);
-    constexpr int RPT = sizeof(VT) == 4 ? 4 : 2;
     if (!gp.ok) return;
     with_view(m, [&](auto view) {
-        using IdxT = typename decltype(view)::index_type;
-        if constexpr (std::is_void_v<IdxT>)
-            allow_lds(k_gather_tile_mf<VT, RPT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
-        else
-            allow_lds(k_gather_tile<VT, RPT, IdxT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
+        using View = decltype(view);
+        using IdxT = typename View::index_type;
+        with_rpt<VT, View>(gp.rpt, [&](auto rc) {
+            constexpr int RPT = decltype(rc)::value;
+            if constexpr (std::is_void_v<IdxT>)
+                allow_lds(k_gather_tile_mf<VT, RPT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
+            else
+                allow_lds(k_gather_tile<VT, RPT, IdxT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
+        });
     });
 }
 
@@ -715,6 +738,126 @@ double rel_normal_residual(cad_matrix* m, const double* b, const double* x, doub
     }
     if (rnorm) *rnorm = std::sqrt(rsum);
     return worst;
+}
+
+// ------------------------------------------------------------ autotune --
+namespace {
+
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventTimer()
+    {
+        CAD_HIP(hipEventCreate(&a));
+        CAD_HIP(hipEventCreate(&b));
+    }
+    ~EventTimer()
+    {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    // average of reps launches after one warm launch, in microseconds
+    template <class F>
+    double us(F&& f, hipStream_t s, int reps = 3)
+    {
+        f();
+        CAD_HIP(hipEventRecord(a, s));
+        for (int i = 0; i < reps; i++) f();
+        CAD_HIP(hipEventRecord(b, s));
+        CAD_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        CAD_HIP(hipEventElapsedTime(&ms, a, b));
+        return 1e3 * double(ms) / reps;
+    }
+};
+
+void drop_graphs(cad_matrix* m)
+{
+    for (auto& kv : m->ws) {
+        Workspace& w = *kv.second;
+        if (w.exec) (void)hipGraphExecDestroy(w.exec);
+        if (w.graph) (void)hipGraphDestroy(w.graph);
+        w.exec = nullptr;
+        w.graph = nullptr;
+        w.graph_iters = 0;
+    }
+}
+
+}  // namespace
+
+std::string autotune(cad_matrix* m)
+{
+    hipStream_t s = m->ctx->stream;
+    CAD_HIP(hipStreamSynchronize(s));
+    drop_graphs(m);
+    const int64_t R = std::max<int64_t>(1, m->rows), Cn = std::max<int64_t>(1, m->cols);
+    DBuf x4(4 * Cn), y4(4 * R), z4(4 * Cn), x8(8 * Cn), y8(8 * R);
+    fill<float>(x4.as<float>(), Cn, 1.0f, s);
+    fill<float>(y4.as<float>(), R, 1.0f, s);
+    fill<double>(x8.as<double>(), Cn, 1.0, s);
+    EventTimer t;
+    std::string log;
+    char line[160];
+    TuneParams& tp = m->tune;
+    // FP32 gather: tile rows per thread, or the row kernel (0)
+    {
+        const std::vector<int> cands{4, 2, 0};
+        int best = tp.rpt4;
+        double best_us = 1e300;
+        for (int c : cands) {
+            build_gather_plan(m, m->gplan4, 4, c);
+            if (c > 0 && !m->gplan4.ok) continue;
+            const double us = t.us([&] { spmv<float>(m, x4.as<float>(), y4.as<float>(), s); }, s);
+            std::snprintf(line, sizeof line, "gather f32 rpt %d: %.1f us\n", c, us);
+            log += line;
+            if (us < best_us) best_us = us, best = c;
+        }
+        tp.rpt4 = best;
+        build_gather_plan(m, m->gplan4, 4, best);
+    }
+    // FP64 gather
+    {
+        int best = tp.rpt8;
+        double best_us = 1e300;
+        for (int c : {2, 1, 0}) {
+            build_gather_plan(m, m->gplan8, 8, c);
+            if (c > 0 && !m->gplan8.ok) continue;
+            const double us = t.us([&] { spmv<double>(m, x8.as<double>(), y8.as<double>(), s); }, s);
+            std::snprintf(line, sizeof line, "gather f64 rpt %d: %.1f us\n", c, us);
+            log += line;
+            if (us < best_us) best_us = us, best = c;
+        }
+        tp.rpt8 = best;
+        build_gather_plan(m, m->gplan8, 8, best);
+    }
+    // scatter: tiles per CU and the partial-traffic cap (FP32 timing)
+    {
+        const TuneParams keep = tp;
+        double bb = tp.blocks_per_cu, bc = tp.partial_cap, best_us = 1e300;
+        for (double bpc : {1.0, 2.0, 4.0})
+            for (double cap : {0.05, 0.1, 0.25, 1.0}) {
+                tp.blocks_per_cu = bpc;
+                tp.partial_cap = cap;
+                build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
+                                   m->mf ? 2 : m->ibytes, *m->ctx, tp);
+                const double us =
+                    t.us([&] { scatter<float, float, float>(m, y4.as<float>(), nullptr, z4.as<float>(), s); }, s);
+                std::snprintf(line, sizeof line, "scatter f32 blocks/CU %.0f cap %.2f (%zu tiles): %.1f us\n", bpc,
+                              cap, m->splan.f4.work.size(), us);
+                log += line;
+                if (us < best_us) best_us = us, bb = bpc, bc = cap;
+            }
+        tp = keep;
+        tp.blocks_per_cu = bb;
+        tp.partial_cap = bc;
+        build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
+                           m->mf ? 2 : m->ibytes, *m->ctx, tp);
+    }
+    CAD_HIP(hipStreamSynchronize(s));
+    m->tuned = true;
+    std::snprintf(line, sizeof line, "chosen: gather f32 rpt %d, f64 rpt %d, scatter blocks/CU %.0f cap %.2f\n",
+                  tp.rpt4, tp.rpt8, tp.blocks_per_cu, tp.partial_cap);
+    log += line;
+    return log;
 }
 
 }  // namespace cad

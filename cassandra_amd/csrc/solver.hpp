@@ -3,6 +3,8 @@
 #include "../../include/cassandra_amd.h"
 #include "runtime.hpp"
 
+#include <string>
+
 namespace cad {
 
 // Solver buffers cached on a matrix (one set per solver / precision), with
@@ -38,8 +40,8 @@ void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o
 void lsmr_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep);
 double rel_normal_residual(cad_matrix* m, const double* b, const double* x, double* rnorm);
 void matrix_targets(cad_matrix* m, const double* xt, uint64_t seed, double sn, double* b);
-void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
-                        const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
-                        int64_t cols_per_sys, int ibytes, const Context& ctx);
+// Times the candidate schedules of the FP32 products on this matrix and
+// keeps the fastest (rebuilds the plans; drops captured solver graphs).
+std::string autotune(cad_matrix* m);
 
 }  // namespace cad

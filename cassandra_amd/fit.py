@@ -205,6 +205,12 @@ class Matrix:
         except Exception:
             pass
 
+    def autotune(self) -> str:
+        """Pick the fastest product schedules for this matrix; returns the log."""
+        buf = C.create_string_buffer(8192)
+        call("cad_matrix_autotune", self.h, buf, len(buf))
+        return buf.value.decode()
+
     def download(self) -> np.ndarray:
         out = np.zeros((self.rows, self.k), dtype=np.uint32)
         call("cad_matrix_download", self.h, out.ctypes.data_as(C.c_void_p))

@@ -112,6 +112,15 @@ int cad_matrix_from_positions_systems(cad_patterns* p, const uint64_t* d_positio
  * on it; cad_matrix_info reports index_bytes 0. */
 int cad_matrix_from_positions_mf(cad_patterns* p, const uint64_t* d_positions, int64_t n,
                                  cad_matrix** out);
+/* No counterpart in the reference (its products have one fixed OpenMP
+ * schedule).  Times the candidate schedules of this matrix's products -- rows
+ * per thread of the LDS tile gather or the row kernel, scatter tiles per CU
+ * and the cap on partial-slice traffic -- with HIP events on the context's
+ * stream, keeps the fastest and drops the solvers' captured graphs.  The
+ * timings and the choice go to log (may be NULL).  Results do not change: every
+ * schedule sums each row in the same order and reduces the scatter partials in
+ * chunk order. */
+int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size);
 /* nsys (1 for an ordinary matrix), columns per system and, when row_begin is
  * not NULL, the nsys + 1 row boundaries. */
 int cad_matrix_systems(const cad_matrix* m, int* nsys, int64_t* cols_per_system, int64_t* row_begin);

@@ -1,0 +1,43 @@
+"""The schedule autotuner: whatever schedule it picks, the products and the
+solver give the same numbers (each row is summed in the same order and the
+scatter partials are reduced in chunk order)."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import cassandra_amd.capi as capi
+from cassandra_amd import names as N
+from cassandra_amd.fit import Patterns
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("free", [False, True], ids=["stored", "matrix_free"])
+def test_autotune_keeps_results(ctx, oracle, free):
+    print("autotune", free, flush=True)
+    sys.stdout.flush()
+    pos = oracle.random_play(91, 40000, 22)
+    idx = oracle.Group(N.FIT8).indices(pos)
+    P = Patterns(ctx, N.FIT8)
+    d = torch.from_numpy(pos.view(np.int64)).cuda()
+    A = P.matrix_free(d) if free else P.matrix(d)
+    _, b = A.make_targets(2006, 3006)
+    x = torch.randn(A.cols, dtype=torch.float32, device="cuda")
+    g = torch.randint(-64, 65, (A.rows,), device="cuda").float()
+    y0, z0 = A.spmv(x), A.spmtv(g)
+    x0, _ = A.solve(b, precision=capi.PREC_FP64, tol=0.0, max_iterations=10, check_every=5)
+    log = A.autotune()
+    print(log, flush=True)
+    assert "chosen:" in log
+    assert torch.equal(A.spmv(x), y0)
+    assert np.array_equal(y0.cpu().numpy(), oracle.spmv(idx, x.cpu().numpy()))
+    assert torch.equal(A.spmtv(g), z0)
+    x1, _ = A.solve(b, precision=capi.PREC_FP64, tol=0.0, max_iterations=10, check_every=5)
+    po, p1 = oracle.spmv(idx, x0.cpu().numpy()), oracle.spmv(idx, x1.cpu().numpy())
+    # FP64 scatter order may change with the chunking: agreement to rounding
+    assert np.linalg.norm(p1 - po) <= 1e-9 * np.linalg.norm(po)
+    xm, rep = A.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
+    assert rep["converged"] == 1
+    assert oracle.rel_normal_residual(idx, A.cols, b.cpu().numpy(), xm.cpu().numpy())[0] <= 1.5e-10
