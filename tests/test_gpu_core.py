@@ -163,3 +163,18 @@ def test_full_size_properties(ctx):
     hi = lo + np.repeat([P.pattern_info(i)["size"] for i in range(P.npatterns)],
                         [P.pattern_info(i)["order"] for i in range(P.npatterns)])
     assert np.all((idx >= lo) & (idx < hi))
+
+
+def test_reference_golden_vectors(ctx):
+    # outputs of the reference's compiled Math sources (tests/golden/make_reference_math.py)
+    import os
+
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_math.npz"),
+                allow_pickle=False)
+    idx, cols = G["idx"], int(G["cols"])
+    P = Patterns(ctx, [N.D8, N.L1, N.D6])
+    for A in (P.matrix(torch.from_numpy(G["positions"].view(np.int64)).cuda()), Matrix.from_host(ctx, idx, cols)):
+        assert np.array_equal(A.download(), idx)
+        assert np.array_equal(A.spmv(torch.from_numpy(G["x"]).cuda()).cpu().numpy(), G["ax"])
+        assert np.array_equal(A.spmtv(torch.from_numpy(G["y"]).cuda()).cpu().numpy(), G["ya"])
+        assert np.array_equal(A.jacobi().cpu().numpy(), G["jacobi"])

@@ -239,3 +239,22 @@ def test_targets_construction():
     # the noise is what remains after the exact prediction (before rounding)
     resid = b - O.spmv(idx, xt)
     assert 3.5 < resid.std() < 4.7
+
+
+def golden():
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_math.npz")
+    return np.load(path, allow_pickle=False)
+
+
+def test_oracle_against_reference_golden_vectors():
+    # outputs of the reference's compiled Math sources (tests/golden/make_reference_math.py)
+    from cassandra_amd import names as N
+
+    G = golden()
+    idx, cols = G["idx"], int(G["cols"])
+    assert np.array_equal(O.Group([N.D8, N.L1, N.D6]).indices(G["positions"]), idx)
+    assert np.array_equal(O.spmv(idx, G["x"]), G["ax"])
+    assert np.array_equal(O.spmtv(idx, G["y"], cols), G["ya"])
+    assert np.array_equal(O.jacobi(idx, cols), G["jacobi"])
