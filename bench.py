@@ -15,7 +15,10 @@ time-to-fit on the same GPU, the SpMV / scatter kernel bandwidth against the
 baseline (the oracle's FP64 LSQR on the host cores, a bounded sample of
 iterations scaled to the iteration count the fit needs).
 
-Three more workloads ride along in the same line (--legs):
+Four more workloads ride along in the same line (--legs):
+  reference_fit  the reference's own fit: 100k positions at 20 empties,
+             {L02X, B5, Q3, D8}, Jacobi-PCG 100 iterations (FitWeights) on the
+             GPU, the oracle on the host cores and the compiled reference.
   buckets    50 independent systems, one per empty count 10..59, rows drawn
              log-uniform in [20k, 800k] (seed 1003), pattern set {L02X, B5, Q3,
              D8}; all fitted in the same launches (mixed LSQR to 1e-10 per
@@ -63,9 +66,12 @@ def event_time_ms(ctx, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def cpu_baseline(A, b, fp64_iterations, budget_s):
-    """The oracle's FP64 LSQR (column-scaled, OpenMP) on a bounded number of
-    iterations of the same matrix; scaled to the iterations the fit needs."""
+def cpu_baseline(A, b, fp64_iterations, budget_s, x_gpu):
+    """The oracle's FP64 LSQR (column-scaled, OpenMP over all host threads) on
+    the same matrix.  When the iterations the GPU FP64 fit needed fit in the
+    budget they all run, the time is measured whole, and the CPU solution is
+    the FP64 reference for the GPU weights; otherwise a bounded sample of
+    iterations is timed and scaled."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -79,17 +85,29 @@ def cpu_baseline(A, b, fp64_iterations, budget_s):
     t0 = time.perf_counter()
     O.lsqr(idx, A.cols, bh, 2, colscale=D)
     per = (time.perf_counter() - t0) / 2
-    n = int(max(2, min(200, budget_s / max(per, 1e-6))))
-    t0 = time.perf_counter()
-    O.lsqr(idx, A.cols, bh, n, colscale=D)
-    per = (time.perf_counter() - t0) / n
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    out = {
-        "value": per * fp64_iterations, "unit": "s", "cores": cores, "kind": "port",
-        "sample": f"{n} iterations of the oracle's column-scaled FP64 LSQR on the full {A.rows}x{A.cols} "
-                  f"matrix ({per:.3f} s/iteration), scaled to the {fp64_iterations} iterations the FP64 "
-                  f"fit needs to reach 1e-10",
-    }
+    out = {"unit": "s", "cores": cores, "kind": "port"}
+    if per * fp64_iterations <= budget_s:
+        t0 = time.perf_counter()
+        xc, _, _ = O.lsqr(idx, A.cols, bh, fp64_iterations, colscale=D)
+        secs = time.perf_counter() - t0
+        rel = O.rel_normal_residual(idx, A.cols, bh, xc)[0]
+        xg = x_gpu.cpu().numpy()
+        pc, pg = O.spmv(idx, xc), O.spmv(idx, xg)
+        out.update(value=secs, sample=f"the whole fit: {fp64_iterations} iterations of the oracle's column-scaled "
+                                      f"FP64 LSQR on the full {A.rows}x{A.cols} matrix",
+                   rel_normal_residual=rel,
+                   gpu_mixed_weight_error=float(np.linalg.norm(xg - xc) / np.linalg.norm(xc)),
+                   gpu_mixed_prediction_error=float(np.linalg.norm(pg - pc) / np.linalg.norm(pc)))
+    else:
+        n = int(max(2, min(200, budget_s / max(per, 1e-6))))
+        t0 = time.perf_counter()
+        O.lsqr(idx, A.cols, bh, n, colscale=D)
+        per = (time.perf_counter() - t0) / n
+        out.update(value=per * fp64_iterations,
+                   sample=f"{n} iterations of the oracle's column-scaled FP64 LSQR on the full {A.rows}x{A.cols} "
+                          f"matrix ({per:.3f} s/iteration), scaled to the {fp64_iterations} iterations the FP64 "
+                          f"fit needs to reach 1e-10")
     R = O.ref()
     if R is not None:
         m = R.ref_csr_create(idx.ctypes.data_as(O.u32p), A.rows, A.k, A.cols)
@@ -99,6 +117,67 @@ def cpu_baseline(A, b, fp64_iterations, budget_s):
             t0 = time.perf_counter()
             R.ref_lsqr(m, bf.ctypes.data_as(O.f32p), A.rows, 3, x.ctypes.data_as(O.f32p), None)
             out["reference_fp32_lsqr_s_per_iteration"] = (time.perf_counter() - t0) / 3
+        finally:
+            R.ref_csr_free(m)
+    return out
+
+
+def leg_reference_fit(ctx):
+    """The reference's own fit: 100,000 positions at 20 empties, {L02X, B5, Q3,
+    D8}, Jacobi-PCG on the normal equations for 100 iterations (FitWeights,
+    PatternFit/PatternFit.cpp:7-39) -- on the GPU in FP64 and FP32, the
+    oracle's FP64 restatement on the host cores, and the compiled reference
+    (float, its own OpenMP loops) where it was built."""
+    import cassandra_amd.capi as capi
+    from cassandra_amd import names as N
+    from cassandra_amd.fit import Patterns
+
+    P = Patterns(ctx, N.FIT4)
+    pos = ctx.random_play(1001, 100_000, 20)
+    A = P.matrix(pos)
+    _, b = A.make_targets(2001, 3001)
+    out = {"rows": A.rows, "k": A.k, "cols": A.cols, "iterations": 100}
+    for name, prec in (("gpu_fp64", capi.PREC_FP64), ("gpu_fp32", capi.PREC_FP32)):
+        opts = dict(solver=capi.SOLVER_PCG, precision=prec, tol=0.0, max_iterations=100, check_every=100)
+        A.solve(b, **opts)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        x, _ = A.solve(b, **opts)
+        ctx.synchronize()
+        out[name + "_seconds"] = time.perf_counter() - t0
+        if prec == capi.PREC_FP64:
+            x64 = x
+    out.update(cpu_baseline_pcg(A.download(), A.cols, b.cpu().numpy(), x64.cpu().numpy(), 100))
+    log(f"reference fit: {out}")
+    return out
+
+
+def cpu_baseline_pcg(idx, cols, bh, x_gpu, iterations):
+    """CPU baseline of the reference fit: the oracle's FP64 PCG and the
+    compiled reference's float FitWeights PCG, timed on the host cores; the
+    oracle's iterate is also the FP64 check of the GPU's."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    out = {"cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    t0 = time.perf_counter()
+    xo, _ = O.pcg(idx, cols, bh, iterations)
+    out["cpu_oracle_fp64_seconds"] = time.perf_counter() - t0
+    po = O.spmv(idx, xo)
+    out["gpu_fp64_prediction_error"] = float(np.linalg.norm(O.spmv(idx, x_gpu) - po) / np.linalg.norm(po))
+    R = O.ref()
+    if R is not None:
+        m = R.ref_csr_create(idx.ctypes.data_as(O.u32p), idx.shape[0], idx.shape[1], cols)
+        try:
+            xr = np.zeros(cols, np.float32)
+            bf = bh.astype(np.float32)
+            t0 = time.perf_counter()
+            R.ref_fit_pcg(m, bf.ctypes.data_as(O.f32p), idx.shape[0], iterations, xr.ctypes.data_as(O.f32p), None)
+            out["cpu_reference_fp32_seconds"] = time.perf_counter() - t0
+            out["reference_prediction_error"] = float(np.linalg.norm(O.spmv(idx, xr.astype(np.float64)) - po)
+                                                      / np.linalg.norm(po))
         finally:
             R.ref_csr_free(m)
     return out
@@ -298,9 +377,9 @@ def main():
     ap.add_argument("--reps", type=int, default=50, help="launches per kernel timing")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--legs", default="buckets,evaluator,matrix_free",
-                    help="extra workloads: buckets,evaluator,matrix_free (or empty)")
+    ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU fit before sampling")
+    ap.add_argument("--legs", default="reference_fit,buckets,evaluator,matrix_free",
+                    help="extra workloads: reference_fit,buckets,evaluator,matrix_free (or empty)")
     ap.add_argument("--traffic", action="store_true",
                     help="only launch the measured kernels a few times and a 1 GiB device copy (for counter runs)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
@@ -391,11 +470,12 @@ def main():
 
     cpu = None
     if not args.no_cpu and fp64 is not None:
-        cpu = cpu_baseline(A, b, fp64["iterations"], args.cpu_budget)
+        cpu = cpu_baseline(A, b, fp64["iterations"], args.cpu_budget, x)
         log(f"cpu: {cpu}")
     shape = dict(rows=A.rows, k=A.k, cols=A.cols)
     del A, b, x
     legs = set(args.legs.split(",")) if args.legs else set()
+    reference_fit = leg_reference_fit(ctx) if "reference_fit" in legs else None
     buckets = leg_buckets(ctx) if "buckets" in legs else None
     evaluator = leg_evaluator(ctx) if "evaluator" in legs else None
     matrix_free = leg_matrix_free(ctx) if "matrix_free" in legs else None
@@ -421,6 +501,7 @@ def main():
                      "unit": "GB/s", "frac": dominant[2] / HBM_PEAK_GBPS, "traffic": traffic},
         "schedule": tune_log.strip().splitlines()[-1] if tune_log else "default",
         "cpu_baseline": cpu,
+        "reference_fit": reference_fit,
         "buckets": buckets,
         "evaluator": evaluator,
         "matrix_free": matrix_free,

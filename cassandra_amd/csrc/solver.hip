@@ -352,9 +352,9 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     CAD_HIP(hipGetLastError());
 }
 
-// Kernel attributes of the iteration's gather (set outside graph capture).
-template <class VT>
-static void lsqr_prepare_gather(cad_matrix* m)
+// Kernel attributes of a gather with epilogue Epi (set outside graph capture).
+template <class VT, class Epi>
+static void prepare_gather(cad_matrix* m)
 {
     GatherPlan& gp = gather_plan<VT>(m);
     if (!gp.ok) return;
@@ -364,9 +364,9 @@ static void lsqr_prepare_gather(cad_matrix* m)
         with_rpt<VT, View>(gp.rpt, [&](auto rc) {
             constexpr int RPT = decltype(rc)::value;
             if constexpr (std::is_void_v<IdxT>)
-                allow_lds(k_gather_tile_mf<VT, RPT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
+                allow_lds(k_gather_tile_mf<VT, RPT, Epi>, size_t(gp.lds_elems) * sizeof(VT));
             else
-                allow_lds(k_gather_tile<VT, RPT, IdxT, EpiLsqrU<VT>>, size_t(gp.lds_elems) * sizeof(VT));
+                allow_lds(k_gather_tile<VT, RPT, IdxT, Epi>, size_t(gp.lds_elems) * sizeof(VT));
         });
     });
 }
@@ -383,7 +383,7 @@ static void lsqr_run(cad_matrix* m, Workspace& ws, int n, int batch, bool graph,
             ws.exec = nullptr;
             ws.graph = nullptr;
             prepare_scatter<VT, VT>(m);
-            lsqr_prepare_gather<VT>(m);
+            prepare_gather<VT, EpiLsqrU<VT>>(m);
             CAD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < batch; i++) lsqr_iteration<VT>(m, ws, s);
             CAD_HIP(hipStreamEndCapture(s, &ws.graph));
@@ -672,7 +672,26 @@ static void pcg_solve(cad_matrix* m, const double* b, double* x, const cad_solve
     int it = 0;
     while (it < o.max_iterations) {
         const int n = std::min(batch, o.max_iterations - it);
-        for (int i = 0; i < n; i++) pcg_iteration<VT>(m, ws, s);
+        if (o.use_graph && batch > 1 && n == batch && s != nullptr) {
+            // `batch` iterations captured once per workspace, replayed
+            if (!ws.exec || ws.graph_iters != batch) {
+                if (ws.exec) (void)hipGraphExecDestroy(ws.exec);
+                if (ws.graph) (void)hipGraphDestroy(ws.graph);
+                ws.exec = nullptr;
+                ws.graph = nullptr;
+                prepare_scatter<VT, VT>(m);
+                prepare_gather<VT, EpiStore<VT>>(m);
+                CAD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int i = 0; i < batch; i++) pcg_iteration<VT>(m, ws, s);
+                CAD_HIP(hipStreamEndCapture(s, &ws.graph));
+                CAD_HIP(hipGraphInstantiate(&ws.exec, ws.graph, nullptr, nullptr, 0));
+                ws.graph_iters = batch;
+                ws.graph_prec = int(sizeof(VT));
+            }
+            CAD_HIP(hipGraphLaunch(ws.exec, s));
+        } else {
+            for (int i = 0; i < n; i++) pcg_iteration<VT>(m, ws, s);
+        }
         it += n;
         const PcgScalars* h = read_scalars<PcgScalars>(m, ws, ws.pscal, s);
         for (int i = 0; i < m->nsys; i++)
