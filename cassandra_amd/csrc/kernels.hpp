@@ -539,7 +539,8 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
 }
 
 // Stage 2: z[c] = sum of the window's chunk slices, in chunk order
-// (deterministic); columns outside every window get 0.
+// (the order of the slices is fixed; the sums inside a slice took the LDS
+// atomics' arrival order); columns outside every window get 0.
 template <class T, class ZT>
 __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* __restrict__ work,
                                                               const T* __restrict__ part, ZT* __restrict__ z,

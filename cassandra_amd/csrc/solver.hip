@@ -75,7 +75,7 @@ PT* partials(ScatterLists& L)
 }
 
 // A^T y in two launches: LDS window tiles into partial slices, then the
-// deterministic column reduction (which writes every column of z).
+// column reduction in chunk order (which writes every column of z).
 template <class VT, class YT, class ZT, class View>
 void launch_scatter(ScatterPlan& sp, const View& view, const uint32_t* sys_col, const YT* y, const LsqrScalars* div,
                     ZT* z, hipStream_t s)
@@ -528,68 +528,89 @@ static void lsqr_plain(cad_matrix* m, const double* b, double* x, const cad_solv
     rep.converged = o.tol > 0 && worst <= o.tol;
 }
 
-// Mixed precision with iterative refinement: FP32 products inside LSQR,
-// FP64 residual r = b - A x and A^T r outside.
-static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
-                       SystemDetail& det)
+// State of a refined LSQR solve shared by its precision stages.
+struct RefineState {
+    std::vector<double> atb, rel, rn2;
+    std::vector<int> act;      // still to solve
+    std::vector<int> stalled;  // left by a stage that stopped making progress
+    int it = 0, refinements = 0;
+};
+
+// One precision stage of the refined LSQR: from the FP64 x in ws.x64,
+// repeatedly r = b - A x and A^T r in FP64, an inner LSQR with VT products on
+// min |A d - r| until its estimated normal residual fell by the target, and
+// x += d.  A system leaves the stage when it meets tol (act = 0) or when a
+// refinement cut its metric by less than 4x (stalled = 1): products in VT
+// cannot resolve that system's residual any further.  An inner solve ends
+// early when its estimate stops falling.
+template <class VT>
+static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const cad_solve_options& o, double floor_tol,
+                         int max_refinements, RefineState& st, SystemDetail& det, cad_solve_report& rep)
 {
     hipStream_t s = m->ctx->stream;
-    Workspace& ws = workspace(m, 200);
-    lsqr_alloc<float>(m, ws, s);
-    column_scaling(m, ws.D.as<double>(), o.precondition != 0, s);
-    const std::vector<double> atb = atb_norms(m, ws, b, s);
-    fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
-    std::vector<int> act(m->nsys, 1);
-    std::vector<double> rel(m->nsys, 1.0), rn2(m->nsys, 0.0), target(m->nsys, o.inner_tol);
-    det.iters.assign(m->nsys, -1);
     const int batch = std::max(1, o.check_every);
-    int it = 0;
+    std::vector<double> target(m->nsys, o.inner_tol), prev(m->nsys, 0.0), best(m->nsys), stall(m->nsys);
+    std::vector<int> act = st.act;
     for (int ref = 0;; ref++) {
         set_active(m, ws, ws.scal, act, s);
-        lsqr_start_from_residual<float>(m, ws, b, s);
+        lsqr_start_from_residual<VT>(m, ws, b, s);
         const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
         bool any = false;
         for (int i = 0; i < m->nsys; i++) {
             if (!act[i]) continue;
-            rel[i] = atb[i] > 0 ? std::sqrt(h[i].reduce_b) / atb[i] : std::sqrt(h[i].reduce_b);
-            rn2[i] = h[i].reduce_a;
-            if (o.tol > 0 && rel[i] <= o.tol) {
+            st.rel[i] = st.atb[i] > 0 ? std::sqrt(h[i].reduce_b) / st.atb[i] : std::sqrt(h[i].reduce_b);
+            st.rn2[i] = h[i].reduce_a;
+            if (o.tol > 0 && st.rel[i] <= o.tol) {
+                act[i] = st.act[i] = 0;
+                det.iters[i] = st.it;
+            } else if (ref > 0 && st.rel[i] > 0.25 * prev[i]) {
                 act[i] = 0;
-                det.iters[i] = it;
+                st.stalled[i] = 1;
             } else {
                 any = true;
                 // the last refinement only needs to close the remaining gap
-                // (with a factor 3 of margin); FP32 products cannot go much
-                // below inner_tol in one solve
-                target[i] = o.tol > 0 ? std::max(o.inner_tol, o.tol / (3.0 * rel[i])) : o.inner_tol;
+                // (with a factor 3 of margin); products in VT cannot go much
+                // below floor_tol in one solve
+                target[i] = o.tol > 0 ? std::max(floor_tol, o.tol / (3.0 * st.rel[i])) : floor_tol;
+                prev[i] = st.rel[i];
             }
         }
         rep.explicit_checks++;
         if (o.verbose && m->nsys == 1)
-            std::fprintf(stderr, "[mixed] refinement %d it %d rel %.3e\n", ref, it, rel[0]);
-        if (!any || it >= o.max_iterations || ref >= o.max_refinements) {
-            rep.refinements = ref;
-            break;
-        }
-        // freeze the converged systems for the inner solve
+            std::fprintf(stderr, "[refine %s] refinement %d it %d rel %.3e\n", sizeof(VT) == 4 ? "fp32" : "fp64",
+                         ref, st.it, st.rel[0]);
+        if (!any || st.it >= o.max_iterations || ref >= max_refinements) break;
+        st.refinements++;
+        // freeze the systems that left for the inner solve
         {
             std::vector<LsqrScalars> hv(h, h + m->nsys);
             for (int i = 0; i < m->nsys; i++) hv[i].active = act[i];
             CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * m->nsys, hipMemcpyHostToDevice, s));
         }
-        // inner FP32 LSQR until the estimated normal residual dropped by the target
+        for (int i = 0; i < m->nsys; i++) {
+            best[i] = 1.0;
+            stall[i] = 0;
+        }
+        // inner LSQR until every active system's estimate fell by its target,
+        // or stopped falling (no 1% gain over four checks)
         for (;;) {
-            const int n = std::min(batch, o.max_iterations - it);
+            const int n = std::min(batch, o.max_iterations - st.it);
             if (n <= 0) break;
-            lsqr_run<float>(m, ws, n, batch, o.use_graph != 0 && n == batch, s);
-            it += n;
+            lsqr_run<VT>(m, ws, n, batch, o.use_graph != 0 && n == batch, s);
+            st.it += n;
             h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
             bool inner_done = true;
             for (int i = 0; i < m->nsys; i++) {
                 if (!h[i].active) continue;
                 const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
                 if (!std::isfinite(est)) throw Error(CAD_ERR_BREAKDOWN, "LSQR breakdown (non-finite recurrence)");
-                if (est > target[i]) inner_done = false;
+                if (est < 0.99 * best[i]) {
+                    best[i] = est;
+                    stall[i] = 0;
+                } else {
+                    stall[i] += 1;
+                }
+                if (est > target[i] && stall[i] < 4) inner_done = false;
             }
             if (inner_done) break;
         }
@@ -598,15 +619,51 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
                            (const LsqrScalars*)ws.scal.as<LsqrScalars>());
         CAD_HIP(hipGetLastError());
     }
-    CAD_HIP(hipMemcpyAsync(x, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+}
+
+// Mixed precision with iterative refinement: FP32 products inside LSQR, FP64
+// residual r = b - A x and A^T r outside.  Systems whose FP32 refinement
+// stops converging (FP32 rounding of A^T u swamps a residual that is tiny
+// against |A| |r|, e.g. many copies of a few positions) finish with FP64
+// products from the same x.
+static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
+                       SystemDetail& det)
+{
+    hipStream_t s = m->ctx->stream;
+    Workspace& ws = workspace(m, 200);
+    lsqr_alloc<float>(m, ws, s);
+    column_scaling(m, ws.D.as<double>(), o.precondition != 0, s);
+    RefineState st;
+    st.atb = atb_norms(m, ws, b, s);
+    st.rel.assign(m->nsys, 1.0);
+    st.rn2.assign(m->nsys, 0.0);
+    st.act.assign(m->nsys, 1);
+    st.stalled.assign(m->nsys, 0);
+    det.iters.assign(m->nsys, -1);
+    fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
+    refine_stage<float>(m, ws, b, o, o.inner_tol, o.max_refinements, st, det, rep);
+    double* xs = ws.x64.as<double>();
+    bool finish = false;  // systems the FP32 stage left unconverged
+    for (int i = 0; i < m->nsys; i++) finish |= st.act[i] != 0;
+    if (finish && st.it < o.max_iterations) {
+        Workspace& w8 = workspace(m, 201);
+        lsqr_alloc<double>(m, w8, s);
+        CAD_HIP(hipMemcpyAsync(w8.D.p, ws.D.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+        CAD_HIP(hipMemcpyAsync(w8.x64.p, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+        for (int i = 0; i < m->nsys; i++) st.stalled[i] = 0;
+        refine_stage<double>(m, w8, b, o, 1e-12, o.max_refinements, st, det, rep);
+        xs = w8.x64.as<double>();
+    }
+    CAD_HIP(hipMemcpyAsync(x, xs, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
     double worst = 0, rsum = 0;
     for (int i = 0; i < m->nsys; i++) {
-        worst = std::max(worst, rel[i]);
-        rsum += rn2[i];
-        if (det.iters[i] < 0) det.iters[i] = it;
+        worst = std::max(worst, st.rel[i]);
+        rsum += st.rn2[i];
+        if (det.iters[i] < 0) det.iters[i] = st.it;
     }
-    det.rel = rel;
-    rep.iterations = it;
+    det.rel = st.rel;
+    rep.iterations = st.it;
+    rep.refinements = st.refinements;
     rep.rel_normal_residual = worst;
     rep.residual_norm = std::sqrt(rsum);
     rep.converged = o.tol > 0 && worst <= o.tol;

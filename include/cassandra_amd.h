@@ -117,9 +117,9 @@ int cad_matrix_from_positions_mf(cad_patterns* p, const uint64_t* d_positions, i
  * per thread of the LDS tile gather or the row kernel, scatter tiles per CU
  * and the cap on partial-slice traffic -- with HIP events on the context's
  * stream, keeps the fastest and drops the solvers' captured graphs.  The
- * timings and the choice go to log (may be NULL).  Results do not change: every
- * schedule sums each row in the same order and reduces the scatter partials in
- * chunk order. */
+ * timings and the choice go to log (may be NULL).  A x does not change (every
+ * schedule sums each row in the same order); A^T y stays exact on integer data
+ * and equal to rounding otherwise. */
 int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size);
 /* nsys (1 for an ordinary matrix), columns per system and, when row_begin is
  * not NULL, the nsys + 1 row boundaries. */

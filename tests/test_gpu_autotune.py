@@ -1,6 +1,6 @@
-"""The schedule autotuner: whatever schedule it picks, the products and the
-solver give the same numbers (each row is summed in the same order and the
-scatter partials are reduced in chunk order)."""
+"""The schedule autotuner: whatever schedule it picks, A x is bit-identical
+(each row is summed in the same order), A^T y is exact on integer data and the
+solver's iterates agree to rounding."""
 import sys
 
 import numpy as np

@@ -139,3 +139,28 @@ def test_argument_errors(ctx):
         P.matrix_systems(pos, [0, 50, 50, 100])  # an empty system
     with pytest.raises(ValueError):
         P.matrix_systems(pos, [0, 50])
+
+
+def test_mixed_finishes_degenerate_systems_in_fp64(ctx, oracle):
+    # positions one or two plies from the start (4 and 12 distinct positions,
+    # thousands of copies): FP32 products cannot resolve their residuals, so
+    # the refined solve must hand them to FP64 products and still meet tol
+    say("batched mixed, degenerate systems")
+    systems = [(58, 5000), (59, 3000), (30, 4000)]
+    g = oracle.Group(N.FIT4)
+    pos, idx, rb = [], [], [0]
+    for s, (e, n) in enumerate(systems):
+        p = oracle.random_play(600 + s, n, e)
+        pos.append(p)
+        idx.append(g.indices(p))
+        rb.append(rb[-1] + n)
+    P = Patterns(ctx, N.FIT4)
+    A = P.matrix_systems(torch.from_numpy(np.concatenate(pos).view(np.int64)).cuda(), rb)
+    _, b = A.make_targets(2200, 3200)
+    x, rep, rel, its = A.solve_systems(b, precision=capi.PREC_MIXED, tol=1e-10)
+    say(rep, rel, its)
+    assert rep["converged"] == 1
+    xh, bh = x.cpu().numpy(), b.cpu().numpy()
+    for s in range(len(systems)):
+        r = oracle.rel_normal_residual(idx[s], g.cols, bh[rb[s]:rb[s + 1]], xh[s * g.cols:(s + 1) * g.cols])[0]
+        assert r <= 1.5e-10, (s, r)
