@@ -387,11 +387,13 @@ cad_matrix* matrix_from_positions(cad_patterns* p, const uint64_t* d_pos, const 
     uint32_t widest = 0;
     for (size_t s = 0; s < m->slot_lo.size(); s++) widest = std::max(widest, m->slot_hi[s] - m->slot_lo[s]);
     m->ibytes = widest <= 65536 ? 2 : 4;
-    m->d_idx = DBuf(size_t(m->ibytes) * size_t(n) * size_t(g.k()));
+    m->ld = padded_rows(n);
+    m->d_idx = DBuf(size_t(m->ibytes) * size_t(m->ld) * size_t(g.k()));
+    CAD_HIP(hipMemsetAsync(m->d_idx.p, 0, m->d_idx.bytes, p->ctx->stream));
     std::vector<uint32_t> base(g.k());
     for (int s = 0; s < g.k(); s++) base[s] = m->ibytes == 2 ? m->slot_lo[s] : 0u;
     m->d_base = to_device(base, p->ctx->stream);
-    extract(p, d_pos, n, m->d_idx.p, m->ibytes, n, 1, m->ibytes == 2 ? 1 : 0);
+    extract(p, d_pos, n, m->d_idx.p, m->ibytes, m->ld, 1, m->ibytes == 2 ? 1 : 0);
     m->nsys = nsys;
     m->sys_row_begin = row_begin;
     m->cols_per_sys = cols_per;
@@ -501,16 +503,18 @@ int cad_matrix_create(cad_context* ctx, const uint32_t* h_idx, int64_t rows, int
         std::vector<uint32_t> base(k);
         for (int s = 0; s < k; s++) base[s] = m->ibytes == 2 ? lo[s] : 0u;
         m->d_base = to_device(base, ctx->stream);
-        // transpose to slot-major (and narrow) on the host
+        // transpose to slot-major (and narrow) on the host; padding rows are 0
+        m->ld = padded_rows(rows);
+        const size_t ld = size_t(m->ld);
         if (m->ibytes == 2) {
-            std::vector<uint16_t> t(size_t(rows) * k);
+            std::vector<uint16_t> t(std::max<size_t>(1, ld * k), 0);
             for (int64_t r = 0; r < rows; r++)
-                for (int s = 0; s < k; s++) t[size_t(s) * rows + r] = uint16_t(h_idx[r * k + s] - lo[s]);
+                for (int s = 0; s < k; s++) t[size_t(s) * ld + r] = uint16_t(h_idx[r * k + s] - lo[s]);
             m->d_idx = to_device(t, ctx->stream);
         } else {
-            std::vector<uint32_t> t(size_t(rows) * k);
+            std::vector<uint32_t> t(std::max<size_t>(1, ld * k), 0);
             for (int64_t r = 0; r < rows; r++)
-                for (int s = 0; s < k; s++) t[size_t(s) * rows + r] = h_idx[r * k + s];
+                for (int s = 0; s < k; s++) t[size_t(s) * ld + r] = h_idx[r * k + s];
             m->d_idx = to_device(t, ctx->stream);
         }
         build_plans(m.get());
@@ -566,17 +570,19 @@ int cad_matrix_download(const cad_matrix* m, uint32_t* out)
             for (int64_t r = 0; r < R; r++)
                 for (int j = 0; j < k; j++) out[r * k + j] = rowbase[r] + t[size_t(j) * R + r];
         } else if (m->ibytes == 2) {
-            std::vector<uint16_t> t(size_t(R) * k);
-            CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 2, hipMemcpyDeviceToHost, s));
+            const size_t ld = size_t(m->ld);
+            std::vector<uint16_t> t(ld * k);
+            if (!t.empty()) CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 2, hipMemcpyDeviceToHost, s));
             CAD_HIP(hipStreamSynchronize(s));
             for (int64_t r = 0; r < R; r++)
-                for (int j = 0; j < k; j++) out[r * k + j] = rowbase[r] + base[j] + t[size_t(j) * R + r];
+                for (int j = 0; j < k; j++) out[r * k + j] = rowbase[r] + base[j] + t[size_t(j) * ld + r];
         } else {
-            std::vector<uint32_t> t(size_t(R) * k);
-            CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 4, hipMemcpyDeviceToHost, s));
+            const size_t ld = size_t(m->ld);
+            std::vector<uint32_t> t(ld * k);
+            if (!t.empty()) CAD_HIP(hipMemcpyAsync(t.data(), m->d_idx.p, t.size() * 4, hipMemcpyDeviceToHost, s));
             CAD_HIP(hipStreamSynchronize(s));
             for (int64_t r = 0; r < R; r++)
-                for (int j = 0; j < k; j++) out[r * k + j] = rowbase[r] + base[j] + t[size_t(j) * R + r];
+                for (int j = 0; j < k; j++) out[r * k + j] = rowbase[r] + base[j] + t[size_t(j) * ld + r];
         }
     });
 }

@@ -82,9 +82,12 @@ struct LsqrScalars {
     double rnorm;             // phibar = |b - Abar y_k|
     double anorm2;            // sum of alpha^2 + beta^2 (Frobenius estimate of Abar)
     double reduce_a, reduce_b;// scratch for explicit reductions
-    int32_t active;           // 0: converged / frozen
+    double target;            // stop on the device once nres <= target * nres0 (0: never)
+    int32_t active;           // 0: converged / frozen by the host
     int32_t iters;
-    double pad;
+    int32_t stop;             // 0 running; 1 target met this iteration (its step still applies); 2 stopped
+    int32_t pad;
+    double pad2;
 };
 static_assert(sizeof(LsqrScalars) % 16 == 0, "LsqrScalars layout");
 

@@ -188,7 +188,7 @@ struct EpiLsqrU {
     __device__ bool begin(int sys)
     {
         const LsqrScalars* st = &scal[sys];
-        if (!st->active) return false;
+        if (!st->active || st->stop) return false;
         const double beta = st->beta;
         f = VT(beta != 0.0 ? st->alpha / beta : 0.0);
         return true;
@@ -292,14 +292,82 @@ __global__ __launch_bounds__(kRowBlock) void k_gather_rows(View m, int k, const 
     reduce_tail<kRowBlock>(epi, acc, w.sys, g);
 }
 
-// LDS-staged gather for pattern matrices.  A block owns a tile of up to
-// kTileBlock * RPT rows (RPT rows per thread, accumulators in registers) and
-// walks the slot segments in slot order; each segment's column block of x is
-// staged in LDS (coalesced) and gathered from there, so the random reads hit
-// LDS instead of pulling a cache line from L2 per element.  Segments wider than
-// the LDS (wmax columns) are staged window by window: each slot's value is
-// kept in registers until every window was seen, then the segment's slots are
-// added in slot order, so the sum order is the sequential one.
+// RPT consecutive entries of one slot column in one aligned vector load.
+template <class IdxT, int RPT>
+struct IdxVec;
+template <>
+struct IdxVec<uint16_t, 1> {
+    using T = uint16_t;
+    static __device__ __forceinline__ uint32_t get(T v, int) { return v; }
+};
+template <>
+struct IdxVec<uint32_t, 1> {
+    using T = uint32_t;
+    static __device__ __forceinline__ uint32_t get(T v, int) { return v; }
+};
+template <>
+struct IdxVec<uint16_t, 2> {
+    using T = uint32_t;
+    static __device__ __forceinline__ uint32_t get(T v, int i) { return (v >> (16 * i)) & 0xFFFFu; }
+};
+template <>
+struct IdxVec<uint16_t, 4> {
+    using T = uint2;
+    static __device__ __forceinline__ uint32_t get(T v, int i)
+    {
+        const uint32_t w = i < 2 ? v.x : v.y;
+        return (w >> (16 * (i & 1))) & 0xFFFFu;
+    }
+};
+template <>
+struct IdxVec<uint16_t, 8> {
+    using T = uint4;
+    static __device__ __forceinline__ uint32_t get(T v, int i)
+    {
+        const uint32_t w = i < 2 ? v.x : i < 4 ? v.y : i < 6 ? v.z : v.w;
+        return (w >> (16 * (i & 1))) & 0xFFFFu;
+    }
+};
+template <>
+struct IdxVec<uint32_t, 2> {
+    using T = uint2;
+    static __device__ __forceinline__ uint32_t get(T v, int i) { return i == 0 ? v.x : v.y; }
+};
+template <>
+struct IdxVec<uint32_t, 4> {
+    using T = uint4;
+    static __device__ __forceinline__ uint32_t get(T v, int i)
+    {
+        return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+    }
+};
+
+// Copies x[0, W) into LDS, four independent loads per thread in flight.
+template <class VT, int BS>
+__device__ __forceinline__ void stage(VT* __restrict__ xs, const VT* __restrict__ xg, uint32_t W)
+{
+    uint32_t i = threadIdx.x;
+    for (; i + 3 * BS < W; i += 4 * BS) {
+        const VT a0 = xg[i], a1 = xg[i + BS], a2 = xg[i + 2 * BS], a3 = xg[i + 3 * BS];
+        xs[i] = a0;
+        xs[i + BS] = a1;
+        xs[i + 2 * BS] = a2;
+        xs[i + 3 * BS] = a3;
+    }
+    for (; i < W; i += BS) xs[i] = xg[i];
+}
+
+// LDS-staged gather for pattern matrices.  A block owns a tile of rows and
+// walks the slot segments (runs of slots over one column block) in slot
+// order; each segment's block of x is staged in LDS and the random reads hit
+// LDS instead of pulling an L2 line per element.  Thread t owns the RPT
+// consecutive rows starting at the tile's first row rounded down to RPT plus
+// RPT * t, so one aligned vector load brings a slot's entries of all its rows;
+// the loads of up to kMaxSegSlots slots are issued before the segment is
+// staged and overlap it.  Segments wider than the LDS (wmax columns) are
+// staged window by window: each slot's value is kept in registers until every
+// window was seen, then the segment's slots are added in slot order, so every
+// row is the sequential sum in slot order (Math/MatrixCSR.cpp:59-62).
 constexpr int kTileBlock = 1024;
 constexpr int kMaxSegSlots = 8;
 
@@ -310,56 +378,78 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                                                             uint32_t wmax, const VT* __restrict__ x, Epi epi,
                                                             GridSys g)
 {
+    using V = IdxVec<IdxT, RPT>;
+    using VecT = typename V::T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     VT* xs = reinterpret_cast<VT*>(smem_raw);
     const RowWork w = work[blockIdx.x];
     if (!epi.begin(w.sys)) return;
     const uint32_t cb = sys_col[w.sys];
-    // rows of this thread: r0 + i * kTileBlock for i * kTileBlock < nval
-    const int64_t r0 = w.row_begin + threadIdx.x;
-    const int nval = int(max<int64_t>(0, min<int64_t>(w.row_end - r0, int64_t(RPT) * kTileBlock)));
+    const int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x;
+    const bool live = a < w.row_end;
+    bool ok[RPT];
     VT acc[RPT];
 #pragma unroll
-    for (int i = 0; i < RPT; i++) acc[i] = VT(0);
+    for (int i = 0; i < RPT; i++) {
+        ok[i] = live && a + i >= w.row_begin && a + i < w.row_end;
+        acc[i] = VT(0);
+    }
+    // slot s of this thread's rows: vec[s * vld]
+    const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + (live ? a : 0));
+    const int64_t vld = m.ld / RPT;
     for (int sg = 0; sg < nseg; sg++) {
         const GatherSeg seg = segs[sg];
         const VT* __restrict__ xg = x + cb + seg.col_lo;
+        const int s_end = seg.slot_first + seg.nslots;
         if (seg.nwin == 1) {
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < seg.width; i += kTileBlock) xs[i] = xg[i];
-            __syncthreads();
-            for (int s = seg.slot_first; s < seg.slot_first + seg.nslots; s++) {
-                const IdxT* __restrict__ p = m.idx + int64_t(s) * m.ld + r0;
-                // local offset in 32-bit arithmetic: base - col_lo wraps for
-                // u32 storage (base 0) and the sum with the index unwraps it
-                const uint32_t off = m.base[s] - seg.col_lo;
+            for (int s0 = seg.slot_first; s0 < s_end; s0 += kMaxSegSlots) {
+                VecT q[kMaxSegSlots];
 #pragma unroll
-                for (int i = 0; i < RPT; i++)
-                    if (i * kTileBlock < nval) acc[i] += xs[off + uint32_t(p[i * kTileBlock])];
+                for (int j = 0; j < kMaxSegSlots; j++) {
+                    q[j] = VecT{};
+                    if (live && s0 + j < s_end) q[j] = vec[int64_t(s0 + j) * vld];
+                }
+                if (s0 == seg.slot_first) {
+                    __syncthreads();
+                    stage<VT, kTileBlock>(xs, xg, seg.width);
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int j = 0; j < kMaxSegSlots; j++) {
+                    if (s0 + j < s_end) {
+                        // 32-bit offset: base - col_lo wraps for u32 storage
+                        // (base 0) and the sum with the entry unwraps it
+                        const uint32_t off = m.base[s0 + j] - seg.col_lo;
+#pragma unroll
+                        for (int i = 0; i < RPT; i++)
+                            if (ok[i]) acc[i] += xs[off + V::get(q[j], i)];
+                    }
+                }
             }
         } else {
+            // at most kMaxSegSlots slots (checked when the plan is built)
+            VecT q[kMaxSegSlots];
             VT v[RPT][kMaxSegSlots];
 #pragma unroll
-            for (int j = 0; j < kMaxSegSlots; j++)
+            for (int j = 0; j < kMaxSegSlots; j++) {
+                q[j] = VecT{};
+                if (live && j < seg.nslots) q[j] = vec[int64_t(seg.slot_first + j) * vld];
 #pragma unroll
                 for (int i = 0; i < RPT; i++) v[i][j] = VT(0);
+            }
             for (uint32_t wl = 0; wl < seg.width; wl += wmax) {
                 const uint32_t W = min(wmax, seg.width - wl);
                 __syncthreads();
-                for (uint32_t i = threadIdx.x; i < W; i += kTileBlock) xs[i] = xg[wl + i];
+                stage<VT, kTileBlock>(xs, xg + wl, W);
                 __syncthreads();
 #pragma unroll
                 for (int j = 0; j < kMaxSegSlots; j++) {
                     if (j < seg.nslots) {
-                        const int s = seg.slot_first + j;
-                        const IdxT* __restrict__ p = m.idx + int64_t(s) * m.ld + r0;
-                        const uint32_t off = m.base[s] - seg.col_lo - wl;
+                        const uint32_t off = m.base[seg.slot_first + j] - seg.col_lo - wl;
 #pragma unroll
                         for (int i = 0; i < RPT; i++) {
-                            if (i * kTileBlock < nval) {
-                                const uint32_t c = off + uint32_t(p[i * kTileBlock]);
-                                if (c < W) v[i][j] = xs[c];
-                            }
+                            const uint32_t c = off + V::get(q[j], i);
+                            if (ok[i] && c < W) v[i][j] = xs[c];
                         }
                     }
                 }
@@ -375,7 +465,7 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
     double red = 0.0;
 #pragma unroll
     for (int i = 0; i < RPT; i++)
-        if (i * kTileBlock < nval) red += epi.row(r0 + int64_t(i) * kTileBlock, acc[i]);
+        if (ok[i]) red += epi.row(a + i, acc[i]);
     reduce_tail<kTileBlock>(epi, red, w.sys, g);
 }
 
@@ -514,7 +604,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
     extern __shared__ __align__(16) unsigned char smem_raw[];
     VT* win = reinterpret_cast<VT*>(smem_raw);
     const ScatterWork w = work[blockIdx.x];
-    if (div && !div[w.sys].active) return;
+    if (div && (!div[w.sys].active || div[w.sys].stop)) return;
     const uint32_t W = w.win_hi - w.win_lo;
     for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = VT(0);
     __syncthreads();
@@ -538,6 +628,76 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
     for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = win[i];
 }
 
+// Stage 1 for stored indices: as k_scatter_part, with each thread taking RPT
+// consecutive rows per step (counted from the chunk's first row rounded down
+// to RPT) so one aligned vector load brings a slot's entries of all of them,
+// and the loads of up to kMaxSegSlots slots in flight before their atomics.
+template <class VT, class YT, class IdxT, int RPT>
+__global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> m,
+                                                                const ScatterWork* __restrict__ work,
+                                                                const SlotGroup* __restrict__ groups,
+                                                                const uint16_t* __restrict__ slot_list,
+                                                                const uint32_t* __restrict__ sys_col,
+                                                                const YT* __restrict__ y, const LsqrScalars* div,
+                                                                VT* __restrict__ part)
+{
+    using V = IdxVec<IdxT, RPT>;
+    using VecT = typename V::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    VT* win = reinterpret_cast<VT*>(smem_raw);
+    const ScatterWork w = work[blockIdx.x];
+    if (div && (!div[w.sys].active || div[w.sys].stop)) return;
+    const uint32_t W = w.win_hi - w.win_lo;
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = VT(0);
+    __syncthreads();
+    const SlotGroup gr = groups[w.group];
+    const uint32_t cb = sys_col[w.sys];
+    VT sc = VT(1);
+    if (div) {
+        const double d = div[w.sys].beta;
+        sc = VT(d != 0.0 ? 1.0 / d : 0.0);
+    }
+    const int64_t vld = m.ld / RPT;
+    for (int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x; a < w.row_end;
+         a += int64_t(RPT) * kScatterBlock) {
+        bool ok[RPT];
+        VT yv[RPT];
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            ok[i] = a + i >= w.row_begin && a + i < w.row_end;
+            yv[i] = ok[i] ? VT(y[a + i]) * sc : VT(0);
+        }
+        const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + a);
+        for (int j0 = 0; j0 < gr.nslots; j0 += kMaxSegSlots) {
+            VecT q[kMaxSegSlots];
+            uint32_t off[kMaxSegSlots];
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++) {
+                q[j] = VecT{};
+                off[j] = 0;
+                if (j0 + j < gr.nslots) {
+                    const int s = slot_list[gr.first + j0 + j];
+                    q[j] = vec[int64_t(s) * vld];
+                    off[j] = cb + m.base[s] - w.win_lo;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++) {
+                if (j0 + j < gr.nslots) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) {
+                        const uint32_t uc = off[j] + V::get(q[j], i);
+                        if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    VT* __restrict__ out = part + w.part_off;
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = win[i];
+}
+
 // Stage 2: z[c] = sum of the window's chunk slices, in chunk order
 // (the order of the slices is fixed; the sums inside a slice took the LDS
 // atomics' arrival order); columns outside every window get 0.
@@ -547,7 +707,7 @@ __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* 
                                                               const LsqrScalars* div)
 {
     const ReduceWork w = work[blockIdx.x];
-    if (div && !div[w.sys].active) return;
+    if (div && (!div[w.sys].active || div[w.sys].stop)) return;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         const T* __restrict__ p = part + w.part_off + (c - w.col_begin);
         T s = T(0);
@@ -616,20 +776,29 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict_
     const ColWork w = work[blockIdx.x];
     LsqrScalars* st = &scal[w.sys];
     if (!st->active) return;
+    // every block reads stop before it arrives, so the last block's update
+    // below is seen only by the next launch
+    const bool stopped = st->stop != 0;
     const double beta = st->beta;
     double acc = 0.0;
-    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
-        const double vb = D[c] * double(z[c]) - beta * v[c];
-        v[c] = vb;
-        acc += vb * vb;
-    }
+    if (!stopped)
+        for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+            const double vb = D[c] * double(z[c]) - beta * v[c];
+            v[c] = vb;
+            acc += vb * vb;
+        }
     acc = block_sum<kRowBlock>(acc, sh);
     if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
     if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
         double s = 0.0;
         for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
         s = block_sum<kRowBlock>(s, sh);
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && stopped) {
+            // the iteration after the stop: no further step
+            st->stop = 2;
+            st->step_x = 0.0;
+            st->step_w = 0.0;
+        } else if (threadIdx.x == 0) {
             const double alpha = sqrt(s);
             const double rho = sqrt(st->rhobar * st->rhobar + beta * beta);
             const double c = rho != 0.0 ? st->rhobar / rho : 0.0;
@@ -645,6 +814,9 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict_
             st->rnorm = st->phibar;
             st->anorm2 += s;
             st->iters += 1;
+            // met the target: this iteration's step is the last (a rank-
+            // deficient system that continued would iterate on rounding noise)
+            if (st->target > 0.0 && st->nres <= st->target * st->nres0) st->stop = 1;
         }
     }
 }
@@ -659,7 +831,7 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_xw(const ColWork* __restrict
 {
     const ColWork w = work[blockIdx.x];
     const LsqrScalars* st = &scal[w.sys];
-    if (!st->active) return;
+    if (!st->active || st->stop > 1) return;
     const double ia = st->alpha != 0.0 ? 1.0 / st->alpha : 0.0;
     const double sx = st->step_x, sw = st->step_w;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
@@ -717,6 +889,7 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_v(const ColWork* __rest
             st->anorm2 = s0 + st->beta * 0.0;
             st->reduce_b = s1;
             st->iters = 0;
+            st->stop = 0;
         }
     }
 }

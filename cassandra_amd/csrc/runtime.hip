@@ -259,9 +259,12 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
         next = first + g->nslots;
     }
     if (next != m->k || gp.segs.empty()) return;
-    // row tiles of at most kTileBlock * rpt rows; a single system is cut into a
-    // multiple of the CU count so that every CU gets the same number of tiles
-    const int64_t cap = int64_t(kTileBlock) * rpt;
+    // Row tiles: a thread owns rpt consecutive rows counted from the tile's
+    // first row rounded down to rpt, so a tile spans at most kTileBlock * rpt
+    // rows from there; tiles of a system are a multiple of rpt rows long (their
+    // misalignment is the system's).  A single system is cut into a multiple of
+    // the CU count so that every CU gets the same number of tiles.
+    const int64_t cap = int64_t(kTileBlock) * rpt - (rpt - 1);
     gp.tiles.clear();
     gp.first.assign(m->nsys, 0);
     gp.count.assign(m->nsys, 0);
@@ -270,8 +273,9 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
         gp.first[s] = int32_t(gp.tiles.size());
         int64_t n = (R + cap - 1) / cap;
         if (m->nsys == 1 && n > 0) n = (n + ctx.cus - 1) / ctx.cus * ctx.cus;
-        const int64_t per = n > 0 ? (R + n - 1) / n : 0;
-        for (int64_t t = 0; t < n && t * per < R; t++)
+        int64_t per = n > 0 ? (R + n - 1) / n : 0;
+        per = std::min(cap / rpt * rpt, (per + rpt - 1) / rpt * rpt);
+        for (int64_t t = 0; per > 0 && t * per < R; t++)
             gp.tiles.push_back(RowWork{r0 + t * per, r0 + std::min(R, (t + 1) * per), s, 0});
         gp.count[s] = int32_t(gp.tiles.size()) - gp.first[s];
     }

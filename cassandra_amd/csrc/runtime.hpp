@@ -100,6 +100,12 @@ struct TuneParams {
     double partial_cap = 0.1;   // partial-slice traffic cap as a fraction of index traffic
 };
 
+// Slot columns of a stored index matrix start at multiples of this many rows,
+// so a thread can load the entries of up to 8 consecutive rows of one slot in
+// one aligned vector load.
+constexpr int64_t kIndexAlign = 64;
+inline int64_t padded_rows(int64_t rows) { return (rows + kIndexAlign - 1) / kIndexAlign * kIndexAlign; }
+
 struct Workspace;
 
 struct Context {
@@ -130,7 +136,8 @@ struct cad_matrix {
     int64_t rows = 0, cols = 0;
     int k = 0;
     int ibytes = 4;                 // 2: u16 local + slot base, 4: u32 with base 0, 0: matrix-free
-    cad::DBuf d_idx, d_base;        // slot-major
+    cad::DBuf d_idx, d_base;        // slot-major: entry (r, s) at d_idx[s * ld + r]
+    int64_t ld = 0;                 // rows rounded up to kIndexAlign (rows r >= rows are zero)
     // matrix-free: the positions and a copy of the pattern group's slot programs
     bool mf = false;
     cad::DBuf d_pos, d_slots, d_sq, d_dense;

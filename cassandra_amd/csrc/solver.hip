@@ -22,9 +22,9 @@ void with_view(cad_matrix* m, F&& f)
         f(PositionsView{m->d_pos.as<uint64_t>(), m->d_slots.as<DevSlot>(), m->d_sq.as<uint8_t>(),
                         m->d_dense.as<uint32_t>()});
     else if (m->ibytes == 2)
-        f(SlotMajor<uint16_t>{m->d_idx.as<uint16_t>(), m->rows, m->d_base.as<uint32_t>()});
+        f(SlotMajor<uint16_t>{m->d_idx.as<uint16_t>(), m->ld, m->d_base.as<uint32_t>()});
     else
-        f(SlotMajor<uint32_t>{m->d_idx.as<uint32_t>(), m->rows, m->d_base.as<uint32_t>()});
+        f(SlotMajor<uint32_t>{m->d_idx.as<uint32_t>(), m->ld, m->d_base.as<uint32_t>()});
 }
 
 int grid_for(int64_t n, int bs = 256)
@@ -76,6 +76,17 @@ PT* partials(ScatterLists& L)
 
 // A^T y in two launches: LDS window tiles into partial slices, then the
 // column reduction in chunk order (which writes every column of z).
+// The first-stage scatter kernel of a view: vector loads of 4 consecutive
+// rows for stored indices, one row per step otherwise.
+template <class VT, class YT, class View>
+constexpr auto scatter_kernel()
+{
+    if constexpr (std::is_same_v<View, SlotMajor<uint16_t>> || std::is_same_v<View, SlotMajor<uint32_t>>)
+        return k_scatter_tile<VT, YT, typename View::index_type, 4>;
+    else
+        return k_scatter_part<VT, YT, View>;
+}
+
 template <class VT, class YT, class ZT, class View>
 void launch_scatter(ScatterPlan& sp, const View& view, const uint32_t* sys_col, const YT* y, const LsqrScalars* div,
                     ZT* z, hipStream_t s)
@@ -84,7 +95,7 @@ void launch_scatter(ScatterPlan& sp, const View& view, const uint32_t* sys_col, 
     VT* part = partials<VT>(L);
     if (!L.work.empty()) {
         const size_t lds = size_t(L.widest) * sizeof(VT);
-        auto kern = k_scatter_part<VT, YT, View>;
+        auto kern = scatter_kernel<VT, YT, View>();
         allow_lds(kern, lds);
         hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
                            L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(), sp.d_slot_list.as<uint16_t>(),
@@ -104,7 +115,7 @@ void prepare_scatter(cad_matrix* m)
     ScatterLists& L = lists<VT>(m->splan);
     partials<VT>(L);
     const size_t lds = size_t(L.widest) * sizeof(VT);
-    with_view(m, [&](auto view) { allow_lds(k_scatter_part<VT, YT, decltype(view)>, lds); });
+    with_view(m, [&](auto view) { allow_lds(scatter_kernel<VT, YT, decltype(view)>(), lds); });
 }
 
 template <class VT, class YT, class ZT>
@@ -122,15 +133,17 @@ GatherPlan& gather_plan(cad_matrix* m)
 }
 
 // The tile gather's rows per thread is a schedule knob: the instantiated
-// values are 2, 4 (FP32) and 1, 2 (FP64), the ones whose registers fit a
-// 1024-thread block without spilling; f receives the value as a compile-time
-// constant.
+// values are 2, 4 (FP32; and 8 for u16 indices) and 1, 2 (FP64), the ones
+// whose registers fit a 1024-thread block without spilling; f receives the
+// value as a compile-time constant.
 template <class VT, class View, class F>
 void with_rpt(int rpt, F&& f)
 {
     if constexpr (sizeof(VT) == 4) {
         if (rpt == 2) return f(std::integral_constant<int, 2>{});
         if (rpt == 4) return f(std::integral_constant<int, 4>{});
+        if constexpr (std::is_same_v<typename View::index_type, uint16_t>)
+            if (rpt == 8) return f(std::integral_constant<int, 8>{});
     } else {
         if (rpt == 1) return f(std::integral_constant<int, 1>{});
         if (rpt == 2) return f(std::integral_constant<int, 2>{});
@@ -541,8 +554,7 @@ struct RefineState {
 // min |A d - r| until its estimated normal residual fell by the target, and
 // x += d.  A system leaves the stage when it meets tol (act = 0) or when a
 // refinement cut its metric by less than 4x (stalled = 1): products in VT
-// cannot resolve that system's residual any further.  An inner solve ends
-// early when its estimate stops falling.
+// cannot resolve that system's residual any further.
 template <class VT>
 static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const cad_solve_options& o, double floor_tol,
                          int max_refinements, RefineState& st, SystemDetail& det, cad_solve_report& rep)
@@ -581,19 +593,25 @@ static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const ca
                          ref, st.it, st.rel[0]);
         if (!any || st.it >= o.max_iterations || ref >= max_refinements) break;
         st.refinements++;
-        // freeze the systems that left for the inner solve
+        // freeze the systems that left; the others stop on the device at their target
         {
             std::vector<LsqrScalars> hv(h, h + m->nsys);
-            for (int i = 0; i < m->nsys; i++) hv[i].active = act[i];
+            for (int i = 0; i < m->nsys; i++) {
+                hv[i].active = act[i];
+                hv[i].target = act[i] ? target[i] : 0.0;
+                hv[i].stop = 0;
+            }
             CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * m->nsys, hipMemcpyHostToDevice, s));
         }
         for (int i = 0; i < m->nsys; i++) {
             best[i] = 1.0;
             stall[i] = 0;
         }
-        // inner LSQR until every active system's estimate fell by its target,
-        // or stopped falling (no 1% gain over four checks)
-        for (;;) {
+        // inner LSQR until every active system's estimate fell by its target.
+        // The estimate of |A^T r| is not monotone in LSQR and can plateau, so
+        // a system only counts as stuck when its best estimate did not improve
+        // by 1% for as many checks as the inner solve has run (at least 16).
+        for (int checks = 1;; checks++) {
             const int n = std::min(batch, o.max_iterations - st.it);
             if (n <= 0) break;
             lsqr_run<VT>(m, ws, n, batch, o.use_graph != 0 && n == batch, s);
@@ -610,7 +628,8 @@ static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const ca
                 } else {
                     stall[i] += 1;
                 }
-                if (est > target[i] && stall[i] < 4) inner_done = false;
+                const bool stuck = stall[i] >= std::max(16, checks / 2 + 1) && stall[i] * 2 >= checks;
+                if (est > target[i] && !stuck) inner_done = false;
             }
             if (inner_done) break;
         }
@@ -621,17 +640,20 @@ static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const ca
     }
 }
 
-// Mixed precision with iterative refinement: FP32 products inside LSQR, FP64
-// residual r = b - A x and A^T r outside.  Systems whose FP32 refinement
-// stops converging (FP32 rounding of A^T u swamps a residual that is tiny
-// against |A| |r|, e.g. many copies of a few positions) finish with FP64
-// products from the same x.
-static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
-                       SystemDetail& det)
+// Refined LSQR.  MIXED: FP32 products inside LSQR, FP64 residual
+// r = b - A x and A^T r outside; systems whose FP32 refinement stops
+// converging (FP32 rounding of A^T u swamps a residual that is tiny against
+// |A| |r|) finish with FP64 products from the same x.  FP64: the FP64 stage
+// alone from x = 0, whose first inner solve is plain LSQR (Math/Solver.h:146-
+// 176) stopped on the device at tol / 3, then confirmed (or refined) by the
+// explicit FP64 metric.
+static void lsqr_refined(cad_matrix* m, const double* b, double* x, const cad_solve_options& o,
+                         cad_solve_report& rep, SystemDetail& det, bool mixed)
 {
     hipStream_t s = m->ctx->stream;
-    Workspace& ws = workspace(m, 200);
-    lsqr_alloc<float>(m, ws, s);
+    Workspace& ws = workspace(m, mixed ? 200 : 201);
+    if (mixed) lsqr_alloc<float>(m, ws, s);
+    else lsqr_alloc<double>(m, ws, s);
     column_scaling(m, ws.D.as<double>(), o.precondition != 0, s);
     RefineState st;
     st.atb = atb_norms(m, ws, b, s);
@@ -641,18 +663,22 @@ static void lsqr_mixed(cad_matrix* m, const double* b, double* x, const cad_solv
     st.stalled.assign(m->nsys, 0);
     det.iters.assign(m->nsys, -1);
     fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
-    refine_stage<float>(m, ws, b, o, o.inner_tol, o.max_refinements, st, det, rep);
     double* xs = ws.x64.as<double>();
-    bool finish = false;  // systems the FP32 stage left unconverged
-    for (int i = 0; i < m->nsys; i++) finish |= st.act[i] != 0;
-    if (finish && st.it < o.max_iterations) {
-        Workspace& w8 = workspace(m, 201);
-        lsqr_alloc<double>(m, w8, s);
-        CAD_HIP(hipMemcpyAsync(w8.D.p, ws.D.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
-        CAD_HIP(hipMemcpyAsync(w8.x64.p, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
-        for (int i = 0; i < m->nsys; i++) st.stalled[i] = 0;
-        refine_stage<double>(m, w8, b, o, 1e-12, o.max_refinements, st, det, rep);
-        xs = w8.x64.as<double>();
+    if (mixed) {
+        refine_stage<float>(m, ws, b, o, o.inner_tol, o.max_refinements, st, det, rep);
+        bool finish = false;  // systems the FP32 stage left unconverged
+        for (int i = 0; i < m->nsys; i++) finish |= st.act[i] != 0;
+        if (finish && st.it < o.max_iterations) {
+            Workspace& w8 = workspace(m, 201);
+            lsqr_alloc<double>(m, w8, s);
+            CAD_HIP(hipMemcpyAsync(w8.D.p, ws.D.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+            CAD_HIP(hipMemcpyAsync(w8.x64.p, ws.x64.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+            for (int i = 0; i < m->nsys; i++) st.stalled[i] = 0;
+            refine_stage<double>(m, w8, b, o, 0.0, o.max_refinements, st, det, rep);
+            xs = w8.x64.as<double>();
+        }
+    } else {
+        refine_stage<double>(m, ws, b, o, 0.0, o.max_refinements, st, det, rep);
     }
     CAD_HIP(hipMemcpyAsync(x, xs, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
     double worst = 0, rsum = 0;
@@ -782,9 +808,8 @@ void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o
     SystemDetail det;
     auto t0 = std::chrono::steady_clock::now();
     if (o.solver == CAD_SOLVER_LSQR) {
-        if (o.precision == CAD_PREC_MIXED) lsqr_mixed(m, b, x, o, rep, det);
-        else if (o.precision == CAD_PREC_FP32) lsqr_plain<float>(m, b, x, o, rep, det);
-        else lsqr_plain<double>(m, b, x, o, rep, det);
+        if (o.precision == CAD_PREC_FP32) lsqr_plain<float>(m, b, x, o, rep, det);
+        else lsqr_refined(m, b, x, o, rep, det, o.precision == CAD_PREC_MIXED);
     } else if (o.solver == CAD_SOLVER_PCG || o.solver == CAD_SOLVER_CG) {
         const bool jac = o.solver == CAD_SOLVER_PCG;
         if (o.precision == CAD_PREC_FP32) pcg_solve<float>(m, b, x, o, rep, jac, det);
@@ -876,7 +901,8 @@ std::string autotune(cad_matrix* m)
     TuneParams& tp = m->tune;
     // FP32 gather: tile rows per thread, or the row kernel (0)
     {
-        const std::vector<int> cands{4, 2, 0};
+        const std::vector<int> cands = (!m->mf && m->ibytes == 2) ? std::vector<int>{4, 8, 2, 0}
+                                                                  : std::vector<int>{4, 2, 0};
         int best = tp.rpt4;
         double best_us = 1e300;
         for (int c : cands) {

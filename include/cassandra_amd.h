@@ -150,10 +150,13 @@ int cad_jacobi(cad_matrix* m, float* d_out);
 #define CAD_SOLVER_CG 2    /* CG on A^T A */
 #define CAD_SOLVER_LSMR 3
 
-#define CAD_PREC_FP64 0    /* FP64 products and vectors */
+#define CAD_PREC_FP64 0    /* FP64 products and vectors; LSQR stops on the device at
+                              tol / 3 and restarts from the FP64 residual if the
+                              explicit metric is not met */
 #define CAD_PREC_FP32 1    /* FP32 products and vectors (the reference's precision) */
 #define CAD_PREC_MIXED 2   /* FP32 products, FP64 scalars/dots/norms/x and FP64
-                              refinement residual r = b - A x, A^T r (LSQR only) */
+                              refinement residual r = b - A x, A^T r (LSQR only);
+                              systems FP32 cannot resolve finish in FP64 */
 
 typedef struct cad_solve_options {
     int solver;           /* CAD_SOLVER_* */
