@@ -201,33 +201,41 @@ def leg_buckets(ctx):
     A = P.matrix_systems(torch.cat(parts), rb)
     _, b = A.make_targets(2003, 3003)
     log(f"buckets: {A.nsys} systems, {A.rows} rows, k={A.k}, {A.cols} columns")
-    opts = dict(precision=capi.PREC_MIXED, tol=1e-10)
-    A.solve_systems(b, **opts)  # warm: workspace, graph
+    # some small buckets are slow to reach 1e-10 in the normal-residual metric:
+    # both runs stop at the same iteration budget and report what each bucket reached
+    opts = dict(precision=capi.PREC_MIXED, tol=1e-10, max_iterations=4000)
+    A.solve_systems(b, **dict(opts, max_iterations=50))  # warm: workspace, graph
     ctx.synchronize()
     t0 = time.perf_counter()
     x, rep, rel, its = A.solve_systems(b, **opts)
     ctx.synchronize()
     t_batch = time.perf_counter() - t0
-    log(f"buckets batched: {t_batch:.4f} s, {rep}")
+    log(f"buckets batched: {t_batch:.4f} s, {rep}, {int(np.sum(rel <= 1e-10))} buckets at 1e-10")
     # the same fits one system at a time (each warmed first)
     mats = [P.matrix(p_) for p_ in parts]
     del parts
     bs = [b[int(rb[i]):int(rb[i + 1])].contiguous() for i in range(len(mats))]
     for M, bi in zip(mats, bs):
-        M.solve(bi, **opts)
+        M.solve(bi, **dict(opts, max_iterations=50))
     ctx.synchronize()
+    log("buckets: one-by-one warm done")
     t0 = time.perf_counter()
     seq_rel = []
-    for M, bi in zip(mats, bs):
+    for i, (M, bi) in enumerate(zip(mats, bs)):
         _, r = M.solve(bi, **opts)
         seq_rel.append(r["rel_normal_residual"])
+        if i % 10 == 9:
+            log(f"buckets one by one: {i + 1} done")
     ctx.synchronize()
     t_seq = time.perf_counter() - t0
-    log(f"buckets one by one: {t_seq:.4f} s")
+    seq_rel = np.array(seq_rel)
+    log(f"buckets one by one: {t_seq:.4f} s, {int(np.sum(seq_rel <= 1e-10))} buckets at 1e-10")
     return {"systems": int(A.nsys), "rows": int(A.rows), "k": int(A.k), "cols_per_system": int(A.cols_per_system),
-            "seconds_batched": t_batch, "seconds_one_by_one": t_seq, "speedup": t_seq / t_batch,
-            "iterations": rep["iterations"], "converged": rep["converged"],
-            "max_rel_normal_residual": float(np.max(rel)), "median_iterations": float(np.median(its)),
+            "max_iterations": opts["max_iterations"], "seconds_batched": t_batch, "seconds_one_by_one": t_seq,
+            "speedup": t_seq / t_batch, "iterations": rep["iterations"],
+            "buckets_at_tol": int(np.sum(rel <= 1e-10)), "max_rel_normal_residual": float(np.max(rel)),
+            "median_rel_normal_residual": float(np.median(rel)), "median_iterations": float(np.median(its)),
+            "buckets_at_tol_one_by_one": int(np.sum(seq_rel <= 1e-10)),
             "max_rel_one_by_one": float(np.max(seq_rel))}
 
 

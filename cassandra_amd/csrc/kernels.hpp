@@ -630,8 +630,10 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
 
 // Stage 1 for stored indices: as k_scatter_part, with each thread taking RPT
 // consecutive rows per step (counted from the chunk's first row rounded down
-// to RPT) so one aligned vector load brings a slot's entries of all of them,
-// and the loads of up to kMaxSegSlots slots in flight before their atomics.
+// to RPT) so one aligned vector load brings a slot's entries of all of them.
+// For groups of up to kMaxSegSlots slots the slot numbers and window offsets
+// are hoisted out of the row loop, so each step issues its y and index loads
+// back to back before the atomics.
 template <class VT, class YT, class IdxT, int RPT>
 __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> m,
                                                                 const ScatterWork* __restrict__ work,
@@ -651,6 +653,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = VT(0);
     __syncthreads();
     const SlotGroup gr = groups[w.group];
+    const int ns = gr.nslots;
     const uint32_t cb = sys_col[w.sys];
     VT sc = VT(1);
     if (div) {
@@ -658,6 +661,19 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
         sc = VT(d != 0.0 ? 1.0 / d : 0.0);
     }
     const int64_t vld = m.ld / RPT;
+    const int64_t last = w.row_end - 1;
+    int64_t slot[kMaxSegSlots];
+    uint32_t off[kMaxSegSlots];
+#pragma unroll
+    for (int j = 0; j < kMaxSegSlots; j++) {
+        slot[j] = 0;
+        off[j] = 0;
+        if (j < ns) {
+            const int sj = slot_list[gr.first + j];
+            slot[j] = int64_t(sj) * vld;
+            off[j] = cb + m.base[sj] - w.win_lo;
+        }
+    }
     for (int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x; a < w.row_end;
          a += int64_t(RPT) * kScatterBlock) {
         bool ok[RPT];
@@ -665,30 +681,33 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             ok[i] = a + i >= w.row_begin && a + i < w.row_end;
-            yv[i] = ok[i] ? VT(y[a + i]) * sc : VT(0);
+            const VT t = VT(y[min(a + i, last)]) * sc;  // clamped: no branch around the load
+            yv[i] = ok[i] ? t : VT(0);
         }
         const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + a);
-        for (int j0 = 0; j0 < gr.nslots; j0 += kMaxSegSlots) {
+        if (ns <= kMaxSegSlots) {
             VecT q[kMaxSegSlots];
-            uint32_t off[kMaxSegSlots];
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++) q[j] = j < ns ? vec[slot[j]] : VecT{};
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++) {
-                q[j] = VecT{};
-                off[j] = 0;
-                if (j0 + j < gr.nslots) {
-                    const int s = slot_list[gr.first + j0 + j];
-                    q[j] = vec[int64_t(s) * vld];
-                    off[j] = cb + m.base[s] - w.win_lo;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < kMaxSegSlots; j++) {
-                if (j0 + j < gr.nslots) {
+                if (j < ns) {
 #pragma unroll
                     for (int i = 0; i < RPT; i++) {
                         const uint32_t uc = off[j] + V::get(q[j], i);
                         if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
                     }
+                }
+            }
+        } else {
+            for (int j = 0; j < ns; j++) {
+                const int sj = slot_list[gr.first + j];
+                const VecT q = vec[int64_t(sj) * vld];
+                const uint32_t o = cb + m.base[sj] - w.win_lo;
+#pragma unroll
+                for (int i = 0; i < RPT; i++) {
+                    const uint32_t uc = o + V::get(q, i);
+                    if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
                 }
             }
         }

@@ -96,8 +96,9 @@ struct GridPlan {
 struct TuneParams {
     int rpt4 = 4;               // rows per thread of the FP32 tile gather (0: row kernel)
     int rpt8 = 2;               // rows per thread of the FP64 tile gather (0: row kernel)
-    double blocks_per_cu = 2.0; // scatter tiles per CU the row chunks aim at
-    double partial_cap = 0.1;   // partial-slice traffic cap as a fraction of index traffic
+    int srpt = 4;               // consecutive rows per step of the stored-index scatter (4, or 8 for u16)
+    double blocks_per_cu = 1.0; // scatter tiles per CU the row chunks aim at
+    double partial_cap = 1.0;   // partial-slice traffic cap as a fraction of index traffic
 };
 
 // Slot columns of a stored index matrix start at multiples of this many rows,

@@ -76,30 +76,36 @@ PT* partials(ScatterLists& L)
 
 // A^T y in two launches: LDS window tiles into partial slices, then the
 // column reduction in chunk order (which writes every column of z).
-// The first-stage scatter kernel of a view: vector loads of 4 consecutive
-// rows for stored indices, one row per step otherwise.
-template <class VT, class YT, class View>
-constexpr auto scatter_kernel()
+// The first-stage scatter kernel of a view: vector loads of rpt (4, or 8 for
+// u16 indices) consecutive rows for stored indices, one row per step
+// otherwise; f receives the kernel.
+template <class VT, class YT, class View, class F>
+void with_scatter_kernel(int rpt, F&& f)
 {
-    if constexpr (std::is_same_v<View, SlotMajor<uint16_t>> || std::is_same_v<View, SlotMajor<uint32_t>>)
-        return k_scatter_tile<VT, YT, typename View::index_type, 4>;
-    else
-        return k_scatter_part<VT, YT, View>;
+    if constexpr (std::is_same_v<View, SlotMajor<uint16_t>>) {
+        if (rpt == 8) return f(k_scatter_tile<VT, YT, uint16_t, 8>);
+        return f(k_scatter_tile<VT, YT, uint16_t, 4>);
+    } else if constexpr (std::is_same_v<View, SlotMajor<uint32_t>>) {
+        return f(k_scatter_tile<VT, YT, uint32_t, 4>);
+    } else {
+        return f(k_scatter_part<VT, YT, View>);
+    }
 }
 
 template <class VT, class YT, class ZT, class View>
-void launch_scatter(ScatterPlan& sp, const View& view, const uint32_t* sys_col, const YT* y, const LsqrScalars* div,
-                    ZT* z, hipStream_t s)
+void launch_scatter(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
+                    const LsqrScalars* div, ZT* z, hipStream_t s)
 {
     ScatterLists& L = lists<VT>(sp);
     VT* part = partials<VT>(L);
     if (!L.work.empty()) {
         const size_t lds = size_t(L.widest) * sizeof(VT);
-        auto kern = scatter_kernel<VT, YT, View>();
-        allow_lds(kern, lds);
-        hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
-                           L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(), sp.d_slot_list.as<uint16_t>(),
-                           sys_col, y, div, part);
+        with_scatter_kernel<VT, YT, View>(rpt, [&](auto kern) {
+            allow_lds(kern, lds);
+            hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
+                               L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
+                               sp.d_slot_list.as<uint16_t>(), sys_col, y, div, part);
+        });
         CAD_HIP(hipGetLastError());
     }
     if (!L.reduce.empty()) {
@@ -115,14 +121,16 @@ void prepare_scatter(cad_matrix* m)
     ScatterLists& L = lists<VT>(m->splan);
     partials<VT>(L);
     const size_t lds = size_t(L.widest) * sizeof(VT);
-    with_view(m, [&](auto view) { allow_lds(scatter_kernel<VT, YT, decltype(view)>(), lds); });
+    with_view(m, [&](auto view) {
+        with_scatter_kernel<VT, YT, decltype(view)>(m->tune.srpt, [&](auto kern) { allow_lds(kern, lds); });
+    });
 }
 
 template <class VT, class YT, class ZT>
 void scatter(cad_matrix* m, const YT* y, const LsqrScalars* div, ZT* z, hipStream_t s)
 {
     with_view(m, [&](auto view) {
-        launch_scatter<VT, YT, ZT>(m->splan, view, m->d_sys_col.as<uint32_t>(), y, div, z, s);
+        launch_scatter<VT, YT, ZT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), y, div, z, s);
     });
 }
 
@@ -935,20 +943,25 @@ std::string autotune(cad_matrix* m)
     {
         const TuneParams keep = tp;
         double bb = tp.blocks_per_cu, bc = tp.partial_cap, best_us = 1e300;
+        int bs = tp.srpt;
+        const std::vector<int> srpts = (!m->mf && m->ibytes == 2) ? std::vector<int>{4, 8} : std::vector<int>{4};
+        for (int sr : srpts)
         for (double bpc : {1.0, 2.0, 4.0})
-            for (double cap : {0.05, 0.1, 0.25, 1.0}) {
+            for (double cap : {0.1, 0.25, 1.0}) {
+                tp.srpt = sr;
                 tp.blocks_per_cu = bpc;
                 tp.partial_cap = cap;
                 build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
                                    m->mf ? 2 : m->ibytes, *m->ctx, tp);
                 const double us =
                     t.us([&] { scatter<float, float, float>(m, y4.as<float>(), nullptr, z4.as<float>(), s); }, s);
-                std::snprintf(line, sizeof line, "scatter f32 blocks/CU %.0f cap %.2f (%zu tiles): %.1f us\n", bpc,
-                              cap, m->splan.f4.work.size(), us);
+                std::snprintf(line, sizeof line, "scatter f32 rpt %d blocks/CU %.0f cap %.2f (%zu tiles): %.1f us\n",
+                              sr, bpc, cap, m->splan.f4.work.size(), us);
                 log += line;
-                if (us < best_us) best_us = us, bb = bpc, bc = cap;
+                if (us < best_us) best_us = us, bb = bpc, bc = cap, bs = sr;
             }
         tp = keep;
+        tp.srpt = bs;
         tp.blocks_per_cu = bb;
         tp.partial_cap = bc;
         build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
@@ -956,8 +969,9 @@ std::string autotune(cad_matrix* m)
     }
     CAD_HIP(hipStreamSynchronize(s));
     m->tuned = true;
-    std::snprintf(line, sizeof line, "chosen: gather f32 rpt %d, f64 rpt %d, scatter blocks/CU %.0f cap %.2f\n",
-                  tp.rpt4, tp.rpt8, tp.blocks_per_cu, tp.partial_cap);
+    std::snprintf(line, sizeof line,
+                  "chosen: gather f32 rpt %d, f64 rpt %d, scatter rpt %d blocks/CU %.0f cap %.2f\n", tp.rpt4,
+                  tp.rpt8, tp.srpt, tp.blocks_per_cu, tp.partial_cap);
     log += line;
     return log;
 }
