@@ -135,12 +135,12 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
         return;
     }
     ScatterPlan& sp = const_cast<ScatterPlan&>(eval_plan(p, B));
-    ScatterLists& L = sizeof(VT) == 4 ? sp.f4 : sp.f8;
-    const size_t need = sizeof(VT) * size_t(std::max<int64_t>(1, L.part_elems));
+    ScatterLists& L = sp.f8;  // FP64 LDS windows (k_scatter_part accumulates in FP64)
+    const size_t need = sizeof(double) * size_t(std::max<int64_t>(1, L.part_elems));  // either precision
     if (L.part.bytes < need) L.part = DBuf(need);
     VT* part = L.part.as<VT>();
     if (!L.work.empty()) {
-        const size_t lds = size_t(L.widest) * sizeof(VT);
+        const size_t lds = size_t(L.widest) * sizeof(double);
         auto kern = k_scatter_part<VT, VT, RowMajor32>;
         if (lds > 64 * 1024)
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),

@@ -9,6 +9,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "board.hpp"
 #include "device.hpp"
 
@@ -590,8 +592,10 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_u(const RowWork* __rest
 
 // ------------------------------------------------------------ scatter ------
 // A^T (y * scale), stage 1: each block owns a column window of one slot group
-// in LDS, scatters a chunk of rows into it (ds_add) and stores the window to
-// its slice of the partial buffer -- no global atomics.
+// in LDS, scatters a chunk of rows into it and stores the window to its slice
+// of the partial buffer -- no global atomics.  The window accumulates in FP64
+// whatever VT is: LDS atomic adds on doubles run several times faster than on
+// floats on this GPU, and the sums are exact for integer data either way.
 // scale = 1 / div[sys].beta when div is given (0 when beta == 0), else 1.
 template <class VT, class YT, class View>
 __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const ScatterWork* __restrict__ work,
@@ -602,11 +606,11 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
                                                                 VT* __restrict__ part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    VT* win = reinterpret_cast<VT*>(smem_raw);
+    double* win = reinterpret_cast<double*>(smem_raw);
     const ScatterWork w = work[blockIdx.x];
     if (div && (!div[w.sys].active || div[w.sys].stop)) return;
     const uint32_t W = w.win_hi - w.win_lo;
-    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = VT(0);
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = 0.0;
     __syncthreads();
     const SlotGroup gr = groups[w.group];
     const uint32_t cb = sys_col[w.sys];
@@ -616,7 +620,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
         sc = VT(d != 0.0 ? 1.0 / d : 0.0);
     }
     for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kScatterBlock) {
-        const VT yv = VT(y[r]) * sc;
+        const double yv = double(VT(y[r]) * sc);
         for (int j = 0; j < gr.nslots; j++) {
             const int s = slot_list[gr.first + j];
             const uint32_t uc = cb + m.col(s, r) - w.win_lo;
@@ -625,7 +629,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
     }
     __syncthreads();
     VT* __restrict__ out = part + w.part_off;
-    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = win[i];
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = VT(win[i]);
 }
 
 // Stage 1 for stored indices: as k_scatter_part, with each thread taking RPT
@@ -646,11 +650,11 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    VT* win = reinterpret_cast<VT*>(smem_raw);
+    double* win = reinterpret_cast<double*>(smem_raw);  // FP64 window, as in k_scatter_part
     const ScatterWork w = work[blockIdx.x];
     if (div && (!div[w.sys].active || div[w.sys].stop)) return;
     const uint32_t W = w.win_hi - w.win_lo;
-    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = VT(0);
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = 0.0;
     __syncthreads();
     const SlotGroup gr = groups[w.group];
     const int ns = gr.nslots;
@@ -677,12 +681,12 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     for (int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x; a < w.row_end;
          a += int64_t(RPT) * kScatterBlock) {
         bool ok[RPT];
-        VT yv[RPT];
+        double yv[RPT];
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             ok[i] = a + i >= w.row_begin && a + i < w.row_end;
-            const VT t = VT(y[min(a + i, last)]) * sc;  // clamped: no branch around the load
-            yv[i] = ok[i] ? t : VT(0);
+            const double t = double(VT(y[min(a + i, last)]) * sc);  // clamped: no branch around the load
+            yv[i] = ok[i] ? t : 0.0;
         }
         const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + a);
         if (ns <= kMaxSegSlots) {
@@ -714,7 +718,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     }
     __syncthreads();
     VT* __restrict__ out = part + w.part_off;
-    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = win[i];
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = VT(win[i]);
 }
 
 // Stage 2: z[c] = sum of the window's chunk slices, in chunk order
@@ -729,8 +733,10 @@ __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* 
     if (div && (!div[w.sys].active || div[w.sys].stop)) return;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         const T* __restrict__ p = part + w.part_off + (c - w.col_begin);
-        T s = T(0);
-        for (int j = 0; j < w.nchunk; j++) s += p[int64_t(j) * w.stride];
+        // integer counts stay integer; real sums add up in FP64
+        using AccT = std::conditional_t<std::is_integral_v<T>, T, double>;
+        AccT s = AccT(0);
+        for (int j = 0; j < w.nchunk; j++) s += AccT(p[int64_t(j) * w.stride]);
         z[c] = ZT(s);
     }
 }

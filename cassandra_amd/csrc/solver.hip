@@ -58,18 +58,23 @@ void allow_lds(K kernel, size_t bytes)
     }
 }
 
+// The scatter windows: every real-valued scatter accumulates in FP64 LDS, so
+// it uses the 8-byte windows whatever its value type (f4 serves the u32
+// multiplicity counts of the Jacobi diagonal).
 template <class VT>
 ScatterLists& lists(ScatterPlan& sp)
 {
-    return sizeof(VT) == 4 ? sp.f4 : sp.f8;
+    return sp.f8;
 }
 
-// The partial buffer of a scatter list (allocated outside graph capture: every
-// solver calls prepare_scatter first).
+// The partial buffer of a scatter list, sized for 8-byte elements whatever PT
+// is: FP32 and FP64 scatters share the list, and a buffer grown for one
+// precision would free the one a captured graph of the other still uses.
+// (Allocated outside graph capture: every solver calls prepare_scatter first.)
 template <class PT>
 PT* partials(ScatterLists& L)
 {
-    const size_t bytes = sizeof(PT) * size_t(std::max<int64_t>(1, L.part_elems));
+    const size_t bytes = sizeof(double) * size_t(std::max<int64_t>(1, L.part_elems));
     if (L.part.bytes < bytes) L.part = DBuf(bytes);
     return L.part.as<PT>();
 }
@@ -99,7 +104,7 @@ void launch_scatter(ScatterPlan& sp, const View& view, int rpt, const uint32_t* 
     ScatterLists& L = lists<VT>(sp);
     VT* part = partials<VT>(L);
     if (!L.work.empty()) {
-        const size_t lds = size_t(L.widest) * sizeof(VT);
+        const size_t lds = size_t(L.widest) * sizeof(double);
         with_scatter_kernel<VT, YT, View>(rpt, [&](auto kern) {
             allow_lds(kern, lds);
             hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
@@ -120,7 +125,7 @@ void prepare_scatter(cad_matrix* m)
 {
     ScatterLists& L = lists<VT>(m->splan);
     partials<VT>(L);
-    const size_t lds = size_t(L.widest) * sizeof(VT);
+    const size_t lds = size_t(L.widest) * sizeof(double);
     with_view(m, [&](auto view) {
         with_scatter_kernel<VT, YT, decltype(view)>(m->tune.srpt, [&](auto kern) { allow_lds(kern, lds); });
     });
@@ -956,7 +961,7 @@ std::string autotune(cad_matrix* m)
                 const double us =
                     t.us([&] { scatter<float, float, float>(m, y4.as<float>(), nullptr, z4.as<float>(), s); }, s);
                 std::snprintf(line, sizeof line, "scatter f32 rpt %d blocks/CU %.0f cap %.2f (%zu tiles): %.1f us\n",
-                              sr, bpc, cap, m->splan.f4.work.size(), us);
+                              sr, bpc, cap, m->splan.f8.work.size(), us);
                 log += line;
                 if (us < best_us) best_us = us, bb = bpc, bc = cap, bs = sr;
             }
