@@ -86,7 +86,7 @@ struct LsqrScalars {
     int32_t active;           // 0: converged / frozen by the host
     int32_t iters;
     int32_t stop;             // 0 running; 1 target met this iteration (its step still applies); 2 stopped
-    int32_t pad;
+    int32_t restart;          // selects the systems that the start / apply kernels (re)initialise
     double pad2;
 };
 static_assert(sizeof(LsqrScalars) % 16 == 0, "LsqrScalars layout");

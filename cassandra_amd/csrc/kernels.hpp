@@ -218,7 +218,7 @@ struct EpiResidual {
     VT* __restrict__ u;
     LsqrScalars* scal;
     static constexpr bool kReduce = true;
-    __device__ bool begin(int sys) { return scal[sys].active != 0; }
+    __device__ bool begin(int sys) { return scal[sys].active != 0 && scal[sys].restart != 0; }
     __device__ double row(int64_t r, double a)
     {
         const double v = b[r] - a;
@@ -591,6 +591,28 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_u(const RowWork* __rest
 }
 
 // ------------------------------------------------------------ scatter ------
+// Which systems a scatter serves, and its scale:
+//   kScatterAll      every system, scale 1 (plain products)
+//   kScatterLsqr     the systems iterating (active, not stopped), y scaled by 1 / beta
+//   kScatterRestart  the systems being (re)started (active, restart), scale 1
+constexpr int kScatterAll = 0, kScatterLsqr = 1, kScatterRestart = 2;
+
+__device__ __forceinline__ bool scatter_selected(const LsqrScalars* div, int mode, int sys)
+{
+    if (mode == kScatterAll) return true;
+    const LsqrScalars& d = div[sys];
+    if (!d.active) return false;
+    return mode == kScatterLsqr ? d.stop == 0 : d.restart != 0;
+}
+
+template <class VT>
+__device__ __forceinline__ VT scatter_scale(const LsqrScalars* div, int mode, int sys)
+{
+    if (mode != kScatterLsqr) return VT(1);
+    const double d = div[sys].beta;
+    return VT(d != 0.0 ? 1.0 / d : 0.0);
+}
+
 // A^T (y * scale), stage 1: each block owns a column window of one slot group
 // in LDS, scatters a chunk of rows into it and stores the window to its slice
 // of the partial buffer -- no global atomics.  The window accumulates in FP64
@@ -603,22 +625,18 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
                                                                 const uint16_t* __restrict__ slot_list,
                                                                 const uint32_t* __restrict__ sys_col,
                                                                 const YT* __restrict__ y, const LsqrScalars* div,
-                                                                VT* __restrict__ part)
+                                                                int mode, VT* __restrict__ part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* win = reinterpret_cast<double*>(smem_raw);
     const ScatterWork w = work[blockIdx.x];
-    if (div && (!div[w.sys].active || div[w.sys].stop)) return;
+    if (!scatter_selected(div, mode, w.sys)) return;
     const uint32_t W = w.win_hi - w.win_lo;
     for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = 0.0;
     __syncthreads();
     const SlotGroup gr = groups[w.group];
     const uint32_t cb = sys_col[w.sys];
-    VT sc = VT(1);
-    if (div) {
-        const double d = div[w.sys].beta;
-        sc = VT(d != 0.0 ? 1.0 / d : 0.0);
-    }
+    const VT sc = scatter_scale<VT>(div, mode, w.sys);
     for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kScatterBlock) {
         const double yv = double(VT(y[r]) * sc);
         for (int j = 0; j < gr.nslots; j++) {
@@ -645,25 +663,21 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
                                                                 const uint16_t* __restrict__ slot_list,
                                                                 const uint32_t* __restrict__ sys_col,
                                                                 const YT* __restrict__ y, const LsqrScalars* div,
-                                                                VT* __restrict__ part)
+                                                                int mode, VT* __restrict__ part)
 {
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* win = reinterpret_cast<double*>(smem_raw);  // FP64 window, as in k_scatter_part
     const ScatterWork w = work[blockIdx.x];
-    if (div && (!div[w.sys].active || div[w.sys].stop)) return;
+    if (!scatter_selected(div, mode, w.sys)) return;
     const uint32_t W = w.win_hi - w.win_lo;
     for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = 0.0;
     __syncthreads();
     const SlotGroup gr = groups[w.group];
     const int ns = gr.nslots;
     const uint32_t cb = sys_col[w.sys];
-    VT sc = VT(1);
-    if (div) {
-        const double d = div[w.sys].beta;
-        sc = VT(d != 0.0 ? 1.0 / d : 0.0);
-    }
+    const VT sc = scatter_scale<VT>(div, mode, w.sys);
     const int64_t vld = m.ld / RPT;
     const int64_t last = w.row_end - 1;
     int64_t slot[kMaxSegSlots];
@@ -727,10 +741,10 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
 template <class T, class ZT>
 __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* __restrict__ work,
                                                               const T* __restrict__ part, ZT* __restrict__ z,
-                                                              const LsqrScalars* div)
+                                                              const LsqrScalars* div, int mode)
 {
     const ReduceWork w = work[blockIdx.x];
-    if (div && (!div[w.sys].active || div[w.sys].stop)) return;
+    if (!scatter_selected(div, mode, w.sys)) return;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         const T* __restrict__ p = part + w.part_off + (c - w.col_begin);
         // integer counts stay integer; real sums add up in FP64
@@ -879,7 +893,7 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_v(const ColWork* __rest
     __shared__ double sh[kRowBlock / 64];
     const ColWork w = work[blockIdx.x];
     LsqrScalars* st = &scal[w.sys];
-    if (!st->active) return;
+    if (!st->active || !st->restart) return;
     const double ib = st->beta != 0.0 ? 1.0 / st->beta : 0.0;
     double a0 = 0.0, a1 = 0.0;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
@@ -928,7 +942,7 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_w(const ColWork* __rest
 {
     const ColWork w = work[blockIdx.x];
     const LsqrScalars* st = &scal[w.sys];
-    if (!st->active) return;
+    if (!st->active || !st->restart) return;
     const double ia = st->alpha != 0.0 ? 1.0 / st->alpha : 0.0;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         const double vn = v[c] * ia;
@@ -945,7 +959,7 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_apply(const ColWork* __restr
                                                           double* __restrict__ x, const LsqrScalars* scal)
 {
     const ColWork w = work[blockIdx.x];
-    if (!scal[w.sys].active) return;
+    if (!scal[w.sys].active || !scal[w.sys].restart) return;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         x[c] += D[c] * y[c];
         y[c] = 0.0;

@@ -99,7 +99,7 @@ void with_scatter_kernel(int rpt, F&& f)
 
 template <class VT, class YT, class ZT, class View>
 void launch_scatter(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
-                    const LsqrScalars* div, ZT* z, hipStream_t s)
+                    const LsqrScalars* div, int mode, ZT* z, hipStream_t s)
 {
     ScatterLists& L = lists<VT>(sp);
     VT* part = partials<VT>(L);
@@ -109,13 +109,13 @@ void launch_scatter(ScatterPlan& sp, const View& view, int rpt, const uint32_t* 
             allow_lds(kern, lds);
             hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
                                L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
-                               sp.d_slot_list.as<uint16_t>(), sys_col, y, div, part);
+                               sp.d_slot_list.as<uint16_t>(), sys_col, y, div, mode, part);
         });
         CAD_HIP(hipGetLastError());
     }
     if (!L.reduce.empty()) {
         hipLaunchKernelGGL((k_scatter_reduce<VT, ZT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
-                           L.d_reduce.as<ReduceWork>(), (const VT*)part, z, div);
+                           L.d_reduce.as<ReduceWork>(), (const VT*)part, z, div, mode);
         CAD_HIP(hipGetLastError());
     }
 }
@@ -131,11 +131,24 @@ void prepare_scatter(cad_matrix* m)
     });
 }
 
+// A^T y for every system (div == nullptr), or, in an LSQR iteration, for the
+// iterating systems with y scaled by 1 / beta.
 template <class VT, class YT, class ZT>
 void scatter(cad_matrix* m, const YT* y, const LsqrScalars* div, ZT* z, hipStream_t s)
 {
+    const int mode = div ? kScatterLsqr : kScatterAll;
     with_view(m, [&](auto view) {
-        launch_scatter<VT, YT, ZT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), y, div, z, s);
+        launch_scatter<VT, YT, ZT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), y, div, mode, z, s);
+    });
+}
+
+// A^T y for the systems being (re)started (active and restart in sel).
+template <class VT, class YT, class ZT>
+void scatter_restart(cad_matrix* m, const YT* y, const LsqrScalars* sel, ZT* z, hipStream_t s)
+{
+    with_view(m, [&](auto view) {
+        launch_scatter<VT, YT, ZT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), y, sel,
+                                   kScatterRestart, z, s);
     });
 }
 
@@ -249,7 +262,7 @@ const uint32_t* jacobi_counts(cad_matrix* m)
     }
     hipLaunchKernelGGL((k_scatter_reduce<uint32_t, uint32_t>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0,
                        s, L.d_reduce.as<ReduceWork>(), (const uint32_t*)part, m->d_jacobi_cnt.as<uint32_t>(),
-                       (const LsqrScalars*)nullptr);
+                       (const LsqrScalars*)nullptr, kScatterAll);
     CAD_HIP(hipGetLastError());
     m->have_cnt = true;
     return m->d_jacobi_cnt.as<uint32_t>();
@@ -300,7 +313,7 @@ static void set_active(cad_matrix* m, Workspace& ws, DBuf& d, const std::vector<
 {
     std::vector<LsqrScalars> h(m->nsys);
     std::memset(h.data(), 0, sizeof(LsqrScalars) * m->nsys);
-    for (int i = 0; i < m->nsys; i++) h[i].active = active[i];
+    for (int i = 0; i < m->nsys; i++) h[i].active = h[i].restart = active[i];
     CAD_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(LsqrScalars) * m->nsys, hipMemcpyHostToDevice, s));
     CAD_HIP(hipStreamSynchronize(s));
     (void)ws;
@@ -434,7 +447,7 @@ static void lsqr_start_from_residual(cad_matrix* m, Workspace& ws, const double*
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
     gather<double>(m, ws.x64.as<double>(), EpiResidual<VT>{b, ws.r64.as<double>(), ws.u.as<VT>(), sc}, s);
-    scatter<double, double, double>(m, ws.r64.as<double>(), nullptr, ws.z64.as<double>(), s);
+    scatter_restart<double, double, double>(m, ws.r64.as<double>(), sc, ws.z64.as<double>(), s);
     hipLaunchKernelGGL(k_lsqr_init_v<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
                        g.d_cols.as<ColWork>(), ws.D.as<double>(), ws.z64.as<double>(), ws.v.as<double>(), g.col_sys(),
                        sc);
@@ -562,94 +575,147 @@ struct RefineState {
     int it = 0, refinements = 0;
 };
 
-// One precision stage of the refined LSQR: from the FP64 x in ws.x64,
-// repeatedly r = b - A x and A^T r in FP64, an inner LSQR with VT products on
-// min |A d - r| until its estimated normal residual fell by the target, and
-// x += d.  A system leaves the stage when it meets tol (act = 0) or when a
-// refinement cut its metric by less than 4x (stalled = 1): products in VT
-// cannot resolve that system's residual any further.
+// One precision stage of the refined LSQR, from the FP64 x in ws.x64.  Every
+// system runs its own refinement loop -- r = b - A x and A^T r in FP64, an
+// inner LSQR with VT products on min |A d - r| that stops on the device once
+// its estimate fell by the system's target, x += d -- and the systems of a
+// batch advance independently: at each host check the systems whose inner
+// solve ended are restarted (apply, FP64 residual, new start, selected by the
+// restart flag) while the others keep iterating in the same launches.  A
+// system leaves the stage when it meets tol (act = 0), when a refinement cut
+// its metric by less than 4x (stalled = 1: products in VT cannot resolve its
+// residual further) or after max_refinements.
 template <class VT>
 static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const cad_solve_options& o, double floor_tol,
                          int max_refinements, RefineState& st, SystemDetail& det, cad_solve_report& rep)
 {
     hipStream_t s = m->ctx->stream;
+    const int n = m->nsys;
     const int batch = std::max(1, o.check_every);
-    std::vector<double> target(m->nsys, o.inner_tol), prev(m->nsys, 0.0), best(m->nsys), stall(m->nsys);
-    std::vector<int> act = st.act;
-    for (int ref = 0;; ref++) {
-        set_active(m, ws, ws.scal, act, s);
-        lsqr_start_from_residual<VT>(m, ws, b, s);
-        const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
-        bool any = false;
-        for (int i = 0; i < m->nsys; i++) {
-            if (!act[i]) continue;
-            st.rel[i] = st.atb[i] > 0 ? std::sqrt(h[i].reduce_b) / st.atb[i] : std::sqrt(h[i].reduce_b);
-            st.rn2[i] = h[i].reduce_a;
-            if (o.tol > 0 && st.rel[i] <= o.tol) {
-                act[i] = st.act[i] = 0;
-                det.iters[i] = st.it;
-            } else if (ref > 0 && st.rel[i] > 0.25 * prev[i]) {
-                act[i] = 0;
-                st.stalled[i] = 1;
-            } else {
-                any = true;
-                // the last refinement only needs to close the remaining gap
-                // (with a factor 3 of margin); products in VT cannot go much
-                // below floor_tol in one solve
-                target[i] = o.tol > 0 ? std::max(floor_tol, o.tol / (3.0 * st.rel[i])) : floor_tol;
-                prev[i] = st.rel[i];
+    std::vector<double> target(n, 0.0), prev(n, 0.0), best(n, 1.0);
+    std::vector<int> refs(n, 0), stall(n, 0), checks(n, 0), running(n, 0);
+    std::vector<int> restart = st.act;
+    set_active(m, ws, ws.scal, st.act, s);  // all scalars zero, restart = active
+    const LsqrScalars* h = nullptr;
+    bool first = true;
+    auto any_of = [&](const std::vector<int>& v) {
+        for (int i = 0; i < n; i++)
+            if (v[i]) return true;
+        return false;
+    };
+    // writes the flags: active = running or restarting, restart as given
+    auto write_flags = [&](const std::vector<int>& rs) {
+        std::vector<LsqrScalars> hv(n);
+        if (h) std::memcpy(hv.data(), h, sizeof(LsqrScalars) * n);
+        else std::memset(hv.data(), 0, sizeof(LsqrScalars) * n);
+        for (int i = 0; i < n; i++) {
+            hv[i].active = (running[i] || rs[i]) ? 1 : 0;
+            hv[i].restart = rs[i];
+        }
+        CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * n, hipMemcpyHostToDevice, s));
+    };
+    for (;;) {
+        if (any_of(restart)) {
+            if (!first) {
+                write_flags(restart);
+                hipLaunchKernelGGL(k_lsqr_apply, dim3(unsigned(m->grid.cols.size())), dim3(kRowBlock), 0, s,
+                                   m->grid.d_cols.as<ColWork>(), ws.D.as<double>(), ws.y.as<double>(),
+                                   ws.x64.as<double>(), (const LsqrScalars*)ws.scal.as<LsqrScalars>());
+                CAD_HIP(hipGetLastError());
             }
-        }
-        rep.explicit_checks++;
-        if (o.verbose && m->nsys == 1)
-            std::fprintf(stderr, "[refine %s] refinement %d it %d rel %.3e\n", sizeof(VT) == 4 ? "fp32" : "fp64",
-                         ref, st.it, st.rel[0]);
-        if (!any || st.it >= o.max_iterations || ref >= max_refinements) break;
-        st.refinements++;
-        // freeze the systems that left; the others stop on the device at their target
-        {
-            std::vector<LsqrScalars> hv(h, h + m->nsys);
-            for (int i = 0; i < m->nsys; i++) {
-                hv[i].active = act[i];
-                hv[i].target = act[i] ? target[i] : 0.0;
-                hv[i].stop = 0;
-            }
-            CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * m->nsys, hipMemcpyHostToDevice, s));
-        }
-        for (int i = 0; i < m->nsys; i++) {
-            best[i] = 1.0;
-            stall[i] = 0;
-        }
-        // inner LSQR until every active system's estimate fell by its target.
-        // The estimate of |A^T r| is not monotone in LSQR and can plateau, so
-        // a system only counts as stuck when its best estimate did not improve
-        // by 1% for as many checks as the inner solve has run (at least 16).
-        for (int checks = 1;; checks++) {
-            const int n = std::min(batch, o.max_iterations - st.it);
-            if (n <= 0) break;
-            lsqr_run<VT>(m, ws, n, batch, o.use_graph != 0 && n == batch, s);
-            st.it += n;
+            lsqr_start_from_residual<VT>(m, ws, b, s);
             h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
-            bool inner_done = true;
-            for (int i = 0; i < m->nsys; i++) {
-                if (!h[i].active) continue;
-                const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
-                if (!std::isfinite(est)) throw Error(CAD_ERR_BREAKDOWN, "LSQR breakdown (non-finite recurrence)");
-                if (est < 0.99 * best[i]) {
-                    best[i] = est;
-                    stall[i] = 0;
+            rep.explicit_checks++;
+            for (int i = 0; i < n; i++) {
+                if (!restart[i]) continue;
+                st.rel[i] = st.atb[i] > 0 ? std::sqrt(h[i].reduce_b) / st.atb[i] : std::sqrt(h[i].reduce_b);
+                st.rn2[i] = h[i].reduce_a;
+                if (o.tol > 0 && st.rel[i] <= o.tol) {
+                    st.act[i] = 0;
+                    det.iters[i] = st.it;
+                } else if (refs[i] > 0 && st.rel[i] > 0.25 * prev[i]) {
+                    st.stalled[i] = 1;
+                } else if (refs[i] >= max_refinements) {
+                    // left to the caller: not converged
                 } else {
-                    stall[i] += 1;
+                    // the last refinement only needs to close the remaining gap
+                    // (with a factor 3 of margin); products in VT cannot go much
+                    // below floor_tol in one solve
+                    target[i] = o.tol > 0 ? std::max(floor_tol, o.tol / (3.0 * st.rel[i])) : floor_tol;
+                    prev[i] = st.rel[i];
+                    refs[i]++;
+                    st.refinements = std::max(st.refinements, refs[i]);
+                    running[i] = 1;
+                    best[i] = 1.0;
+                    stall[i] = checks[i] = 0;
                 }
-                const bool stuck = stall[i] >= std::max(16, checks / 2 + 1) && stall[i] * 2 >= checks;
-                if (est > target[i] && !stuck) inner_done = false;
             }
-            if (inner_done) break;
+            if (o.verbose && n == 1)
+                std::fprintf(stderr, "[refine %s] it %d rel %.3e\n", sizeof(VT) == 4 ? "fp32" : "fp64", st.it,
+                             st.rel[0]);
+            // the started systems iterate; targets on the device
+            std::vector<LsqrScalars> hv(h, h + n);
+            for (int i = 0; i < n; i++) {
+                hv[i].active = running[i];
+                hv[i].restart = 0;
+                if (restart[i]) {
+                    hv[i].target = running[i] ? target[i] : 0.0;
+                    hv[i].stop = 0;
+                }
+            }
+            CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * n, hipMemcpyHostToDevice, s));
+            first = false;
+            std::fill(restart.begin(), restart.end(), 0);
         }
+        if (!any_of(running) || st.it >= o.max_iterations) break;
+        const int k = std::min(batch, o.max_iterations - st.it);
+        lsqr_run<VT>(m, ws, k, batch, o.use_graph != 0 && k == batch, s);
+        st.it += k;
+        h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+        // systems whose inner solve ended: met the target on the device, or
+        // stuck (the estimate of |A^T r| is not monotone in LSQR and can
+        // plateau, so only a best estimate that did not improve by 1% for as
+        // many checks as the solve has run, at least 16, counts as stuck)
+        for (int i = 0; i < n; i++) {
+            if (!running[i]) continue;
+            const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
+            if (!std::isfinite(est)) throw Error(CAD_ERR_BREAKDOWN, "LSQR breakdown (non-finite recurrence)");
+            checks[i]++;
+            if (est < 0.99 * best[i]) {
+                best[i] = est;
+                stall[i] = 0;
+            } else {
+                stall[i]++;
+            }
+            const bool stuck = stall[i] >= std::max(16, checks[i] / 2 + 1) && stall[i] * 2 >= checks[i];
+            if (h[i].stop != 0 || est <= target[i] || stuck) {
+                running[i] = 0;
+                restart[i] = 1;
+            }
+        }
+    }
+    // out of iterations with inner solves in flight: keep their corrections
+    // and measure where they got
+    if (any_of(running)) {
+        restart = running;
+        std::fill(running.begin(), running.end(), 0);
+        write_flags(restart);
         hipLaunchKernelGGL(k_lsqr_apply, dim3(unsigned(m->grid.cols.size())), dim3(kRowBlock), 0, s,
                            m->grid.d_cols.as<ColWork>(), ws.D.as<double>(), ws.y.as<double>(), ws.x64.as<double>(),
                            (const LsqrScalars*)ws.scal.as<LsqrScalars>());
         CAD_HIP(hipGetLastError());
+        lsqr_start_from_residual<VT>(m, ws, b, s);
+        h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+        rep.explicit_checks++;
+        for (int i = 0; i < n; i++) {
+            if (!restart[i]) continue;
+            st.rel[i] = st.atb[i] > 0 ? std::sqrt(h[i].reduce_b) / st.atb[i] : std::sqrt(h[i].reduce_b);
+            st.rn2[i] = h[i].reduce_a;
+            if (o.tol > 0 && st.rel[i] <= o.tol) {
+                st.act[i] = 0;
+                det.iters[i] = st.it;
+            }
+        }
     }
 }
 
