@@ -745,12 +745,21 @@ __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* 
 {
     const ReduceWork w = work[blockIdx.x];
     if (!scatter_selected(div, mode, w.sys)) return;
+    // integer counts stay integer; real sums add up in FP64, in chunk order,
+    // eight slices loaded ahead of their additions
+    using AccT = std::conditional_t<std::is_integral_v<T>, T, double>;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         const T* __restrict__ p = part + w.part_off + (c - w.col_begin);
-        // integer counts stay integer; real sums add up in FP64
-        using AccT = std::conditional_t<std::is_integral_v<T>, T, double>;
         AccT s = AccT(0);
-        for (int j = 0; j < w.nchunk; j++) s += AccT(p[int64_t(j) * w.stride]);
+        int j = 0;
+        for (; j + 8 <= w.nchunk; j += 8) {
+            T t[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) t[q] = p[int64_t(j + q) * w.stride];
+#pragma unroll
+            for (int q = 0; q < 8; q++) s += AccT(t[q]);
+        }
+        for (; j < w.nchunk; j++) s += AccT(p[int64_t(j) * w.stride]);
         z[c] = ZT(s);
     }
 }

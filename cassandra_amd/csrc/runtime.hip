@@ -71,6 +71,8 @@ GridSys GridPlan::col_sys() const
 
 namespace {
 
+constexpr uint32_t kReduceCols = kRowBlock;  // columns of one scatter-reduce work item
+
 // Group slots whose column ranges overlap (transitively).
 void make_groups(const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi, ScatterPlan& sp)
 {
@@ -154,10 +156,12 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
                         L.work.push_back(w);
                     }
                 }
-                for (uint32_t c0 = wl; c0 < wh; c0 += 1024) {
+                // one column per thread of a reduce block (kReduceCols), so the
+                // reduce launches enough blocks to cover the CUs
+                for (uint32_t c0 = wl; c0 < wh; c0 += kReduceCols) {
                     ReduceWork r;
                     r.col_begin = int64_t(sys_col[s]) + c0;
-                    r.col_end = int64_t(sys_col[s]) + std::min(wh, c0 + 1024);
+                    r.col_end = int64_t(sys_col[s]) + std::min(wh, c0 + kReduceCols);
                     r.part_off = off + (c0 - wl);
                     r.nchunk = int32_t(C);
                     r.stride = W;
@@ -173,10 +177,10 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
         covered.push_back({uint32_t(cols_per_sys), uint32_t(cols_per_sys)});
         uint32_t at = 0;
         for (auto& iv : covered) {
-            for (uint32_t c0 = at; c0 < iv.first; c0 += 1024) {
+            for (uint32_t c0 = at; c0 < iv.first; c0 += kReduceCols) {
                 ReduceWork r;
                 r.col_begin = int64_t(sys_col[s]) + c0;
-                r.col_end = int64_t(sys_col[s]) + std::min(iv.first, c0 + 1024);
+                r.col_end = int64_t(sys_col[s]) + std::min(iv.first, c0 + kReduceCols);
                 r.part_off = 0;
                 r.nchunk = 0;
                 r.stride = 0;
@@ -313,7 +317,7 @@ void build_plans(cad_matrix* m)
     g.col_first.assign(m->nsys, 0);
     g.col_count.assign(m->nsys, 0);
     const int64_t rchunk = int64_t(kRowBlock) * kRowsPerThread;
-    const int64_t cchunk = int64_t(kRowBlock) * 4;
+    const int64_t cchunk = int64_t(kRowBlock) * 2;  // enough column blocks to cover the CUs
     for (int s = 0; s < m->nsys; s++) {
         g.row_first[s] = int32_t(g.rows.size());
         for (int64_t r = m->sys_row_begin[s]; r < m->sys_row_begin[s + 1]; r += rchunk)
