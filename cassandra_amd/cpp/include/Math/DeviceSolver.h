@@ -14,8 +14,8 @@ struct DeviceSolveOptions
     bool precondition = true;     // LSQR: column scaling; PCG: Jacobi
     int max_iterations = 100000;
     double tol = 1e-10;           // |A^T(b - A x)| / |A^T b|; 0 runs max_iterations
-    double inner_tol = 1e-4;
-    int check_every = 25;
+    double inner_tol = 1e-5;
+    int check_every = 10;
 };
 
 struct DeviceSolveReport

@@ -639,9 +639,9 @@ void cad_solve_options_default(cad_solve_options* o)
     o->precondition = 1;
     o->max_iterations = 100000;
     o->tol = 1e-10;
-    o->inner_tol = 1e-4;
+    o->inner_tol = 1e-5;
     o->max_refinements = 20;
-    o->check_every = 25;
+    o->check_every = 10;
     o->use_graph = 1;
     o->verbose = 0;
 }

@@ -65,6 +65,26 @@ struct GatherSeg {
     uint16_t nwin, pad;
 };
 
+// One step of the stored-index tile gather: up to kMaxSegSlots consecutive
+// slots of one segment.  A step with phase_count > 0 first stages the phase's
+// segments (phases[phase_first ..]) into LDS side by side; the segment's
+// columns start at lds_off there.  A step with nwin > 1 is a segment wider than
+// the LDS, staged window by window by the step itself.
+struct GatherStep {
+    uint16_t slot_first, nslots;
+    int16_t phase_first, phase_count;
+    uint32_t col_lo;   // the segment's first column (system-local)
+    uint32_t lds_off;  // the segment's offset in the staged phase
+    uint32_t width;    // the segment's width (windows of a wide segment)
+    uint16_t nwin, pad;
+};
+
+// A segment of a staging phase: columns [col_lo, col_lo + width) go to
+// LDS [lds_off, lds_off + width).
+struct PhaseSeg {
+    uint32_t col_lo, width, lds_off, pad;
+};
+
 // Slots whose column ranges overlap: a column index can repeat inside a row
 // only among the slots of one group.
 struct SlotGroup {

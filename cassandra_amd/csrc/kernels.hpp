@@ -360,25 +360,39 @@ __device__ __forceinline__ void stage(VT* __restrict__ xs, const VT* __restrict_
 }
 
 // LDS-staged gather for pattern matrices.  A block owns a tile of rows and
-// walks the slot segments (runs of slots over one column block) in slot
-// order; each segment's block of x is staged in LDS and the random reads hit
-// LDS instead of pulling an L2 line per element.  Thread t owns the RPT
-// consecutive rows starting at the tile's first row rounded down to RPT plus
-// RPT * t, so one aligned vector load brings a slot's entries of all its rows;
-// the loads of up to kMaxSegSlots slots are issued before the segment is
-// staged and overlap it.  Segments wider than the LDS (wmax columns) are
-// staged window by window: each slot's value is kept in registers until every
-// window was seen, then the segment's slots are added in slot order, so every
-// row is the sequential sum in slot order (Math/MatrixCSR.cpp:59-62).
+// walks the plan's steps (runs of at most kMaxSegSlots slots of one segment,
+// a segment being a run of slots over one column block) in slot order.  The
+// columns of consecutive segments that fit the LDS together are staged in one
+// phase, and the random reads of x then hit LDS instead of pulling an L2 line
+// per element.  Thread t owns the RPT consecutive rows starting at the tile's
+// first row rounded down to RPT plus RPT * t, so one aligned vector load brings
+// a slot's entries of all its rows; the next step's loads are issued before
+// the current step is staged and summed, so they overlap it.  Segments wider
+// than the LDS (wmax columns) are staged window by window: each slot's value is
+// kept in registers until every window was seen, then the segment's slots are
+// added in slot order, so every row is the sequential sum in slot order
+// (Math/MatrixCSR.cpp:59-62).
 constexpr int kTileBlock = 1024;
 constexpr int kMaxSegSlots = 8;
+
+template <class IdxT, int RPT>
+__device__ __forceinline__ void load_step(typename IdxVec<IdxT, RPT>::T (&q)[kMaxSegSlots],
+                                          const typename IdxVec<IdxT, RPT>::T* __restrict__ vec, int64_t vld,
+                                          const GatherStep& st, bool live)
+{
+#pragma unroll
+    for (int j = 0; j < kMaxSegSlots; j++) {
+        q[j] = typename IdxVec<IdxT, RPT>::T{};
+        if (live && j < st.nslots) q[j] = vec[int64_t(st.slot_first + j) * vld];
+    }
+}
 
 template <class VT, int RPT, class IdxT, class Epi>
 __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, const RowWork* __restrict__ work,
                                                             const uint32_t* __restrict__ sys_col,
-                                                            const GatherSeg* __restrict__ segs, int nseg,
-                                                            uint32_t wmax, const VT* __restrict__ x, Epi epi,
-                                                            GridSys g)
+                                                            const GatherStep* __restrict__ steps, int nsteps,
+                                                            const PhaseSeg* __restrict__ phases, uint32_t wmax,
+                                                            const VT* __restrict__ x, Epi epi, GridSys g)
 {
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
@@ -399,55 +413,33 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
     // slot s of this thread's rows: vec[s * vld]
     const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + (live ? a : 0));
     const int64_t vld = m.ld / RPT;
-    for (int sg = 0; sg < nseg; sg++) {
-        const GatherSeg seg = segs[sg];
-        const VT* __restrict__ xg = x + cb + seg.col_lo;
-        const int s_end = seg.slot_first + seg.nslots;
-        if (seg.nwin == 1) {
-            for (int s0 = seg.slot_first; s0 < s_end; s0 += kMaxSegSlots) {
-                VecT q[kMaxSegSlots];
-#pragma unroll
-                for (int j = 0; j < kMaxSegSlots; j++) {
-                    q[j] = VecT{};
-                    if (live && s0 + j < s_end) q[j] = vec[int64_t(s0 + j) * vld];
-                }
-                if (s0 == seg.slot_first) {
-                    __syncthreads();
-                    stage<VT, kTileBlock>(xs, xg, seg.width);
-                    __syncthreads();
-                }
-#pragma unroll
-                for (int j = 0; j < kMaxSegSlots; j++) {
-                    if (s0 + j < s_end) {
-                        // 32-bit offset: base - col_lo wraps for u32 storage
-                        // (base 0) and the sum with the entry unwraps it
-                        const uint32_t off = m.base[s0 + j] - seg.col_lo;
-#pragma unroll
-                        for (int i = 0; i < RPT; i++)
-                            if (ok[i]) acc[i] += xs[off + V::get(q[j], i)];
-                    }
-                }
-            }
-        } else {
-            // at most kMaxSegSlots slots (checked when the plan is built)
-            VecT q[kMaxSegSlots];
+    // the next step's loads are held beside the current ones only where the
+    // registers allow it (up to 8 bytes of entries per slot)
+    constexpr bool kPrefetch = sizeof(VecT) <= 8;
+    VecT q[kMaxSegSlots], qn[kPrefetch ? kMaxSegSlots : 1];
+    if (nsteps > 0) load_step<IdxT, RPT>(q, vec, vld, steps[0], live);
+    for (int k = 0; k < nsteps; k++) {
+        const GatherStep st = steps[k];
+        if (!kPrefetch && k > 0) load_step<IdxT, RPT>(q, vec, vld, st, live);
+        if (st.nwin > 1) {
+            // a segment wider than the LDS, at most kMaxSegSlots slots
             VT v[RPT][kMaxSegSlots];
 #pragma unroll
-            for (int j = 0; j < kMaxSegSlots; j++) {
-                q[j] = VecT{};
-                if (live && j < seg.nslots) q[j] = vec[int64_t(seg.slot_first + j) * vld];
+            for (int j = 0; j < kMaxSegSlots; j++)
 #pragma unroll
                 for (int i = 0; i < RPT; i++) v[i][j] = VT(0);
-            }
-            for (uint32_t wl = 0; wl < seg.width; wl += wmax) {
-                const uint32_t W = min(wmax, seg.width - wl);
+            const VT* __restrict__ xg = x + cb + st.col_lo;
+            for (uint32_t wl = 0; wl < st.width; wl += wmax) {
+                const uint32_t W = min(wmax, st.width - wl);
                 __syncthreads();
                 stage<VT, kTileBlock>(xs, xg + wl, W);
                 __syncthreads();
 #pragma unroll
                 for (int j = 0; j < kMaxSegSlots; j++) {
-                    if (j < seg.nslots) {
-                        const uint32_t off = m.base[seg.slot_first + j] - seg.col_lo - wl;
+                    if (j < st.nslots) {
+                        // 32-bit offset: base - col_lo wraps for u32 storage
+                        // (base 0) and the sum with the entry unwraps it
+                        const uint32_t off = m.base[st.slot_first + j] - st.col_lo - wl;
 #pragma unroll
                         for (int i = 0; i < RPT; i++) {
                             const uint32_t c = off + V::get(q[j], i);
@@ -458,9 +450,36 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
             }
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++)
-                if (j < seg.nslots) {
+                if (j < st.nslots) {
 #pragma unroll
                     for (int i = 0; i < RPT; i++) acc[i] += v[i][j];
+                }
+            if (kPrefetch && k + 1 < nsteps) load_step<IdxT, RPT>(q, vec, vld, steps[k + 1], live);
+        } else {
+            const bool more = kPrefetch && k + 1 < nsteps;
+            if constexpr (kPrefetch)
+                if (more) load_step<IdxT, RPT>(qn, vec, vld, steps[k + 1], live);
+            if (st.phase_count > 0) {
+                __syncthreads();
+                for (int p = st.phase_first; p < st.phase_first + st.phase_count; p++) {
+                    const PhaseSeg ps = phases[p];
+                    stage<VT, kTileBlock>(xs + ps.lds_off, x + cb + ps.col_lo, ps.width);
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++) {
+                if (j < st.nslots) {
+                    const uint32_t off = m.base[st.slot_first + j] - st.col_lo + st.lds_off;
+#pragma unroll
+                    for (int i = 0; i < RPT; i++)
+                        if (ok[i]) acc[i] += xs[off + V::get(q[j], i)];
+                }
+            }
+            if constexpr (kPrefetch)
+                if (more) {
+#pragma unroll
+                    for (int j = 0; j < kMaxSegSlots; j++) q[j] = qn[j];
                 }
         }
     }
@@ -727,6 +746,58 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
                     const uint32_t uc = o + V::get(q, i);
                     if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
                 }
+            }
+        }
+    }
+    __syncthreads();
+    VT* __restrict__ out = part + w.part_off;
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = VT(win[i]);
+}
+
+// Stage 1 for a matrix-free matrix: as k_scatter_tile, with each thread's RPT
+// positions loaded once per step and every slot's column computed by its
+// program (the DevSlot is the same for the whole block).
+template <class VT, class YT, int RPT>
+__global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView m,
+                                                                   const ScatterWork* __restrict__ work,
+                                                                   const SlotGroup* __restrict__ groups,
+                                                                   const uint16_t* __restrict__ slot_list,
+                                                                   const uint32_t* __restrict__ sys_col,
+                                                                   const YT* __restrict__ y, const LsqrScalars* div,
+                                                                   int mode, VT* __restrict__ part)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* win = reinterpret_cast<double*>(smem_raw);
+    const ScatterWork w = work[blockIdx.x];
+    if (!scatter_selected(div, mode, w.sys)) return;
+    const uint32_t W = w.win_hi - w.win_lo;
+    for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = 0.0;
+    __syncthreads();
+    const SlotGroup gr = groups[w.group];
+    const uint32_t cb = sys_col[w.sys];
+    const VT sc = scatter_scale<VT>(div, mode, w.sys);
+    const int64_t last = w.row_end - 1;
+    for (int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x; a < w.row_end;
+         a += int64_t(RPT) * kScatterBlock) {
+        bool ok[RPT];
+        double yv[RPT];
+        uint64_t P[RPT], O[RPT];
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            ok[i] = a + i >= w.row_begin && a + i < w.row_end;
+            const int64_t r = min(a + i, last);  // clamped: no branch around the loads
+            const double t = double(VT(y[r]) * sc);
+            yv[i] = ok[i] ? t : 0.0;
+            P[i] = m.pos[2 * r];
+            O[i] = m.pos[2 * r + 1];
+        }
+        for (int j = 0; j < gr.nslots; j++) {
+            const DevSlot sl = m.slots[slot_list[gr.first + j]];
+            const uint32_t o = cb + sl.col_offset - w.win_lo;
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const uint32_t uc = o + slot_index(sl, m.sq, m.dense, P[i], O[i]);
+                if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
             }
         }
     }

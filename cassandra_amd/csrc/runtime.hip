@@ -263,6 +263,47 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
         next = first + g->nslots;
     }
     if (next != m->k || gp.segs.empty()) return;
+    // Steps of the stored-index kernel: consecutive single-window segments are
+    // packed into one staging phase while they fit the LDS together; a wide
+    // segment is a step of its own.
+    gp.steps.clear();
+    gp.phases.clear();
+    uint32_t phase_w = 0;
+    int open_step = -1;  // the step that stages the current phase
+    for (const GatherSeg& sg : gp.segs) {
+        const bool wide = sg.nwin > 1;
+        if (wide || open_step < 0 || phase_w + sg.width > gp.wmax) {
+            open_step = -1;
+            phase_w = 0;
+        }
+        for (int j0 = 0; j0 < sg.nslots; j0 += kMaxSegSlots) {
+            GatherStep st{};
+            st.slot_first = uint16_t(sg.slot_first + j0);
+            st.nslots = uint16_t(std::min<int>(kMaxSegSlots, sg.nslots - j0));
+            st.col_lo = sg.col_lo;
+            st.width = sg.width;
+            st.nwin = sg.nwin;
+            st.phase_first = -1;
+            st.phase_count = 0;
+            if (!wide) {
+                if (j0 == 0) {
+                    if (open_step < 0) {
+                        st.phase_first = int16_t(gp.phases.size());
+                        open_step = int(gp.steps.size());
+                    }
+                    gp.phases.push_back(PhaseSeg{sg.col_lo, sg.width, phase_w, 0u});
+                }
+                st.lds_off = phase_w;
+            }
+            gp.steps.push_back(st);
+        }
+        if (!wide) {
+            GatherStep& opener = gp.steps[size_t(open_step)];
+            opener.phase_count = int16_t(int(gp.phases.size()) - opener.phase_first);
+            phase_w += sg.width;
+            gp.lds_elems = std::max(gp.lds_elems, phase_w);
+        }
+    }
     // Row tiles: a thread owns rpt consecutive rows counted from the tile's
     // first row rounded down to rpt, so a tile spans at most kTileBlock * rpt
     // rows from there; tiles of a system are a multiple of rpt rows long (their
@@ -284,6 +325,8 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
         gp.count[s] = int32_t(gp.tiles.size()) - gp.first[s];
     }
     gp.d_segs = to_device(gp.segs, ctx.stream);
+    gp.d_steps = to_device(gp.steps, ctx.stream);
+    gp.d_phases = to_device(gp.phases, ctx.stream);
     gp.d_tiles = to_device(gp.tiles, ctx.stream);
     gp.d_first = to_device(gp.first, ctx.stream);
     gp.d_count = to_device(gp.count, ctx.stream);

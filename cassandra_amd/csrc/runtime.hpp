@@ -72,10 +72,12 @@ struct GatherPlan {
     int rpt = 0;      // rows per thread
     uint32_t wmax = 0;
     uint32_t lds_elems = 0;
-    std::vector<GatherSeg> segs;
+    std::vector<GatherSeg> segs;     // matrix-free kernel: one staging per segment
+    std::vector<GatherStep> steps;   // stored-index kernel: packed phases, prefetched steps
+    std::vector<PhaseSeg> phases;
     std::vector<RowWork> tiles;
     std::vector<int32_t> first, count;
-    DBuf d_segs, d_tiles, d_first, d_count, d_counters, d_partials;
+    DBuf d_segs, d_steps, d_phases, d_tiles, d_first, d_count, d_counters, d_partials;
     GridSys sys() const;
 };
 

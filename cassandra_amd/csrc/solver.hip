@@ -82,8 +82,8 @@ PT* partials(ScatterLists& L)
 // A^T y in two launches: LDS window tiles into partial slices, then the
 // column reduction in chunk order (which writes every column of z).
 // The first-stage scatter kernel of a view: vector loads of rpt (4, or 8 for
-// u16 indices) consecutive rows for stored indices, one row per step
-// otherwise; f receives the kernel.
+// u16 indices) consecutive rows for stored indices, 4 positions per step for
+// a matrix-free matrix, one row per step otherwise; f receives the kernel.
 template <class VT, class YT, class View, class F>
 void with_scatter_kernel(int rpt, F&& f)
 {
@@ -92,6 +92,8 @@ void with_scatter_kernel(int rpt, F&& f)
         return f(k_scatter_tile<VT, YT, uint16_t, 4>);
     } else if constexpr (std::is_same_v<View, SlotMajor<uint32_t>>) {
         return f(k_scatter_tile<VT, YT, uint32_t, 4>);
+    } else if constexpr (std::is_same_v<View, PositionsView>) {
+        return f(k_scatter_tile_mf<VT, YT, 4>);
     } else {
         return f(k_scatter_part<VT, YT, View>);
     }
@@ -202,7 +204,8 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
                     allow_lds(kern, lds);
                     hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
                                        gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
-                                       gp.d_segs.as<GatherSeg>(), int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
+                                       gp.d_steps.as<GatherStep>(), int(gp.steps.size()),
+                                       gp.d_phases.as<PhaseSeg>(), gp.wmax, x, epi, gp.sys());
                 }
             });
         } else {
