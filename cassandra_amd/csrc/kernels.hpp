@@ -456,17 +456,20 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                 }
             if (kPrefetch && k + 1 < nsteps) load_step<IdxT, RPT>(q, vec, vld, steps[k + 1], live);
         } else {
+            // the next step's loads go out after the staging loads (the wait
+            // for those would otherwise wait for them too) and stay in
+            // flight through the barrier and this step's sums
             const bool more = kPrefetch && k + 1 < nsteps;
-            if constexpr (kPrefetch)
-                if (more) load_step<IdxT, RPT>(qn, vec, vld, steps[k + 1], live);
             if (st.phase_count > 0) {
                 __syncthreads();
                 for (int p = st.phase_first; p < st.phase_first + st.phase_count; p++) {
                     const PhaseSeg ps = phases[p];
                     stage<VT, kTileBlock>(xs + ps.lds_off, x + cb + ps.col_lo, ps.width);
                 }
-                __syncthreads();
             }
+            if constexpr (kPrefetch)
+                if (more) load_step<IdxT, RPT>(qn, vec, vld, steps[k + 1], live);
+            if (st.phase_count > 0) __syncthreads();
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++) {
                 if (j < st.nslots) {
