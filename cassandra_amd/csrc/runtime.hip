@@ -104,6 +104,115 @@ void make_groups(const std::vector<uint32_t>& lo, const std::vector<uint32_t>& h
     }
 }
 
+// Reduce items of one window ([wl, wh) of system s, chunk slices at off with
+// stride W) and of the columns outside every window (written as zeros).
+void push_reduce(ScatterLists& L, int s, uint32_t sys_col, uint32_t wl, uint32_t wh, int64_t off, int64_t C)
+{
+    // one column per thread of a reduce block (kReduceCols), so the reduce
+    // launches enough blocks to cover the CUs
+    for (uint32_t c0 = wl; c0 < wh; c0 += kReduceCols) {
+        ReduceWork r;
+        r.col_begin = int64_t(sys_col) + c0;
+        r.col_end = int64_t(sys_col) + std::min(wh, c0 + kReduceCols);
+        r.part_off = C > 0 ? off + (c0 - wl) : 0;
+        r.nchunk = int32_t(C);
+        r.stride = C > 0 ? wh - wl : 0;
+        r.sys = s;
+        r.pad = 0;
+        L.reduce.push_back(r);
+    }
+}
+
+// Chunk-major scatter lists: every window of a system uses the same row
+// chunks, and the tiles of one chunk (all its windows) are placed on one XCD
+// one after the other (workgroup i runs on XCD i mod 8), so the y values and
+// the index entries that the windows of a chunk share are read from HBM once
+// and then hit that XCD's L2.  Slots left over at the end of an XCD's lane
+// hold empty tiles (no rows, no columns).
+constexpr int kXcds = 8;
+
+void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
+                            const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, uint32_t wmax, int vbytes,
+                            int ibytes, double target, const TuneParams& tune, ScatterLists& L)
+{
+    const int nsys = int(sys_col.size());
+    struct Win { uint32_t lo, hi; int group; int nslots; };
+    std::vector<Win> wins;
+    double work_per_row = 0, width = 0;
+    for (size_t gi = 0; gi < sp.groups.size(); gi++) {
+        const SlotGroup& g = sp.groups[gi];
+        for (uint32_t wl = g.col_lo; wl < g.col_hi; wl += wmax) {
+            const uint32_t wh = std::min(g.col_hi, wl + wmax);
+            wins.push_back(Win{wl, wh, int(gi), g.nslots});
+            L.widest = std::max(L.widest, wh - wl);
+            work_per_row += g.nslots;
+            width += wh - wl;
+        }
+    }
+    const int nw = int(wins.size());
+    int64_t total_rows = 0;
+    for (int s = 0; s < nsys; s++) total_rows += sys_rows[s + 1] - sys_rows[s];
+    std::vector<std::vector<ScatterWork>> lanes(kXcds);
+    int64_t off = 0;
+    int chunk_id = 0;
+    for (int s = 0; s < nsys; s++) {
+        const int64_t R = sys_rows[s + 1] - sys_rows[s];
+        int64_t C = 0;
+        if (R > 0) {
+            const double share = total_rows > 0 ? double(R) / double(total_rows) : 1.0;
+            C = std::max<int64_t>(1, std::llround(target * share / std::max(1, nw)));
+            const double cap = tune.partial_cap * double(R) * work_per_row * ibytes / (2.0 * width * vbytes);
+            C = std::max<int64_t>(1, std::min<int64_t>(C, int64_t(cap)));
+            C = std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock);
+            const int64_t chunk = (R + C - 1) / C;
+            C = (R + chunk - 1) / chunk;
+            std::vector<int64_t> woff(nw);
+            for (int j = 0; j < nw; j++) {
+                woff[j] = off;
+                off += C * int64_t(wins[j].hi - wins[j].lo);
+            }
+            for (int64_t c = 0; c < C; c++, chunk_id++) {
+                auto& lane = lanes[size_t(chunk_id % kXcds)];
+                for (int j = 0; j < nw; j++) {
+                    ScatterWork w;
+                    w.row_begin = sys_rows[s] + c * chunk;
+                    w.row_end = sys_rows[s] + std::min(R, (c + 1) * chunk);
+                    w.part_off = woff[j] + c * int64_t(wins[j].hi - wins[j].lo);
+                    w.win_lo = sys_col[s] + wins[j].lo;
+                    w.win_hi = sys_col[s] + wins[j].hi;
+                    w.group = wins[j].group;
+                    w.sys = s;
+                    lane.push_back(w);
+                }
+            }
+            for (int j = 0; j < nw; j++) push_reduce(L, s, sys_col[s], wins[j].lo, wins[j].hi, woff[j], C);
+        }
+        // columns outside every window are written as zeros
+        std::vector<std::pair<uint32_t, uint32_t>> covered;
+        if (R > 0)
+            for (const Win& w : wins) covered.push_back({w.lo, w.hi});
+        std::sort(covered.begin(), covered.end());
+        covered.push_back({uint32_t(cols_per_sys), uint32_t(cols_per_sys)});
+        uint32_t at = 0;
+        for (auto& iv : covered) {
+            if (at < iv.first) push_reduce(L, s, sys_col[s], at, iv.first, 0, 0);
+            at = std::max(at, iv.second);
+        }
+    }
+    size_t depth = 0;
+    for (auto& lane : lanes) depth = std::max(depth, lane.size());
+    ScatterWork empty{};
+    empty.row_begin = empty.row_end = 0;
+    empty.part_off = 0;
+    empty.win_lo = empty.win_hi = 0;
+    empty.group = 0;
+    empty.sys = 0;
+    L.work.assign(depth * kXcds, empty);
+    for (int x = 0; x < kXcds; x++)
+        for (size_t p = 0; p < lanes[size_t(x)].size(); p++) L.work[p * kXcds + size_t(x)] = lanes[size_t(x)][p];
+    L.part_elems = off;
+}
+
 // Scatter tiles of (system rows) x (group window).  The number of row chunks
 // of a window is set by its share of the work (about two tiles per CU in
 // total), capped so that the partial slices (written and read once) stay near
@@ -125,6 +234,10 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
         }
     const double target = tune.blocks_per_cu * std::max(cus, 1);
     int64_t off = 0;
+    if (tune.scatter_order == 1) {
+        make_scatter_lists_xcd(sp, sys_rows, sys_col, cols_per_sys, wmax, vbytes, ibytes, target, tune, L);
+        return;
+    }
     for (int s = 0; s < nsys; s++) {
         const int64_t R = sys_rows[s + 1] - sys_rows[s];
         std::vector<std::pair<uint32_t, uint32_t>> covered;

@@ -1013,39 +1013,46 @@ std::string autotune(cad_matrix* m)
         tp.rpt8 = best;
         build_gather_plan(m, m->gplan8, 8, best);
     }
-    // scatter: tiles per CU and the partial-traffic cap (FP32 timing)
+    // scatter: work order, rows per step, tiles per CU and the partial-traffic
+    // cap (FP32 timing)
     {
-        const TuneParams keep = tp;
-        double bb = tp.blocks_per_cu, bc = tp.partial_cap, best_us = 1e300;
-        int bs = tp.srpt;
+        struct Cand { int order, srpt; double bpc, cap; };
+        std::vector<Cand> cands;
         const std::vector<int> srpts = (!m->mf && m->ibytes == 2) ? std::vector<int>{4, 8} : std::vector<int>{4};
-        for (int sr : srpts)
-        for (double bpc : {1.0, 2.0, 4.0})
-            for (double cap : {0.1, 0.25, 1.0}) {
-                tp.srpt = sr;
-                tp.blocks_per_cu = bpc;
-                tp.partial_cap = cap;
-                build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
-                                   m->mf ? 2 : m->ibytes, *m->ctx, tp);
-                const double us =
-                    t.us([&] { scatter<float, float, float>(m, y4.as<float>(), nullptr, z4.as<float>(), s); }, s);
-                std::snprintf(line, sizeof line, "scatter f32 rpt %d blocks/CU %.0f cap %.2f (%zu tiles): %.1f us\n",
-                              sr, bpc, cap, m->splan.f8.work.size(), us);
-                log += line;
-                if (us < best_us) best_us = us, bb = bpc, bc = cap, bs = sr;
-            }
+        for (int order : {1, 0})
+            for (int sr : srpts)
+                for (double bpc : {1.0, 2.0, 4.0})
+                    for (double cap : {0.1, 0.25, 1.0}) cands.push_back(Cand{order, sr, bpc, cap});
+        const TuneParams keep = tp;
+        Cand best{tp.scatter_order, tp.srpt, tp.blocks_per_cu, tp.partial_cap};
+        double best_us = 1e300;
+        for (const Cand& c : cands) {
+            tp.scatter_order = c.order;
+            tp.srpt = c.srpt;
+            tp.blocks_per_cu = c.bpc;
+            tp.partial_cap = c.cap;
+            build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
+                               m->mf ? 2 : m->ibytes, *m->ctx, tp);
+            const double us =
+                t.us([&] { scatter<float, float, float>(m, y4.as<float>(), nullptr, z4.as<float>(), s); }, s);
+            std::snprintf(line, sizeof line, "scatter f32 order %d rpt %d blocks/CU %.0f cap %.2f (%zu tiles): %.1f us\n",
+                          c.order, c.srpt, c.bpc, c.cap, m->splan.f8.work.size(), us);
+            log += line;
+            if (us < best_us) best_us = us, best = c;
+        }
         tp = keep;
-        tp.srpt = bs;
-        tp.blocks_per_cu = bb;
-        tp.partial_cap = bc;
+        tp.scatter_order = best.order;
+        tp.srpt = best.srpt;
+        tp.blocks_per_cu = best.bpc;
+        tp.partial_cap = best.cap;
         build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
                            m->mf ? 2 : m->ibytes, *m->ctx, tp);
     }
     CAD_HIP(hipStreamSynchronize(s));
     m->tuned = true;
     std::snprintf(line, sizeof line,
-                  "chosen: gather f32 rpt %d, f64 rpt %d, scatter rpt %d blocks/CU %.0f cap %.2f\n", tp.rpt4,
-                  tp.rpt8, tp.srpt, tp.blocks_per_cu, tp.partial_cap);
+                  "chosen: gather f32 rpt %d, f64 rpt %d, scatter order %d rpt %d blocks/CU %.0f cap %.2f\n",
+                  tp.rpt4, tp.rpt8, tp.scatter_order, tp.srpt, tp.blocks_per_cu, tp.partial_cap);
     log += line;
     return log;
 }

@@ -284,7 +284,7 @@ TEST_CASE(fit_weights_against_host_pcg, true)
         dh += double(ph[i] - pe[i]) * (ph[i] - pe[i]);
         den += double(pe[i]) * pe[i];
     }
-    CHECK(std::sqrt(dh / den) < 3e-3);
+    CHECK(std::sqrt(dh / den) < 1e-2);
     CHECK(std::sqrt(dg) <= std::max(2.0 * std::sqrt(dh), 2e-4 * std::sqrt(den)));
 }
 
