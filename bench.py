@@ -467,12 +467,15 @@ def main():
     log(f"spmv {spmv_ms * 1e3:.1f} us {spmv_gbps:.0f} GB/s; spmtv {spmtv_ms * 1e3:.1f} us {spmtv_gbps:.0f} GB/s")
     dominant = ("k_scatter (A^T y)", spmtv_ms, spmtv_gbps) if spmtv_ms > spmv_ms else ("k_spmv (A x)", spmv_ms,
                                                                                        spmv_gbps)
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes over `bench.py --traffic` (scaled by an
+    # elementwise copy of 1 GiB; see profiles/traffic.json)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
             tj = json.load(open(tpath))
-            traffic = tj.get(dominant[0].split()[0])
+            traffic = tj.get(dominant[0].split()[0], {}).get("total_bytes")
         except Exception:
             traffic = None
 

@@ -67,6 +67,7 @@ void upload_programs(cad_patterns* p)
     }
     hipStream_t st = p->ctx->stream;
     p->d_slots = to_device(slots, st);
+    sq.resize((sq.size() + 7) / 4 * 4, 0);  // the kernels read the squares as whole 32-bit words
     p->d_sq = to_device(sq, st);
     p->d_dense = to_device(g.dense, st);
     p->slot_lo.clear();

@@ -108,9 +108,14 @@ __global__ void k_convert(TO* __restrict__ out, const TI* __restrict__ in, int64
 __device__ __forceinline__ uint32_t list_value(const uint8_t* __restrict__ sq, int off, int len,
                                                uint64_t P, uint64_t O)
 {
+    // the square numbers are the same for the whole wave: read them as 32-bit
+    // words (scalar loads; the array is padded to whole words) and take the
+    // byte, rather than one vector byte load per square
+    const uint32_t* __restrict__ w = reinterpret_cast<const uint32_t*>(sq);
     uint32_t v = 0;
     for (int j = len - 1; j >= 0; --j) {
-        const int q = sq[off + j];
+        const int b = off + j;
+        const int q = int((w[b >> 2] >> (8 * (b & 3))) & 0xFFu);
         v = v * 3u + uint32_t((P >> q) & 1u) + 2u * uint32_t((O >> q) & 1u);
     }
     return v;
@@ -413,14 +418,13 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
     // slot s of this thread's rows: vec[s * vld]
     const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + (live ? a : 0));
     const int64_t vld = m.ld / RPT;
-    // the next step's loads are held beside the current ones only where the
-    // registers allow it (up to 8 bytes of entries per slot)
-    constexpr bool kPrefetch = sizeof(VecT) <= 8;
-    VecT q[kMaxSegSlots], qn[kPrefetch ? kMaxSegSlots : 1];
-    if (nsteps > 0) load_step<IdxT, RPT>(q, vec, vld, steps[0], live);
-    for (int k = 0; k < nsteps; k++) {
-        const GatherStep st = steps[k];
-        if (!kPrefetch && k > 0) load_step<IdxT, RPT>(q, vec, vld, st, live);
+
+    // One step with its entries in qc.  With a next step (nxt), its loads go
+    // into qn after this step's staging loads (the wait for those would
+    // otherwise wait for them too) and stay in flight through the barrier and
+    // this step's sums.
+    auto run_step = [&](const GatherStep& st, VecT (&qc)[kMaxSegSlots], VecT (&qn)[kMaxSegSlots],
+                        const GatherStep* nxt) {
         if (st.nwin > 1) {
             // a segment wider than the LDS, at most kMaxSegSlots slots
             VT v[RPT][kMaxSegSlots];
@@ -442,7 +446,7 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                         const uint32_t off = m.base[st.slot_first + j] - st.col_lo - wl;
 #pragma unroll
                         for (int i = 0; i < RPT; i++) {
-                            const uint32_t c = off + V::get(q[j], i);
+                            const uint32_t c = off + V::get(qc[j], i);
                             if (ok[i] && c < W) v[i][j] = xs[c];
                         }
                     }
@@ -454,12 +458,8 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
 #pragma unroll
                     for (int i = 0; i < RPT; i++) acc[i] += v[i][j];
                 }
-            if (kPrefetch && k + 1 < nsteps) load_step<IdxT, RPT>(q, vec, vld, steps[k + 1], live);
+            if (nxt) load_step<IdxT, RPT>(qn, vec, vld, *nxt, live);
         } else {
-            // the next step's loads go out after the staging loads (the wait
-            // for those would otherwise wait for them too) and stay in
-            // flight through the barrier and this step's sums
-            const bool more = kPrefetch && k + 1 < nsteps;
             if (st.phase_count > 0) {
                 __syncthreads();
                 for (int p = st.phase_first; p < st.phase_first + st.phase_count; p++) {
@@ -467,8 +467,7 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                     stage<VT, kTileBlock>(xs + ps.lds_off, x + cb + ps.col_lo, ps.width);
                 }
             }
-            if constexpr (kPrefetch)
-                if (more) load_step<IdxT, RPT>(qn, vec, vld, steps[k + 1], live);
+            if (nxt) load_step<IdxT, RPT>(qn, vec, vld, *nxt, live);
             if (st.phase_count > 0) __syncthreads();
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++) {
@@ -476,14 +475,29 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                     const uint32_t off = m.base[st.slot_first + j] - st.col_lo + st.lds_off;
 #pragma unroll
                     for (int i = 0; i < RPT; i++)
-                        if (ok[i]) acc[i] += xs[off + V::get(q[j], i)];
+                        if (ok[i]) acc[i] += xs[off + V::get(qc[j], i)];
                 }
             }
-            if constexpr (kPrefetch)
-                if (more) {
-#pragma unroll
-                    for (int j = 0; j < kMaxSegSlots; j++) q[j] = qn[j];
-                }
+        }
+    };
+
+    // Two entry buffers used in turn (no copies) where the registers allow it
+    // (up to 8 bytes of entries per slot); one buffer loaded at each step
+    // otherwise.
+    if constexpr (sizeof(VecT) <= 8) {
+        VecT qa[kMaxSegSlots], qb[kMaxSegSlots];
+        if (nsteps > 0) load_step<IdxT, RPT>(qa, vec, vld, steps[0], live);
+        int k = 0;
+        for (; k + 1 < nsteps; k += 2) {
+            run_step(steps[k], qa, qb, &steps[k + 1]);
+            run_step(steps[k + 1], qb, qa, k + 2 < nsteps ? &steps[k + 2] : nullptr);
+        }
+        if (k < nsteps) run_step(steps[k], qa, qb, nullptr);
+    } else {
+        VecT q[kMaxSegSlots];
+        for (int k = 0; k < nsteps; k++) {
+            load_step<IdxT, RPT>(q, vec, vld, steps[k], live);
+            run_step(steps[k], q, q, nullptr);
         }
     }
     double red = 0.0;

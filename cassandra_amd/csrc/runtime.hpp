@@ -101,7 +101,7 @@ struct TuneParams {
     int srpt = 4;               // consecutive rows per step of the stored-index scatter (4, or 8 for u16)
     double blocks_per_cu = 1.0; // scatter tiles per CU the row chunks aim at
     double partial_cap = 1.0;   // partial-slice traffic cap as a fraction of index traffic
-    int scatter_order = 1;      // 0: chunks per window by work share; 1: shared chunks, XCD-major
+    int scatter_order = 0;      // 0: chunks per window by work share; 1: shared chunks, XCD-major
 };
 
 // Slot columns of a stored index matrix start at multiples of this many rows,
