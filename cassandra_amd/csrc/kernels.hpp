@@ -349,19 +349,35 @@ struct IdxVec<uint32_t, 4> {
     }
 };
 
-// Copies x[0, W) into LDS, four independent loads per thread in flight.
+// Copies the W elements at src into LDS at dst + h, h = the element offset of
+// src inside its 16-byte line (dst is 16-byte aligned), and returns h: the
+// 16-byte lines wholly inside [src, src + W) go from memory straight into LDS
+// (global_load_lds_dwordx4: a wave's 64 lanes fill 1 KB of consecutive LDS,
+// no registers held, every load of the thread in flight at once), the ragged
+// head and tail through registers.  Reads nothing outside [src, src + W).
+// The caller waits for the loads (vmcnt) and synchronises the block before
+// the LDS is read.
 template <class VT, int BS>
-__device__ __forceinline__ void stage(VT* __restrict__ xs, const VT* __restrict__ xg, uint32_t W)
+__device__ __forceinline__ uint32_t stage_direct(VT* __restrict__ dst, const VT* __restrict__ src, uint32_t W)
 {
-    uint32_t i = threadIdx.x;
-    for (; i + 3 * BS < W; i += 4 * BS) {
-        const VT a0 = xg[i], a1 = xg[i + BS], a2 = xg[i + 2 * BS], a3 = xg[i + 3 * BS];
-        xs[i] = a0;
-        xs[i + BS] = a1;
-        xs[i + 2 * BS] = a2;
-        xs[i + 3 * BS] = a3;
+    constexpr uint32_t Q = 16 / sizeof(VT);
+    const uint32_t h = uint32_t(reinterpret_cast<uintptr_t>(src) / sizeof(VT)) & (Q - 1);
+    const uint32_t j0 = min(W, (Q - h) & (Q - 1));  // first element of a whole line
+    const uint32_t nq = (W - j0) / Q;               // whole lines
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const VT* g = src + j0;
+    VT* l = dst + h + j0;  // 16-byte aligned
+    for (uint32_t t0 = wave * 64; t0 < nq; t0 += BS) {
+        if (t0 + lane < nq)
+            __builtin_amdgcn_global_load_lds(static_cast<const void*>(g + (t0 + lane) * Q),
+                                             (__attribute__((address_space(3))) void*)(l + t0 * Q), 16, 0, 0);
     }
-    for (; i < W; i += BS) xs[i] = xg[i];
+    if (threadIdx.x < Q) {
+        const uint32_t a = threadIdx.x, b = j0 + nq * Q + threadIdx.x;
+        if (a < j0) dst[h + a] = src[a];
+        if (b < W) dst[h + b] = src[b];
+    }
+    return h;
 }
 
 // LDS-staged gather for pattern matrices.  A block owns a tile of rows and
@@ -371,11 +387,11 @@ __device__ __forceinline__ void stage(VT* __restrict__ xs, const VT* __restrict_
 // phase, and the random reads of x then hit LDS instead of pulling an L2 line
 // per element.  Thread t owns the RPT consecutive rows starting at the tile's
 // first row rounded down to RPT plus RPT * t, so one aligned vector load brings
-// a slot's entries of all its rows; the next step's loads are issued before
-// the current step is staged and summed, so they overlap it.  Segments wider
-// than the LDS (wmax columns) are staged window by window: each slot's value is
-// kept in registers until every window was seen, then the segment's slots are
-// added in slot order, so every row is the sequential sum in slot order
+// a slot's entries of all its rows; a step's entry loads go out before its
+// staging, so both latencies overlap.  Segments wider than the LDS (windows of
+// wmax columns) are staged window by window: each slot's value is kept in
+// registers until every window was seen, then the segment's slots are added
+// in slot order, so every row is the sequential sum in slot order
 // (Math/MatrixCSR.cpp:59-62).
 constexpr int kTileBlock = 1024;
 constexpr int kMaxSegSlots = 8;
@@ -392,6 +408,13 @@ __device__ __forceinline__ void load_step(typename IdxVec<IdxT, RPT>::T (&q)[kMa
     }
 }
 
+// element offset of p inside its 16-byte line
+template <class VT>
+__device__ __forceinline__ uint32_t line_head(const VT* p)
+{
+    return uint32_t(reinterpret_cast<uintptr_t>(p) / sizeof(VT)) & (16 / sizeof(VT) - 1);
+}
+
 template <class VT, int RPT, class IdxT, class Epi>
 __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, const RowWork* __restrict__ work,
                                                             const uint32_t* __restrict__ sys_col,
@@ -405,7 +428,7 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
     VT* xs = reinterpret_cast<VT*>(smem_raw);
     const RowWork w = work[blockIdx.x];
     if (!epi.begin(w.sys)) return;
-    const uint32_t cb = sys_col[w.sys];
+    const VT* __restrict__ xb = x + sys_col[w.sys];
     const int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x;
     const bool live = a < w.row_end;
     bool ok[RPT];
@@ -418,13 +441,10 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
     // slot s of this thread's rows: vec[s * vld]
     const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + (live ? a : 0));
     const int64_t vld = m.ld / RPT;
-
-    // One step with its entries in qc.  With a next step (nxt), its loads go
-    // into qn after this step's staging loads (the wait for those would
-    // otherwise wait for them too) and stay in flight through the barrier and
-    // this step's sums.
-    auto run_step = [&](const GatherStep& st, VecT (&qc)[kMaxSegSlots], VecT (&qn)[kMaxSegSlots],
-                        const GatherStep* nxt) {
+    VecT q[kMaxSegSlots];
+    for (int k = 0; k < nsteps; k++) {
+        const GatherStep st = steps[k];
+        load_step<IdxT, RPT>(q, vec, vld, st, live);
         if (st.nwin > 1) {
             // a segment wider than the LDS, at most kMaxSegSlots slots
             VT v[RPT][kMaxSegSlots];
@@ -432,11 +452,11 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
             for (int j = 0; j < kMaxSegSlots; j++)
 #pragma unroll
                 for (int i = 0; i < RPT; i++) v[i][j] = VT(0);
-            const VT* __restrict__ xg = x + cb + st.col_lo;
             for (uint32_t wl = 0; wl < st.width; wl += wmax) {
                 const uint32_t W = min(wmax, st.width - wl);
                 __syncthreads();
-                stage<VT, kTileBlock>(xs, xg + wl, W);
+                const uint32_t h = stage_direct<VT, kTileBlock>(xs, xb + st.col_lo + wl, W);
+                __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
 #pragma unroll
                 for (int j = 0; j < kMaxSegSlots; j++) {
@@ -446,8 +466,8 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                         const uint32_t off = m.base[st.slot_first + j] - st.col_lo - wl;
 #pragma unroll
                         for (int i = 0; i < RPT; i++) {
-                            const uint32_t c = off + V::get(qc[j], i);
-                            if (ok[i] && c < W) v[i][j] = xs[c];
+                            const uint32_t c = off + V::get(q[j], i);
+                            if (ok[i] && c < W) v[i][j] = xs[h + c];
                         }
                     }
                 }
@@ -458,46 +478,26 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
 #pragma unroll
                     for (int i = 0; i < RPT; i++) acc[i] += v[i][j];
                 }
-            if (nxt) load_step<IdxT, RPT>(qn, vec, vld, *nxt, live);
         } else {
             if (st.phase_count > 0) {
                 __syncthreads();
                 for (int p = st.phase_first; p < st.phase_first + st.phase_count; p++) {
                     const PhaseSeg ps = phases[p];
-                    stage<VT, kTileBlock>(xs + ps.lds_off, x + cb + ps.col_lo, ps.width);
+                    stage_direct<VT, kTileBlock>(xs + ps.lds_off, xb + ps.col_lo, ps.width);
                 }
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
             }
-            if (nxt) load_step<IdxT, RPT>(qn, vec, vld, *nxt, live);
-            if (st.phase_count > 0) __syncthreads();
+            const uint32_t h = line_head(xb + st.col_lo);
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++) {
                 if (j < st.nslots) {
-                    const uint32_t off = m.base[st.slot_first + j] - st.col_lo + st.lds_off;
+                    const uint32_t off = m.base[st.slot_first + j] - st.col_lo + st.lds_off + h;
 #pragma unroll
                     for (int i = 0; i < RPT; i++)
-                        if (ok[i]) acc[i] += xs[off + V::get(qc[j], i)];
+                        if (ok[i]) acc[i] += xs[off + V::get(q[j], i)];
                 }
             }
-        }
-    };
-
-    // Two entry buffers used in turn (no copies) where the registers allow it
-    // (up to 8 bytes of entries per slot); one buffer loaded at each step
-    // otherwise.
-    if constexpr (sizeof(VecT) <= 8) {
-        VecT qa[kMaxSegSlots], qb[kMaxSegSlots];
-        if (nsteps > 0) load_step<IdxT, RPT>(qa, vec, vld, steps[0], live);
-        int k = 0;
-        for (; k + 1 < nsteps; k += 2) {
-            run_step(steps[k], qa, qb, &steps[k + 1]);
-            run_step(steps[k + 1], qb, qa, k + 2 < nsteps ? &steps[k + 2] : nullptr);
-        }
-        if (k < nsteps) run_step(steps[k], qa, qb, nullptr);
-    } else {
-        VecT q[kMaxSegSlots];
-        for (int k = 0; k < nsteps; k++) {
-            load_step<IdxT, RPT>(q, vec, vld, steps[k], live);
-            run_step(steps[k], q, q, nullptr);
         }
     }
     double red = 0.0;

@@ -349,8 +349,11 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
     gp.ok = false;
     gp.rpt = rpt;
     if (rpt <= 0) return;  // the row kernel
-    const size_t lds = lds_budget(ctx);
-    gp.wmax = uint32_t(lds / vbytes);
+    // A staged run of columns starts at its element offset inside a 16-byte
+    // line (up to Q - 1 elements) so that whole lines map to whole LDS lines:
+    // windows leave that room, and phase segments start on line boundaries.
+    const uint32_t total = uint32_t(lds_budget(ctx) / vbytes), Q = uint32_t(16 / vbytes);
+    gp.wmax = total - (Q - 1);
     std::vector<const SlotGroup*> order;
     for (auto& g : sp.groups) order.push_back(&g);
     std::sort(order.begin(), order.end(),
@@ -371,7 +374,7 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
         seg.nwin = uint16_t((seg.width + gp.wmax - 1) / gp.wmax);
         seg.pad = 0;
         if (seg.nwin > 1 && seg.nslots > kMaxSegSlots) return;
-        gp.lds_elems = std::max(gp.lds_elems, std::min(seg.width, gp.wmax));
+        gp.lds_elems = std::max(gp.lds_elems, std::min(seg.width, gp.wmax) + Q - 1);
         gp.segs.push_back(seg);
         next = first + g->nslots;
     }
@@ -385,7 +388,7 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
     int open_step = -1;  // the step that stages the current phase
     for (const GatherSeg& sg : gp.segs) {
         const bool wide = sg.nwin > 1;
-        if (wide || open_step < 0 || phase_w + sg.width > gp.wmax) {
+        if (wide || open_step < 0 || phase_w + sg.width + Q - 1 > total) {
             open_step = -1;
             phase_w = 0;
         }
@@ -413,8 +416,8 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
         if (!wide) {
             GatherStep& opener = gp.steps[size_t(open_step)];
             opener.phase_count = int16_t(int(gp.phases.size()) - opener.phase_first);
-            phase_w += sg.width;
-            gp.lds_elems = std::max(gp.lds_elems, phase_w);
+            gp.lds_elems = std::max(gp.lds_elems, phase_w + sg.width + Q - 1);
+            phase_w = (phase_w + sg.width + Q - 1 + Q - 1) / Q * Q;
         }
     }
     // Row tiles: a thread owns rpt consecutive rows counted from the tile's
