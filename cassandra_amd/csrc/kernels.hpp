@@ -441,10 +441,17 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
     // slot s of this thread's rows: vec[s * vld]
     const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + (live ? a : 0));
     const int64_t vld = m.ld / RPT;
-    VecT q[kMaxSegSlots];
+    // Where the registers allow it (up to 8 bytes of entries per slot) the
+    // next step's entries are loaded once this step's staging has landed, so
+    // they are in flight while this step's sums run; otherwise each step
+    // loads its own entries ahead of its staging.
+    constexpr bool kPrefetch = sizeof(VecT) <= 8;
+    VecT q[kMaxSegSlots], qn[kPrefetch ? kMaxSegSlots : 1];
+    if (kPrefetch && nsteps > 0) load_step<IdxT, RPT>(q, vec, vld, steps[0], live);
     for (int k = 0; k < nsteps; k++) {
         const GatherStep st = steps[k];
-        load_step<IdxT, RPT>(q, vec, vld, st, live);
+        const bool more = k + 1 < nsteps;
+        if constexpr (!kPrefetch) load_step<IdxT, RPT>(q, vec, vld, st, live);
         if (st.nwin > 1) {
             // a segment wider than the LDS, at most kMaxSegSlots slots
             VT v[RPT][kMaxSegSlots];
@@ -458,6 +465,8 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                 const uint32_t h = stage_direct<VT, kTileBlock>(xs, xb + st.col_lo + wl, W);
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
+                if constexpr (kPrefetch)
+                    if (more && wl + W >= st.width) load_step<IdxT, RPT>(qn, vec, vld, steps[k + 1], live);
 #pragma unroll
                 for (int j = 0; j < kMaxSegSlots; j++) {
                     if (j < st.nslots) {
@@ -488,6 +497,8 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
             }
+            if constexpr (kPrefetch)
+                if (more) load_step<IdxT, RPT>(qn, vec, vld, steps[k + 1], live);
             const uint32_t h = line_head(xb + st.col_lo);
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++) {
@@ -499,6 +510,11 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                 }
             }
         }
+        if constexpr (kPrefetch)
+            if (more) {
+#pragma unroll
+                for (int j = 0; j < kMaxSegSlots; j++) q[j] = qn[j];
+            }
     }
     double red = 0.0;
 #pragma unroll
@@ -686,12 +702,67 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
     for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) out[i] = VT(win[i]);
 }
 
+// The rows of one scatter work item for a group of at most NS slots (slot j
+// of the group at vec + slot[j], its window offset off[j]; slots ns..NS-1
+// repeat slot 0's loads and add nothing).  Every load is unconditional (rows
+// clamped into the chunk), and the next step's y and entries are loaded
+// before this step's atomics, so their latency overlaps the LDS work.
+template <int NS, class VT, class YT, class IdxT, int RPT>
+__device__ __forceinline__ void scatter_rows(const SlotMajor<IdxT>& m, const ScatterWork& w, int ns,
+                                             const int64_t (&slot)[kMaxSegSlots],
+                                             const uint32_t (&off)[kMaxSegSlots], const YT* __restrict__ y,
+                                             VT sc, uint32_t W, double* win)
+{
+    using V = IdxVec<IdxT, RPT>;
+    using VecT = typename V::T;
+    if (w.row_end <= w.row_begin) return;
+    const int64_t last = w.row_end - 1, alast = last & ~int64_t(RPT - 1);
+    const int64_t stride = int64_t(RPT) * kScatterBlock;
+    auto fetch = [&](int64_t at, VecT (&q)[NS], YT (&yy)[RPT]) {
+        const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + min(at, alast));
+#pragma unroll
+        for (int j = 0; j < NS; j++) q[j] = vec[slot[j]];
+#pragma unroll
+        for (int i = 0; i < RPT; i++) yy[i] = y[min(at + i, last)];
+    };
+    int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x;
+    VecT q[NS];
+    YT yr[RPT];
+    fetch(a, q, yr);
+    for (; a < w.row_end; a += stride) {
+        VecT qn[NS];
+        YT yn[RPT];
+        fetch(a + stride, qn, yn);
+        double yv[RPT];
+        bool ok[RPT];
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            ok[i] = a + i >= w.row_begin && a + i < w.row_end;
+            yv[i] = double(VT(yr[i]) * sc);
+        }
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            if (j < ns) {
+#pragma unroll
+                for (int i = 0; i < RPT; i++) {
+                    const uint32_t uc = off[j] + V::get(q[j], i);
+                    if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NS; j++) q[j] = qn[j];
+#pragma unroll
+        for (int i = 0; i < RPT; i++) yr[i] = yn[i];
+    }
+}
+
 // Stage 1 for stored indices: as k_scatter_part, with each thread taking RPT
 // consecutive rows per step (counted from the chunk's first row rounded down
 // to RPT) so one aligned vector load brings a slot's entries of all of them.
 // For groups of up to kMaxSegSlots slots the slot numbers and window offsets
-// are hoisted out of the row loop, so each step issues its y and index loads
-// back to back before the atomics.
+// are hoisted out of the row loop and the loads are software-pipelined
+// (scatter_rows).
 template <class VT, class YT, class IdxT, int RPT>
 __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> m,
                                                                 const ScatterWork* __restrict__ work,
@@ -715,45 +786,38 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     const uint32_t cb = sys_col[w.sys];
     const VT sc = scatter_scale<VT>(div, mode, w.sys);
     const int64_t vld = m.ld / RPT;
-    const int64_t last = w.row_end - 1;
-    int64_t slot[kMaxSegSlots];
-    uint32_t off[kMaxSegSlots];
+    if (ns <= kMaxSegSlots) {
+        int64_t slot[kMaxSegSlots];
+        uint32_t off[kMaxSegSlots];
+        const int s0 = slot_list[gr.first];
 #pragma unroll
-    for (int j = 0; j < kMaxSegSlots; j++) {
-        slot[j] = 0;
-        off[j] = 0;
-        if (j < ns) {
-            const int sj = slot_list[gr.first + j];
+        for (int j = 0; j < kMaxSegSlots; j++) {
+            const int sj = j < ns ? int(slot_list[gr.first + j]) : s0;
             slot[j] = int64_t(sj) * vld;
             off[j] = cb + m.base[sj] - w.win_lo;
         }
-    }
-    for (int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x; a < w.row_end;
-         a += int64_t(RPT) * kScatterBlock) {
-        bool ok[RPT];
-        double yv[RPT];
+        if (ns == 4)
+            scatter_rows<4, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, W, win);
+        else if (ns == 8)
+            scatter_rows<8, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, W, win);
+        else if (ns <= 2)
+            scatter_rows<2, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, W, win);
+        else if (ns <= 4)
+            scatter_rows<4, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, W, win);
+        else
+            scatter_rows<8, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, W, win);
+    } else {
+        for (int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x; a < w.row_end;
+             a += int64_t(RPT) * kScatterBlock) {
+            bool ok[RPT];
+            double yv[RPT];
 #pragma unroll
-        for (int i = 0; i < RPT; i++) {
-            ok[i] = a + i >= w.row_begin && a + i < w.row_end;
-            const double t = double(VT(y[min(a + i, last)]) * sc);  // clamped: no branch around the load
-            yv[i] = ok[i] ? t : 0.0;
-        }
-        const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + a);
-        if (ns <= kMaxSegSlots) {
-            VecT q[kMaxSegSlots];
-#pragma unroll
-            for (int j = 0; j < kMaxSegSlots; j++) q[j] = j < ns ? vec[slot[j]] : VecT{};
-#pragma unroll
-            for (int j = 0; j < kMaxSegSlots; j++) {
-                if (j < ns) {
-#pragma unroll
-                    for (int i = 0; i < RPT; i++) {
-                        const uint32_t uc = off[j] + V::get(q[j], i);
-                        if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
-                    }
-                }
+            for (int i = 0; i < RPT; i++) {
+                ok[i] = a + i >= w.row_begin && a + i < w.row_end;
+                const double t = double(VT(y[min(a + i, w.row_end - 1)]) * sc);
+                yv[i] = ok[i] ? t : 0.0;
             }
-        } else {
+            const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + a);
             for (int j = 0; j < ns; j++) {
                 const int sj = slot_list[gr.first + j];
                 const VecT q = vec[int64_t(sj) * vld];
