@@ -65,19 +65,29 @@ struct GatherSeg {
     uint16_t nwin, pad;
 };
 
+// Slots of one gather step (and of a slot group that the scatter tile
+// handles with hoisted slot offsets).
+constexpr int kMaxSegSlots = 8;
+
 // One step of the stored-index tile gather: up to kMaxSegSlots consecutive
 // slots of one segment.  A step with phase_count > 0 first stages the phase's
 // segments (phases[phase_first ..]) into LDS side by side; the segment's
 // columns start at lds_off there.  A step with nwin > 1 is a segment wider than
-// the LDS, staged window by window by the step itself.
+// the LDS, staged window by window by the step itself.  off[j] is slot
+// (slot_first + j)'s offset from its stored entry to the staged column:
+// base - col_lo (+ lds_off for a phased step), modulo 2^32.  64 bytes, read
+// by one scalar load.
 struct GatherStep {
+    uint32_t off[kMaxSegSlots];
     uint16_t slot_first, nslots;
     int16_t phase_first, phase_count;
     uint32_t col_lo;   // the segment's first column (system-local)
     uint32_t lds_off;  // the segment's offset in the staged phase
     uint32_t width;    // the segment's width (windows of a wide segment)
     uint16_t nwin, pad;
+    uint32_t pad2[2];
 };
+static_assert(sizeof(GatherStep) == 64, "GatherStep is one 64-byte scalar load");
 
 // A segment of a staging phase: columns [col_lo, col_lo + width) go to
 // LDS [lds_off, lds_off + width).

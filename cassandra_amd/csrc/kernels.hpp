@@ -349,6 +349,24 @@ struct IdxVec<uint32_t, 4> {
     }
 };
 
+// One 16-byte line from global memory into LDS: lane i of the wave writes
+// lds + 16 i (lds is the same for the whole wave).  Issued as an instruction
+// the compiler does not see as an LDS write, so its own waits for LDS reads
+// do not wait for this load -- nor for every vector load issued after it
+// (vmcnt counts in order); the caller waits for it (s_waitcnt vmcnt(0)) and
+// synchronises the block before any thread reads those bytes, and the block
+// barriers before and after a staging keep the LDS accesses in order around
+// it.  Declared without a memory clobber: one would mark every later load of
+// the kernel's read-only tables as possibly overwritten and turn their scalar
+// loads into vector loads with full vmcnt waits.  No code in these kernels
+// keeps a value in M0 across it (checked in the ISA).
+__device__ __forceinline__ void load_line_to_lds(const void* src, void* lds)
+{
+    const uint32_t la = __builtin_amdgcn_readfirstlane(
+        uint32_t(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) void*)lds)));
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la), "v"(src));
+}
+
 // Copies the W elements at src into LDS at dst + h, h = the element offset of
 // src inside its 16-byte line (dst is 16-byte aligned), and returns h: the
 // 16-byte lines wholly inside [src, src + W) go from memory straight into LDS
@@ -368,9 +386,7 @@ __device__ __forceinline__ uint32_t stage_direct(VT* __restrict__ dst, const VT*
     const VT* g = src + j0;
     VT* l = dst + h + j0;  // 16-byte aligned
     for (uint32_t t0 = wave * 64; t0 < nq; t0 += BS) {
-        if (t0 + lane < nq)
-            __builtin_amdgcn_global_load_lds(static_cast<const void*>(g + (t0 + lane) * Q),
-                                             (__attribute__((address_space(3))) void*)(l + t0 * Q), 16, 0, 0);
+        if (t0 + lane < nq) load_line_to_lds(g + (t0 + lane) * Q, l + t0 * Q);
     }
     if (threadIdx.x < Q) {
         const uint32_t a = threadIdx.x, b = j0 + nq * Q + threadIdx.x;
@@ -394,17 +410,36 @@ __device__ __forceinline__ uint32_t stage_direct(VT* __restrict__ dst, const VT*
 // in slot order, so every row is the sequential sum in slot order
 // (Math/MatrixCSR.cpp:59-62).
 constexpr int kTileBlock = 1024;
-constexpr int kMaxSegSlots = 8;
 
-template <class IdxT, int RPT>
+// Step k's descriptor in scalar registers (the same for the whole block).
+__device__ __forceinline__ GatherStep step_at(const GatherStep* __restrict__ steps, int k)
+{
+    const uint32_t* __restrict__ w = reinterpret_cast<const uint32_t*>(steps + k);
+    uint32_t r[sizeof(GatherStep) / 4];
+#pragma unroll
+    for (int i = 0; i < int(sizeof(GatherStep) / 4); i++) r[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    GatherStep st;
+    __builtin_memcpy(&st, r, sizeof(st));
+    return st;
+}
+
+// A step's entries of this thread's rows (threads past the tile read row 0
+// through vec; the sums ignore them).  With Uncond every slot up to
+// kMaxSegSlots is loaded (slots past the step's last repeat it): no branch
+// around the loads, so the compiler can count them and need not wait for all
+// loads in flight before a use; otherwise only the step's slots.
+template <class IdxT, int RPT, bool Uncond>
 __device__ __forceinline__ void load_step(typename IdxVec<IdxT, RPT>::T (&q)[kMaxSegSlots],
                                           const typename IdxVec<IdxT, RPT>::T* __restrict__ vec, int64_t vld,
-                                          const GatherStep& st, bool live)
+                                          const GatherStep& st)
 {
+    const int last = st.nslots - 1;
 #pragma unroll
     for (int j = 0; j < kMaxSegSlots; j++) {
-        q[j] = typename IdxVec<IdxT, RPT>::T{};
-        if (live && j < st.nslots) q[j] = vec[int64_t(st.slot_first + j) * vld];
+        if (Uncond)
+            q[j] = vec[int64_t(st.slot_first + min(j, last)) * vld];
+        else if (j <= last)
+            q[j] = vec[int64_t(st.slot_first + j) * vld];
     }
 }
 
@@ -447,11 +482,12 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
     // loads its own entries ahead of its staging.
     constexpr bool kPrefetch = sizeof(VecT) <= 8;
     VecT q[kMaxSegSlots], qn[kPrefetch ? kMaxSegSlots : 1];
-    if (kPrefetch && nsteps > 0) load_step<IdxT, RPT>(q, vec, vld, steps[0], live);
+    GatherStep st = nsteps > 0 ? step_at(steps, 0) : GatherStep{};
+    if (kPrefetch && nsteps > 0) load_step<IdxT, RPT, true>(q, vec, vld, st);
     for (int k = 0; k < nsteps; k++) {
-        const GatherStep st = steps[k];
         const bool more = k + 1 < nsteps;
-        if constexpr (!kPrefetch) load_step<IdxT, RPT>(q, vec, vld, st, live);
+        const GatherStep nx = step_at(steps, more ? k + 1 : k);
+        if constexpr (!kPrefetch) load_step<IdxT, RPT, false>(q, vec, vld, st);
         if (st.nwin > 1) {
             // a segment wider than the LDS, at most kMaxSegSlots slots
             VT v[RPT][kMaxSegSlots];
@@ -461,22 +497,25 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                 for (int i = 0; i < RPT; i++) v[i][j] = VT(0);
             for (uint32_t wl = 0; wl < st.width; wl += wmax) {
                 const uint32_t W = min(wmax, st.width - wl);
+                // the next step's entries load with the last window's staging
+                if constexpr (kPrefetch)
+                    if (more && wl + W >= st.width) load_step<IdxT, RPT, true>(qn, vec, vld, nx);
                 __syncthreads();
                 const uint32_t h = stage_direct<VT, kTileBlock>(xs, xb + st.col_lo + wl, W);
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
-                if constexpr (kPrefetch)
-                    if (more && wl + W >= st.width) load_step<IdxT, RPT>(qn, vec, vld, steps[k + 1], live);
 #pragma unroll
                 for (int j = 0; j < kMaxSegSlots; j++) {
                     if (j < st.nslots) {
                         // 32-bit offset: base - col_lo wraps for u32 storage
                         // (base 0) and the sum with the entry unwraps it
-                        const uint32_t off = m.base[st.slot_first + j] - st.col_lo - wl;
+                        const uint32_t off = st.off[j] - wl;
 #pragma unroll
                         for (int i = 0; i < RPT; i++) {
                             const uint32_t c = off + V::get(q[j], i);
-                            if (ok[i] && c < W) v[i][j] = xs[h + c];
+                            const bool in = ok[i] && c < W;
+                            const VT t = xs[h + (in ? c : 0u)];
+                            v[i][j] = in ? t : v[i][j];
                         }
                     }
                 }
@@ -496,17 +535,25 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                 }
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
+            } else if constexpr (kPrefetch) {
+                // this step's entries (loaded during the last step's sums)
+                // must be in before the next step's loads go out: the
+                // compiler's own wait for them would be a vmcnt(0) after
+                __builtin_amdgcn_s_waitcnt(0);
             }
             if constexpr (kPrefetch)
-                if (more) load_step<IdxT, RPT>(qn, vec, vld, steps[k + 1], live);
+                if (more) load_step<IdxT, RPT, true>(qn, vec, vld, nx);
             const uint32_t h = line_head(xb + st.col_lo);
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++) {
                 if (j < st.nslots) {
-                    const uint32_t off = m.base[st.slot_first + j] - st.col_lo + st.lds_off + h;
+                    const uint32_t off = st.off[j] + h;
 #pragma unroll
                     for (int i = 0; i < RPT; i++)
-                        if (ok[i]) acc[i] += xs[off + V::get(q[j], i)];
+                    {
+                        const VT t = xs[ok[i] ? off + V::get(q[j], i) : 0u];
+                        acc[i] += ok[i] ? t : VT(0);
+                    }
                 }
             }
         }
@@ -515,6 +562,7 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
 #pragma unroll
                 for (int j = 0; j < kMaxSegSlots; j++) q[j] = qn[j];
             }
+        st = nx;
     }
     double red = 0.0;
 #pragma unroll

@@ -401,6 +401,10 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
             st.nwin = sg.nwin;
             st.phase_first = -1;
             st.phase_count = 0;
+            for (int j = 0; j < st.nslots; j++) {
+                const int slot = st.slot_first + j;
+                st.off[j] = (m->ibytes == 2 ? m->slot_lo[size_t(slot)] : 0u) - sg.col_lo;
+            }
             if (!wide) {
                 if (j0 == 0) {
                     if (open_step < 0) {
@@ -410,6 +414,7 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
                     gp.phases.push_back(PhaseSeg{sg.col_lo, sg.width, phase_w, 0u});
                 }
                 st.lds_off = phase_w;
+                for (int j = 0; j < st.nslots; j++) st.off[j] += phase_w;
             }
             gp.steps.push_back(st);
         }
