@@ -16,6 +16,7 @@ ERRORS = {
     -1: "argument", -2: "size mismatch", -3: "pattern", -4: "device", -5: "out of memory", -6: "breakdown",
 }
 
+ERR_ARGUMENT, ERR_SIZE_MISMATCH, ERR_PATTERN, ERR_DEVICE, ERR_OUT_OF_MEMORY, ERR_BREAKDOWN = -1, -2, -3, -4, -5, -6
 SOLVER_LSQR, SOLVER_PCG, SOLVER_CG, SOLVER_LSMR = 0, 1, 2, 3
 PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 2
 LAYOUT_ROW_MAJOR, LAYOUT_SLOT_MAJOR = 0, 1
@@ -25,6 +26,16 @@ class CadError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"cassandra_amd error {code} ({ERRORS.get(code, '?')}): {msg}")
         self.code = code
+
+
+class Schedule(C.Structure):
+    """cad_schedule: the product schedule of a matrix."""
+    _fields_ = [
+        ("gather_f32_rpt", C.c_int), ("gather_f32_block", C.c_int),
+        ("gather_f64_rpt", C.c_int), ("gather_f64_block", C.c_int),
+        ("scatter_rpt", C.c_int), ("scatter_order", C.c_int),
+        ("scatter_tiles_per_cu", C.c_double), ("scatter_partial_cap", C.c_double),
+    ]
 
 
 class SolveOptions(C.Structure):
@@ -78,6 +89,8 @@ SIGNATURES = {
     "cad_matrix_systems": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(i64), vp]),
     "cad_matrix_from_positions_mf": (C.c_int, [vp, vp, i64, PP]),
     "cad_matrix_autotune": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
+    "cad_matrix_get_schedule": (C.c_int, [vp, C.POINTER(Schedule)]),
+    "cad_matrix_set_schedule": (C.c_int, [vp, C.POINTER(Schedule)]),
     "cad_matrix_create": (C.c_int, [vp, vp, i64, C.c_int, i64, PP]),
     "cad_matrix_destroy": (C.c_int, [vp]),
     "cad_matrix_info": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_int), C.POINTER(i64), C.POINTER(C.c_int)]),

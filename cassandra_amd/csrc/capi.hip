@@ -470,6 +470,32 @@ int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size)
     });
 }
 
+int cad_matrix_get_schedule(const cad_matrix* m, cad_schedule* out)
+{
+    return guarded([&] {
+        need(m != nullptr && out != nullptr, "null argument");
+        const TuneParams& t = m->tune;
+        *out = cad_schedule{t.rpt4, t.bs4, t.rpt8, t.bs8, t.srpt, t.scatter_order, t.blocks_per_cu, t.partial_cap};
+    });
+}
+
+int cad_matrix_set_schedule(cad_matrix* m, const cad_schedule* sc)
+{
+    return guarded([&] {
+        need(m != nullptr && sc != nullptr, "null argument");
+        TuneParams t;
+        t.rpt4 = sc->gather_f32_rpt;
+        t.bs4 = sc->gather_f32_block;
+        t.rpt8 = sc->gather_f64_rpt;
+        t.bs8 = sc->gather_f64_block;
+        t.srpt = sc->scatter_rpt;
+        t.scatter_order = sc->scatter_order;
+        t.blocks_per_cu = sc->scatter_tiles_per_cu;
+        t.partial_cap = sc->scatter_partial_cap;
+        set_schedule(m, t);
+    });
+}
+
 int cad_matrix_systems(const cad_matrix* m, int* nsys, int64_t* cols_per_system, int64_t* row_begin)
 {
     return guarded([&] {

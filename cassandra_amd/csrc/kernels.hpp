@@ -409,6 +409,9 @@ __device__ __forceinline__ uint32_t stage_direct(VT* __restrict__ dst, const VT*
 // registers until every window was seen, then the segment's slots are added
 // in slot order, so every row is the sequential sum in slot order
 // (Math/MatrixCSR.cpp:59-62).
+// Threads of a tile-gather block: kTileBlock (one block and the whole LDS per
+// CU), or half of it, with two blocks of half the LDS per CU whose staging
+// and sums overlap each other.
 constexpr int kTileBlock = 1024;
 
 // Step k's descriptor in scalar registers (the same for the whole block).
@@ -450,8 +453,8 @@ __device__ __forceinline__ uint32_t line_head(const VT* p)
     return uint32_t(reinterpret_cast<uintptr_t>(p) / sizeof(VT)) & (16 / sizeof(VT) - 1);
 }
 
-template <class VT, int RPT, class IdxT, class Epi>
-__global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, const RowWork* __restrict__ work,
+template <class VT, int RPT, class IdxT, class Epi, int BS>
+__global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const RowWork* __restrict__ work,
                                                             const uint32_t* __restrict__ sys_col,
                                                             const GatherStep* __restrict__ steps, int nsteps,
                                                             const PhaseSeg* __restrict__ phases, uint32_t wmax,
@@ -501,7 +504,7 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                 if constexpr (kPrefetch)
                     if (more && wl + W >= st.width) load_step<IdxT, RPT, true>(qn, vec, vld, nx);
                 __syncthreads();
-                const uint32_t h = stage_direct<VT, kTileBlock>(xs, xb + st.col_lo + wl, W);
+                const uint32_t h = stage_direct<VT, BS>(xs, xb + st.col_lo + wl, W);
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
 #pragma unroll
@@ -531,7 +534,7 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
                 __syncthreads();
                 for (int p = st.phase_first; p < st.phase_first + st.phase_count; p++) {
                     const PhaseSeg ps = phases[p];
-                    stage_direct<VT, kTileBlock>(xs + ps.lds_off, xb + ps.col_lo, ps.width);
+                    stage_direct<VT, BS>(xs + ps.lds_off, xb + ps.col_lo, ps.width);
                 }
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
@@ -568,7 +571,7 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile(SlotMajor<IdxT> m, c
 #pragma unroll
     for (int i = 0; i < RPT; i++)
         if (ok[i]) red += epi.row(a + i, acc[i]);
-    reduce_tail<kTileBlock>(epi, red, w.sys, g);
+    reduce_tail<BS>(epi, red, w.sys, g);
 }
 
 // The LDS-staged gather of a matrix-free pattern matrix: as k_gather_tile,

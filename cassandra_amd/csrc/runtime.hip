@@ -342,17 +342,21 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
 // slots that follow each other in slot order (true for pattern matrices, whose
 // groups are the patterns); groups wider than the LDS may hold at most
 // kMaxSegSlots slots.
-void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
+void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int bs)
 {
     const Context& ctx = *m->ctx;
     const ScatterPlan& sp = m->splan;
     gp.ok = false;
     gp.rpt = rpt;
+    gp.bs = bs;
     if (rpt <= 0) return;  // the row kernel
+    if (bs != kTileBlock && (bs != kTileBlock / 2 || m->mf)) return;
+    const int per_cu = kTileBlock / bs;  // blocks resident per CU
     // A staged run of columns starts at its element offset inside a 16-byte
     // line (up to Q - 1 elements) so that whole lines map to whole LDS lines:
     // windows leave that room, and phase segments start on line boundaries.
-    const uint32_t total = uint32_t(lds_budget(ctx) / vbytes), Q = uint32_t(16 / vbytes);
+    const uint32_t total = uint32_t((lds_budget(ctx) + 1024) / per_cu - 1024) / uint32_t(vbytes);
+    const uint32_t Q = uint32_t(16 / vbytes);
     gp.wmax = total - (Q - 1);
     std::vector<const SlotGroup*> order;
     for (auto& g : sp.groups) order.push_back(&g);
@@ -426,11 +430,11 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
         }
     }
     // Row tiles: a thread owns rpt consecutive rows counted from the tile's
-    // first row rounded down to rpt, so a tile spans at most kTileBlock * rpt
+    // first row rounded down to rpt, so a tile spans at most bs * rpt
     // rows from there; tiles of a system are a multiple of rpt rows long (their
     // misalignment is the system's).  A single system is cut into a multiple of
     // the CU count so that every CU gets the same number of tiles.
-    const int64_t cap = int64_t(kTileBlock) * rpt - (rpt - 1);
+    const int64_t cap = int64_t(bs) * rpt - (rpt - 1);
     gp.tiles.clear();
     gp.first.assign(m->nsys, 0);
     gp.count.assign(m->nsys, 0);
@@ -438,7 +442,7 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt)
         const int64_t r0 = m->sys_row_begin[s], R = m->sys_row_begin[s + 1] - r0;
         gp.first[s] = int32_t(gp.tiles.size());
         int64_t n = (R + cap - 1) / cap;
-        if (m->nsys == 1 && n > 0) n = (n + ctx.cus - 1) / ctx.cus * ctx.cus;
+        if (m->nsys == 1 && n > 0) n = (n + ctx.cus * per_cu - 1) / (ctx.cus * per_cu) * (ctx.cus * per_cu);
         int64_t per = n > 0 ? (R + n - 1) / n : 0;
         per = std::min(cap / rpt * rpt, (per + rpt - 1) / rpt * rpt);
         for (int64_t t = 0; per > 0 && t * per < R; t++)
@@ -470,8 +474,8 @@ void build_plans(cad_matrix* m)
     // (a matrix-free matrix sizes its scatter partials as a u16 matrix would)
     build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
                        m->mf ? 2 : m->ibytes, ctx, m->tune);
-    build_gather_plan(m, m->gplan4, 4, m->tune.rpt4);
-    build_gather_plan(m, m->gplan8, 8, m->tune.rpt8);
+    build_gather_plan(m, m->gplan4, 4, m->tune.rpt4, m->tune.bs4);
+    build_gather_plan(m, m->gplan8, 8, m->tune.rpt8, m->tune.bs8);
 
     GridPlan& g = m->grid;
     g.rows.clear();

@@ -179,6 +179,16 @@ void with_rpt(int rpt, F&& f)
     throw Error(CAD_ERR_ARGUMENT, "unsupported rows per thread of the tile gather");
 }
 
+// Threads per block of the stored-index tile gather (a schedule knob):
+// kTileBlock, or half of it with two blocks per CU.
+template <class F>
+void with_tile_block(int bs, F&& f)
+{
+    if (bs == kTileBlock) return f(std::integral_constant<int, kTileBlock>{});
+    if (bs == kTileBlock / 2) return f(std::integral_constant<int, kTileBlock / 2>{});
+    throw Error(CAD_ERR_ARGUMENT, "unsupported block size of the tile gather");
+}
+
 // Row gather with an epilogue: the LDS-staged tile kernel for pattern
 // matrices, the generic row kernel otherwise.
 template <class VT, class Epi>
@@ -200,12 +210,15 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
                                        gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
                                        gp.d_segs.as<GatherSeg>(), int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
                 } else {
-                    auto kern = k_gather_tile<VT, RPT, IdxT, Epi>;
-                    allow_lds(kern, lds);
-                    hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
-                                       gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
-                                       gp.d_steps.as<GatherStep>(), int(gp.steps.size()),
-                                       gp.d_phases.as<PhaseSeg>(), gp.wmax, x, epi, gp.sys());
+                    with_tile_block(gp.bs, [&](auto bc) {
+                        constexpr int BS = decltype(bc)::value;
+                        auto kern = k_gather_tile<VT, RPT, IdxT, Epi, BS>;
+                        allow_lds(kern, lds);
+                        hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(BS), lds, s, view,
+                                           gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
+                                           gp.d_steps.as<GatherStep>(), int(gp.steps.size()),
+                                           gp.d_phases.as<PhaseSeg>(), gp.wmax, x, epi, gp.sys());
+                    });
                 }
             });
         } else {
@@ -408,7 +421,10 @@ static void prepare_gather(cad_matrix* m)
             if constexpr (std::is_void_v<IdxT>)
                 allow_lds(k_gather_tile_mf<VT, RPT, Epi>, size_t(gp.lds_elems) * sizeof(VT));
             else
-                allow_lds(k_gather_tile<VT, RPT, IdxT, Epi>, size_t(gp.lds_elems) * sizeof(VT));
+                with_tile_block(gp.bs, [&](auto bc) {
+                    allow_lds(k_gather_tile<VT, RPT, IdxT, Epi, decltype(bc)::value>,
+                              size_t(gp.lds_elems) * sizeof(VT));
+                });
         });
     });
 }
@@ -967,6 +983,39 @@ void drop_graphs(cad_matrix* m)
 
 }  // namespace
 
+void set_schedule(cad_matrix* m, const TuneParams& tp)
+{
+    const bool u16 = !m->mf && m->ibytes == 2;
+    auto one_of = [](int v, std::initializer_list<int> ok) {
+        for (int o : ok)
+            if (v == o) return true;
+        return false;
+    };
+    if (!(u16 ? one_of(tp.rpt4, {0, 2, 4, 8}) : one_of(tp.rpt4, {0, 2, 4})) || !one_of(tp.rpt8, {0, 1, 2}))
+        throw Error(CAD_ERR_ARGUMENT, "unsupported rows per thread of the tile gather");
+    for (int b : {tp.bs4, tp.bs8})
+        if (!(b == kTileBlock || (b == kTileBlock / 2 && !m->mf)))
+            throw Error(CAD_ERR_ARGUMENT, "unsupported block size of the tile gather");
+    if (!(u16 ? one_of(tp.srpt, {4, 8}) : tp.srpt == 4) || !one_of(tp.scatter_order, {0, 1}) ||
+        !(tp.blocks_per_cu > 0 && tp.blocks_per_cu <= 64) || !(tp.partial_cap > 0 && tp.partial_cap <= 1))
+        throw Error(CAD_ERR_ARGUMENT, "unsupported scatter schedule");
+    CAD_HIP(hipStreamSynchronize(m->ctx->stream));
+    drop_graphs(m);
+    const TuneParams keep = m->tune;
+    m->tune = tp;
+    build_gather_plan(m, m->gplan4, 4, tp.rpt4, tp.bs4);
+    build_gather_plan(m, m->gplan8, 8, tp.rpt8, tp.bs8);
+    if ((tp.rpt4 > 0 && !m->gplan4.ok) || (tp.rpt8 > 0 && !m->gplan8.ok)) {
+        m->tune = keep;
+        build_gather_plan(m, m->gplan4, 4, keep.rpt4, keep.bs4);
+        build_gather_plan(m, m->gplan8, 8, keep.rpt8, keep.bs8);
+        throw Error(CAD_ERR_ARGUMENT, "this matrix cannot use the tile gather");
+    }
+    build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
+                       m->mf ? 2 : m->ibytes, *m->ctx, tp);
+    CAD_HIP(hipStreamSynchronize(m->ctx->stream));
+}
+
 std::string autotune(cad_matrix* m)
 {
     hipStream_t s = m->ctx->stream;
@@ -981,38 +1030,31 @@ std::string autotune(cad_matrix* m)
     std::string log;
     char line[160];
     TuneParams& tp = m->tune;
-    // FP32 gather: tile rows per thread, or the row kernel (0)
-    {
-        const std::vector<int> cands = (!m->mf && m->ibytes == 2) ? std::vector<int>{4, 8, 2, 0}
-                                                                  : std::vector<int>{4, 2, 0};
-        int best = tp.rpt4;
+    // gathers: tile rows per thread (0: the row kernel) x threads per block
+    // (1024, or 512 with two blocks per CU; stored indices only)
+    const std::vector<int> blocks = m->mf ? std::vector<int>{kTileBlock} : std::vector<int>{kTileBlock, kTileBlock / 2};
+    auto tune_gather = [&](int vbytes, const std::vector<int>& rpts, int& rpt, int& bs, auto&& run) {
+        GatherPlan& gp = vbytes == 4 ? m->gplan4 : m->gplan8;
+        int best_r = rpt, best_b = bs;
         double best_us = 1e300;
-        for (int c : cands) {
-            build_gather_plan(m, m->gplan4, 4, c);
-            if (c > 0 && !m->gplan4.ok) continue;
-            const double us = t.us([&] { spmv<float>(m, x4.as<float>(), y4.as<float>(), s); }, s);
-            std::snprintf(line, sizeof line, "gather f32 rpt %d: %.1f us\n", c, us);
-            log += line;
-            if (us < best_us) best_us = us, best = c;
-        }
-        tp.rpt4 = best;
-        build_gather_plan(m, m->gplan4, 4, best);
-    }
-    // FP64 gather
-    {
-        int best = tp.rpt8;
-        double best_us = 1e300;
-        for (int c : {2, 1, 0}) {
-            build_gather_plan(m, m->gplan8, 8, c);
-            if (c > 0 && !m->gplan8.ok) continue;
-            const double us = t.us([&] { spmv<double>(m, x8.as<double>(), y8.as<double>(), s); }, s);
-            std::snprintf(line, sizeof line, "gather f64 rpt %d: %.1f us\n", c, us);
-            log += line;
-            if (us < best_us) best_us = us, best = c;
-        }
-        tp.rpt8 = best;
-        build_gather_plan(m, m->gplan8, 8, best);
-    }
+        for (int r : rpts)
+            for (int b : blocks) {
+                if (r == 0 && b != kTileBlock) continue;
+                build_gather_plan(m, gp, vbytes, r, b);
+                if (r > 0 && !gp.ok) continue;
+                const double us = t.us(run, s);
+                std::snprintf(line, sizeof line, "gather f%d rpt %d block %d: %.1f us\n", 8 * vbytes, r, b, us);
+                log += line;
+                if (us < best_us) best_us = us, best_r = r, best_b = b;
+            }
+        rpt = best_r;
+        bs = best_b;
+        build_gather_plan(m, gp, vbytes, rpt, bs);
+    };
+    tune_gather(4, (!m->mf && m->ibytes == 2) ? std::vector<int>{4, 8, 2, 0} : std::vector<int>{4, 2, 0}, tp.rpt4,
+                tp.bs4, [&] { spmv<float>(m, x4.as<float>(), y4.as<float>(), s); });
+    tune_gather(8, std::vector<int>{2, 1, 0}, tp.rpt8, tp.bs8,
+                [&] { spmv<double>(m, x8.as<double>(), y8.as<double>(), s); });
     // scatter: work order, rows per step, tiles per CU and the partial-traffic
     // cap (FP32 timing)
     {
@@ -1051,8 +1093,9 @@ std::string autotune(cad_matrix* m)
     CAD_HIP(hipStreamSynchronize(s));
     m->tuned = true;
     std::snprintf(line, sizeof line,
-                  "chosen: gather f32 rpt %d, f64 rpt %d, scatter order %d rpt %d blocks/CU %.0f cap %.2f\n",
-                  tp.rpt4, tp.rpt8, tp.scatter_order, tp.srpt, tp.blocks_per_cu, tp.partial_cap);
+                  "chosen: gather f32 rpt %d block %d, f64 rpt %d block %d, scatter order %d rpt %d blocks/CU %.0f "
+                  "cap %.2f\n",
+                  tp.rpt4, tp.bs4, tp.rpt8, tp.bs8, tp.scatter_order, tp.srpt, tp.blocks_per_cu, tp.partial_cap);
     log += line;
     return log;
 }

@@ -43,5 +43,7 @@ void matrix_targets(cad_matrix* m, const double* xt, uint64_t seed, double sn, d
 // Times the candidate schedules of the FP32 products on this matrix and
 // keeps the fastest (rebuilds the plans; drops captured solver graphs).
 std::string autotune(cad_matrix* m);
+// Sets the schedule (validated; drops the captured graphs, rebuilds the plans).
+void set_schedule(cad_matrix* m, const TuneParams& tp);
 
 }  // namespace cad

@@ -211,6 +211,22 @@ class Matrix:
         call("cad_matrix_autotune", self.h, buf, len(buf))
         return buf.value.decode()
 
+    def schedule(self) -> dict:
+        """The product schedule in use (cad_schedule fields)."""
+        sc = capi.Schedule()
+        call("cad_matrix_get_schedule", self.h, C.byref(sc))
+        return {name: getattr(sc, name) for name, _ in capi.Schedule._fields_}
+
+    def set_schedule(self, **knobs) -> None:
+        """Change some fields of the product schedule (the rest stay)."""
+        cur = self.schedule()
+        unknown = set(knobs) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown schedule fields {sorted(unknown)}")
+        cur.update(knobs)
+        sc = capi.Schedule(**cur)
+        call("cad_matrix_set_schedule", self.h, C.byref(sc))
+
     def download(self) -> np.ndarray:
         out = np.zeros((self.rows, self.k), dtype=np.uint32)
         call("cad_matrix_download", self.h, out.ctypes.data_as(C.c_void_p))

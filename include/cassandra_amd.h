@@ -121,6 +121,25 @@ int cad_matrix_from_positions_mf(cad_patterns* p, const uint64_t* d_positions, i
  * schedule sums each row in the same order); A^T y stays exact on integer data
  * and equal to rounding otherwise. */
 int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size);
+/* No counterpart in the reference.  The schedule of a matrix's products (what
+ * cad_matrix_autotune chooses): reading it, or setting one explicitly (the
+ * solvers' captured graphs are dropped).  Rows per thread of the FP32 tile
+ * gather 2, 4 or 8 (8 for u16 indices only), of the FP64 one 1 or 2, 0 for
+ * the row kernel; threads per gather block 1024, or 512 with two blocks per CU
+ * (stored indices only); scatter rows per step 4, or 8 for u16 indices;
+ * scatter order 0 (chunks per window) or 1 (chunks shared by the windows,
+ * XCD-major); scatter tiles per CU > 0 and partial-traffic cap in (0, 1].
+ * Every schedule computes the same products (see cad_matrix_autotune);
+ * CAD_ERR_ARGUMENT for a value outside these or a tile gather the matrix
+ * cannot use. */
+typedef struct cad_schedule {
+    int gather_f32_rpt, gather_f32_block;
+    int gather_f64_rpt, gather_f64_block;
+    int scatter_rpt, scatter_order;
+    double scatter_tiles_per_cu, scatter_partial_cap;
+} cad_schedule;
+int cad_matrix_get_schedule(const cad_matrix* m, cad_schedule* out);
+int cad_matrix_set_schedule(cad_matrix* m, const cad_schedule* schedule);
 /* nsys (1 for an ordinary matrix), columns per system and, when row_begin is
  * not NULL, the nsys + 1 row boundaries. */
 int cad_matrix_systems(const cad_matrix* m, int* nsys, int64_t* cols_per_system, int64_t* row_begin);
