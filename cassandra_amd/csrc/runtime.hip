@@ -329,6 +329,18 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
     const size_t lds = lds_budget(ctx);
     make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 4), 4, ibytes, ctx.cus, tune, sp.f4);
     make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 8), 8, ibytes, ctx.cus, tune, sp.f8);
+    // bound on the terms a window column receives in one work item (rows x
+    // slots of the group), for the fixed-point scale of the FP32 scatter
+    for (ScatterLists* L : {&sp.f4, &sp.f8}) {
+        L->cbits = 0;
+        for (const ScatterWork& w : L->work) {
+            uint64_t n = uint64_t(std::max<int64_t>(0, w.row_end - w.row_begin)) * sp.groups[size_t(w.group)].nslots;
+            int b = 0;
+            while (n) b++, n >>= 1;
+            L->cbits = std::max(L->cbits, b);
+        }
+    }
+    sp.d_ymax = DBuf(sizeof(unsigned) * std::max<size_t>(1, sys_rows.size() - 1));
     sp.d_groups = to_device(sp.groups, ctx.stream);
     sp.d_slot_list = to_device(sp.slot_list, ctx.stream);
     for (ScatterLists* L : {&sp.f4, &sp.f8}) {

@@ -56,6 +56,7 @@ struct ScatterLists {
     uint32_t widest = 0;     // widest window (LDS of the tile kernel)
     int64_t part_elems = 0;  // partial buffer length
     DBuf part;               // partial buffer (allocated on first use)
+    int cbits = 0;           // a window column gets fewer than 2^cbits terms per work item
 };
 
 // Slot groups + per-slot column ranges -> scatter work lists.
@@ -64,6 +65,7 @@ struct ScatterPlan {
     std::vector<uint16_t> slot_list;
     DBuf d_groups, d_slot_list;
     ScatterLists f4, f8;  // float and double (and uint32 counts) windows
+    DBuf d_ymax;          // per system: bits of max |y| of the last FP32 scatter (fixed-point scale)
 };
 
 // LDS-staged gather of one value size: row tiles and the slot segments.

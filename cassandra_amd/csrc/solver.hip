@@ -101,17 +101,30 @@ void with_scatter_kernel(int rpt, F&& f)
 
 template <class VT, class YT, class ZT, class View>
 void launch_scatter(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
-                    const LsqrScalars* div, int mode, ZT* z, hipStream_t s)
+                    const LsqrScalars* div, int mode, ZT* z, const GridPlan& g, int nsys, hipStream_t s)
 {
     ScatterLists& L = lists<VT>(sp);
     VT* part = partials<VT>(L);
     if (!L.work.empty()) {
+        // the FP32 stored-index scatter accumulates in fixed point: outside an
+        // LSQR iteration (whose terms are bounded) it first needs max |y| per
+        // system
+        FixedScale fx{nullptr, L.cbits};
+        constexpr bool kFixed = std::is_same_v<VT, float> && (std::is_same_v<View, SlotMajor<uint16_t>> ||
+                                                              std::is_same_v<View, SlotMajor<uint32_t>>);
+        if (kFixed && mode != kScatterLsqr && !g.rows.empty()) {
+            fill<unsigned>(sp.d_ymax.as<unsigned>(), nsys, 0u, s);
+            hipLaunchKernelGGL(k_absmax_bits<YT>, dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s,
+                               g.d_rows.as<RowWork>(), y, sp.d_ymax.as<unsigned>());
+            CAD_HIP(hipGetLastError());
+            fx.ymax = sp.d_ymax.as<unsigned>();
+        }
         const size_t lds = size_t(L.widest) * sizeof(double);
         with_scatter_kernel<VT, YT, View>(rpt, [&](auto kern) {
             allow_lds(kern, lds);
             hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
                                L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
-                               sp.d_slot_list.as<uint16_t>(), sys_col, y, div, mode, part);
+                               sp.d_slot_list.as<uint16_t>(), sys_col, y, div, mode, fx, part);
         });
         CAD_HIP(hipGetLastError());
     }
@@ -140,7 +153,8 @@ void scatter(cad_matrix* m, const YT* y, const LsqrScalars* div, ZT* z, hipStrea
 {
     const int mode = div ? kScatterLsqr : kScatterAll;
     with_view(m, [&](auto view) {
-        launch_scatter<VT, YT, ZT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), y, div, mode, z, s);
+        launch_scatter<VT, YT, ZT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), y, div, mode, z,
+                                   m->grid, m->nsys, s);
     });
 }
 
@@ -150,7 +164,7 @@ void scatter_restart(cad_matrix* m, const YT* y, const LsqrScalars* sel, ZT* z, 
 {
     with_view(m, [&](auto view) {
         launch_scatter<VT, YT, ZT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), y, sel,
-                                   kScatterRestart, z, s);
+                                   kScatterRestart, z, m->grid, m->nsys, s);
     });
 }
 
