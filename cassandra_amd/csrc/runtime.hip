@@ -340,7 +340,6 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
             L->cbits = std::max(L->cbits, b);
         }
     }
-    sp.d_ymax = DBuf(sizeof(unsigned) * std::max<size_t>(1, sys_rows.size() - 1));
     sp.d_groups = to_device(sp.groups, ctx.stream);
     sp.d_slot_list = to_device(sp.slot_list, ctx.stream);
     for (ScatterLists* L : {&sp.f4, &sp.f8}) {
@@ -518,6 +517,9 @@ void build_plans(cad_matrix* m)
     g.d_row_counters = to_device(zeros, ctx.stream);
     g.d_col_counters = to_device(zeros, ctx.stream);
     g.d_partials = DBuf(2 * sizeof(double) * std::max<size_t>(1, std::max(g.rows.size(), g.cols.size())));
+    // max |y| per row block and per system, for the fixed-point FP32 scatter
+    m->splan.d_ymax_part = DBuf(sizeof(unsigned) * std::max<size_t>(1, g.rows.size()));
+    m->splan.d_ymax = DBuf(sizeof(unsigned) * size_t(std::max(1, m->nsys)));
 }
 
 }  // namespace cad

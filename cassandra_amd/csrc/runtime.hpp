@@ -66,6 +66,7 @@ struct ScatterPlan {
     DBuf d_groups, d_slot_list;
     ScatterLists f4, f8;  // float and double (and uint32 counts) windows
     DBuf d_ymax;          // per system: bits of max |y| of the last FP32 scatter (fixed-point scale)
+    DBuf d_ymax_part;     // the same per row block of the grid plan
 };
 
 // LDS-staged gather of one value size: row tiles and the slot segments.

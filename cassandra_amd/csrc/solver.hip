@@ -113,9 +113,11 @@ void launch_scatter(ScatterPlan& sp, const View& view, int rpt, const uint32_t* 
         constexpr bool kFixed = std::is_same_v<VT, float> && (std::is_same_v<View, SlotMajor<uint16_t>> ||
                                                               std::is_same_v<View, SlotMajor<uint32_t>>);
         if (kFixed && mode != kScatterLsqr && !g.rows.empty()) {
-            fill<unsigned>(sp.d_ymax.as<unsigned>(), nsys, 0u, s);
-            hipLaunchKernelGGL(k_absmax_bits<YT>, dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s,
-                               g.d_rows.as<RowWork>(), y, sp.d_ymax.as<unsigned>());
+            hipLaunchKernelGGL(k_absmax_blocks<YT>, dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s,
+                               g.d_rows.as<RowWork>(), y, sp.d_ymax_part.as<unsigned>());
+            hipLaunchKernelGGL(k_absmax_systems, dim3(unsigned(nsys)), dim3(kRowBlock), 0, s,
+                               g.d_row_first.as<int32_t>(), g.d_row_count.as<int32_t>(),
+                               (const unsigned*)sp.d_ymax_part.as<unsigned>(), sp.d_ymax.as<unsigned>());
             CAD_HIP(hipGetLastError());
             fx.ymax = sp.d_ymax.as<unsigned>();
         }
