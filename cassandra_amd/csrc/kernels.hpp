@@ -798,64 +798,6 @@ __global__ __launch_bounds__(kRowBlock) void k_absmax_systems(const int32_t* __r
     }
 }
 
-// Integer accumulation of an FP32 scatter window.  Each term y * scale is
-// rounded to a multiple of 2^-s and the window adds the int64 multiples with
-// LDS integer atomics (ds_add_u64), which run about 1.8 times as fast as the
-// FP64 ones on gfx950 and make the sums independent of the atomics' order
-// (the FP32 A^T y is bit-reproducible).  s is chosen so that no column sum
-// can overflow: |term| < 2^ey and fewer than 2^cbits terms per column and
-// work item, so every sum stays below 2^(62 - 1); each term keeps at least 32
-// bits (rounding error <= 2^(ey - 33) of the largest term, far below FP32
-// rounding of the sums).  ymax holds, per system, the bits of max |y| (a
-// non-negative float, NaN and inf compare above every finite value); null in
-// an LSQR iteration, where |u_r / beta| < 2 by construction.
-struct FixedScale {
-    const unsigned* ymax;
-    int cbits;
-};
-
-// 2^s for system sys, or 0 where the FP64 window must be used (non-finite
-// values, or fewer than 32 bits left for the largest term).
-__device__ __forceinline__ double fixed_scale(const FixedScale& fx, int sys, double sc)
-{
-    int ey;
-    if (fx.ymax) {
-        const unsigned b = fx.ymax[sys];
-        if (b >= 0x7f800000u) return 0.0;
-        ey = b == 0 ? -126 : ilogbf(__uint_as_float(b)) + 1;
-    } else {
-        if (!isfinite(sc)) return 0.0;
-        ey = 1;
-    }
-    const int sh = 62 - fx.cbits - ey;
-    if (sh + ey < 32) return 0.0;
-    return ldexp(1.0, sh);
-}
-
-template <class WT>
-__device__ __forceinline__ WT window_term(double v, double fscale)
-{
-    if constexpr (std::is_same_v<WT, double>)
-        return v;
-    else
-        return WT(__double2ll_rn(v * fscale));
-}
-
-// Per system: the bits of max |VT(y)| over its rows (non-negative floats
-// order as unsigned integers; NaN and inf above every finite value), the
-// ymax of FixedScale.  bits must be zero before the launch.
-template <class YT>
-__global__ __launch_bounds__(kRowBlock) void k_absmax_bits(const RowWork* __restrict__ work,
-                                                           const YT* __restrict__ y, unsigned* __restrict__ bits)
-{
-    const RowWork w = work[blockIdx.x];
-    unsigned mb = 0;
-    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kRowBlock)
-        mb = max(mb, __float_as_uint(fabsf(float(y[r]))));
-    for (int o = 32; o > 0; o >>= 1) mb = max(mb, unsigned(__shfl_xor(int(mb), o, 64)));
-    if ((threadIdx.x & 63) == 0 && mb != 0) atomicMax(&bits[w.sys], mb);
-}
-
 // A^T (y * scale), stage 1: each block owns a column window of one slot group
 // in LDS, scatters a chunk of rows into it and stores the window to its slice
 // of the partial buffer -- no global atomics.  The window accumulates in FP64
