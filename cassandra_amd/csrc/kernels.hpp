@@ -178,7 +178,6 @@ template <class VT>
 struct EpiStore {
     VT* __restrict__ y;
     static constexpr bool kReduce = false;
-    static constexpr bool kPreload = false;
     __device__ bool begin(int) { return true; }
     __device__ double row(int64_t r, VT a) { y[r] = a; return 0.0; }
     __device__ void finish(int, double) {}
@@ -192,9 +191,7 @@ struct EpiLsqrU {
     VT* __restrict__ u;
     LsqrScalars* scal;
     VT f;
-    VT pre[8];  // u of the thread's rows, loaded ahead by the tile gather (preload)
     static constexpr bool kReduce = true;
-    static constexpr bool kPreload = true;
     __device__ bool begin(int sys)
     {
         const LsqrScalars* st = &scal[sys];
@@ -206,19 +203,6 @@ struct EpiLsqrU {
     __device__ double row(int64_t r, VT a)
     {
         const VT v = a - f * u[r];
-        u[r] = v;
-        return double(v) * double(v);
-    }
-    // the tile gather's rows a .. a + RPT - 1 (clamped reads past last)
-    template <int RPT>
-    __device__ void preload(int64_t a, int64_t last)
-    {
-#pragma unroll
-        for (int i = 0; i < RPT; i++) pre[i] = u[min(a + i, last)];
-    }
-    __device__ double row_pre(int i, int64_t r, VT a)
-    {
-        const VT v = a - f * pre[i];
         u[r] = v;
         return double(v) * double(v);
     }
@@ -239,7 +223,6 @@ struct EpiResidual {
     VT* __restrict__ u;
     LsqrScalars* scal;
     static constexpr bool kReduce = true;
-    static constexpr bool kPreload = false;
     __device__ bool begin(int sys) { return scal[sys].active != 0 && scal[sys].restart != 0; }
     __device__ double row(int64_t r, double a)
     {
@@ -262,7 +245,6 @@ struct EpiTargets {
     uint64_t seed;
     double sn;
     static constexpr bool kReduce = false;
-    static constexpr bool kPreload = false;
     __device__ bool begin(int) { return true; }
     __device__ double row(int64_t r, double a)
     {
@@ -476,8 +458,7 @@ __global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const Row
                                                             const uint32_t* __restrict__ sys_col,
                                                             const GatherStep* __restrict__ steps, int nsteps,
                                                             const PhaseSeg* __restrict__ phases, uint32_t wmax,
-                                                            int pre_step, const VT* __restrict__ x, Epi epi,
-                                                            GridSys g)
+                                                            const VT* __restrict__ x, Epi epi, GridSys g)
 {
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
@@ -506,20 +487,11 @@ __global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const Row
     VecT q[kMaxSegSlots], qn[kPrefetch ? kMaxSegSlots : 1];
     GatherStep st = nsteps > 0 ? step_at(steps, 0) : GatherStep{};
     if (kPrefetch && nsteps > 0) load_step<IdxT, RPT, true>(q, vec, vld, st);
-    // One step; the steps from pre_step on contain no wide segment (the
-    // plan's guarantee), so their copy of the loop is compiled without that
-    // path and the epilogue's preloaded registers do not add to its peak.
-    auto step = [&](int k, auto wide_tag) {
-        constexpr bool kWide = decltype(wide_tag)::value;
+    for (int k = 0; k < nsteps; k++) {
         const bool more = k + 1 < nsteps;
         const GatherStep nx = step_at(steps, more ? k + 1 : k);
-        // the epilogue's per-row inputs load with the tile's last staging
-        if constexpr (Epi::kPreload) {
-            if constexpr (!kWide)
-                if (k == pre_step) epi.template preload<RPT>(a, w.row_end - 1);
-        }
         if constexpr (!kPrefetch) load_step<IdxT, RPT, false>(q, vec, vld, st);
-        if (kWide && st.nwin > 1) {
+        if (st.nwin > 1) {
             // a segment wider than the LDS, at most kMaxSegSlots slots
             VT v[RPT][kMaxSegSlots];
 #pragma unroll
@@ -594,19 +566,11 @@ __global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const Row
                 for (int j = 0; j < kMaxSegSlots; j++) q[j] = qn[j];
             }
         st = nx;
-    };
-    int k = 0;
-    for (; k < min(pre_step, nsteps); k++) step(k, std::true_type{});
-    for (; k < nsteps; k++) step(k, std::false_type{});
+    }
     double red = 0.0;
 #pragma unroll
-    for (int i = 0; i < RPT; i++) {
-        if constexpr (Epi::kPreload) {
-            if (ok[i]) red += pre_step < nsteps ? epi.row_pre(i, a + i, acc[i]) : epi.row(a + i, acc[i]);
-        } else {
-            if (ok[i]) red += epi.row(a + i, acc[i]);
-        }
-    }
+    for (int i = 0; i < RPT; i++)
+        if (ok[i]) red += epi.row(a + i, acc[i]);
     reduce_tail<BS>(epi, red, w.sys, g);
 }
 

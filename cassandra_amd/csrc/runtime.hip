@@ -440,16 +440,6 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int b
             phase_w = (phase_w + sg.width + Q - 1 + Q - 1) / Q * Q;
         }
     }
-    // the epilogue's per-row inputs are loaded with the last staging, unless
-    // a wide segment comes at or after it (none follows pre_step)
-    gp.pre_step = int(gp.steps.size());
-    for (size_t i = gp.steps.size(); i-- > 0;) {
-        if (gp.steps[i].nwin > 1) break;
-        if (gp.steps[i].phase_count > 0) {
-            gp.pre_step = int(i);
-            break;
-        }
-    }
     // Row tiles: a thread owns rpt consecutive rows counted from the tile's
     // first row rounded down to rpt, so a tile spans at most bs * rpt
     // rows from there; tiles of a system are a multiple of rpt rows long (their

@@ -79,7 +79,6 @@ struct GatherPlan {
     std::vector<GatherSeg> segs;     // matrix-free kernel: one staging per segment
     std::vector<GatherStep> steps;   // stored-index kernel: packed phases, prefetched steps
     std::vector<PhaseSeg> phases;
-    int pre_step = 0;                // first step with no wide segment at or after it that stages
     std::vector<RowWork> tiles;
     std::vector<int32_t> first, count;
     DBuf d_segs, d_steps, d_phases, d_tiles, d_first, d_count, d_counters, d_partials;

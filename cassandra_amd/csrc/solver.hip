@@ -233,7 +233,7 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
                         hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(BS), lds, s, view,
                                            gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
                                            gp.d_steps.as<GatherStep>(), int(gp.steps.size()),
-                                           gp.d_phases.as<PhaseSeg>(), gp.wmax, gp.pre_step, x, epi, gp.sys());
+                                           gp.d_phases.as<PhaseSeg>(), gp.wmax, x, epi, gp.sys());
                     });
                 }
             });
