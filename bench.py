@@ -465,9 +465,17 @@ def main():
     spmv_gbps = bytes_pass / (spmv_ms * 1e-3) / 1e9
     spmtv_gbps = bytes_pass / (spmtv_ms * 1e-3) / 1e9
     log(f"spmv {spmv_ms * 1e3:.1f} us {spmv_gbps:.0f} GB/s; spmtv {spmtv_ms * 1e3:.1f} us {spmtv_gbps:.0f} GB/s")
-    dominant = ("k_scatter (A^T y)", spmtv_ms, spmtv_gbps) if spmtv_ms > spmv_ms else ("k_spmv (A x)", spmv_ms,
-                                                                                       spmv_gbps)
-    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # Each product kernel timed on its own (HIP events on the library's
+    # stream): the roofline is the slower of the A x gather and the A^T y
+    # window tiles, each charged the whole pass's algorithmic bytes (index
+    # stream, row vector, column vector; the tiles' partial slices and the
+    # reduction are overhead, not algorithm).
+    kt = A.kernel_times(args.reps)
+    log(f"kernel times (us): {kt}")
+    cands = [("k_gather_tile (A x)", kt["gather"]), ("k_scatter_tile (A^T y windows)", kt["scatter_tiles"])]
+    dom_name, dom_us = max(cands, key=lambda c: c[1])
+    dom_gbps = bytes_pass / (dom_us * 1e-6) / 1e9
+    # HBM bytes per launch of that kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes over `bench.py --traffic` (scaled by an
     # elementwise copy of 1 GiB; see profiles/traffic.json)
     traffic = None
@@ -475,7 +483,7 @@ def main():
     if os.path.exists(tpath):
         try:
             tj = json.load(open(tpath))
-            traffic = tj.get(dominant[0].split()[0], {}).get("total_bytes")
+            traffic = tj.get(dom_name.split()[0], {}).get("total_bytes")
         except Exception:
             traffic = None
 
@@ -505,11 +513,12 @@ def main():
         "speedup_vs_fp64": None if fp64 is None else fp64["seconds"] / t_mixed,
         "kernels": {"spmv_us": spmv_ms * 1e3, "spmv_GBps": spmv_gbps, "spmtv_us": spmtv_ms * 1e3,
                     "spmtv_GBps": spmtv_gbps, "bytes_per_pass": bytes_pass, "index_bytes": ib,
+                    "kernel_us": kt,
                     # the same pass in the reference's u32 CSR layout, over the same time
                     "spmv_equiv_u32_GBps": bytes_u32 / (spmv_ms * 1e-3) / 1e9,
                     "spmtv_equiv_u32_GBps": bytes_u32 / (spmtv_ms * 1e-3) / 1e9},
-        "roofline": {"kernel": dominant[0], "bound": "hbm", "achieved": dominant[2], "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": dominant[2] / HBM_PEAK_GBPS, "traffic": traffic},
+        "roofline": {"kernel": dom_name, "launch_us": dom_us, "bound": "hbm", "achieved": dom_gbps,
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": dom_gbps / HBM_PEAK_GBPS, "traffic": traffic},
         "schedule": tune_log.strip().splitlines()[-1] if tune_log else "default",
         "cpu_baseline": cpu,
         "reference_fit": reference_fit,

@@ -90,6 +90,7 @@ SIGNATURES = {
     "cad_matrix_from_positions_mf": (C.c_int, [vp, vp, i64, PP]),
     "cad_matrix_autotune": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
     "cad_matrix_get_schedule": (C.c_int, [vp, C.POINTER(Schedule)]),
+    "cad_matrix_kernel_times": (C.c_int, [vp, C.c_int, vp, C.c_int]),
     "cad_matrix_set_schedule": (C.c_int, [vp, C.POINTER(Schedule)]),
     "cad_matrix_create": (C.c_int, [vp, vp, i64, C.c_int, i64, PP]),
     "cad_matrix_destroy": (C.c_int, [vp]),

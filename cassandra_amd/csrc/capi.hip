@@ -470,6 +470,15 @@ int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size)
     });
 }
 
+int cad_matrix_kernel_times(cad_matrix* m, int reps, double* us, int n)
+{
+    return guarded([&] {
+        need(m != nullptr && us != nullptr && n >= 0, "null argument");
+        need(reps >= 1, "reps must be positive");
+        kernel_times(m, reps, us, n);
+    });
+}
+
 int cad_matrix_get_schedule(const cad_matrix* m, cad_schedule* out)
 {
     return guarded([&] {

@@ -99,42 +99,66 @@ void with_scatter_kernel(int rpt, F&& f)
     }
 }
 
+// A^T y in stages: the max |y| pre-pass of the fixed-point FP32 windows
+// (outside an LSQR iteration), the LDS window tiles into partial slices, then
+// the column reduction in chunk order (which writes every column of z).
+template <class VT, class View>
+constexpr bool fixed_point_scatter()
+{
+    return std::is_same_v<VT, float> &&
+           (std::is_same_v<View, SlotMajor<uint16_t>> || std::is_same_v<View, SlotMajor<uint32_t>>);
+}
+
+template <class YT>
+void launch_absmax(ScatterPlan& sp, const YT* y, const GridPlan& g, int nsys, hipStream_t s)
+{
+    if (g.rows.empty()) return;
+    hipLaunchKernelGGL(k_absmax_blocks<YT>, dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s,
+                       g.d_rows.as<RowWork>(), y, sp.d_ymax_part.as<unsigned>());
+    hipLaunchKernelGGL(k_absmax_systems, dim3(unsigned(nsys)), dim3(kRowBlock), 0, s, g.d_row_first.as<int32_t>(),
+                       g.d_row_count.as<int32_t>(), (const unsigned*)sp.d_ymax_part.as<unsigned>(),
+                       sp.d_ymax.as<unsigned>());
+    CAD_HIP(hipGetLastError());
+}
+
+template <class VT, class YT, class View>
+void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
+                          const LsqrScalars* div, int mode, hipStream_t s)
+{
+    ScatterLists& L = lists<VT>(sp);
+    if (L.work.empty()) return;
+    // the fixed-point windows read max |y| per system outside an LSQR
+    // iteration, whose terms are bounded by construction
+    const FixedScale fx{fixed_point_scatter<VT, View>() && mode != kScatterLsqr ? sp.d_ymax.as<unsigned>() : nullptr,
+                        L.cbits};
+    const size_t lds = size_t(L.widest) * sizeof(double);
+    with_scatter_kernel<VT, YT, View>(rpt, [&](auto kern) {
+        allow_lds(kern, lds);
+        hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
+                           L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(), sp.d_slot_list.as<uint16_t>(),
+                           sys_col, y, div, mode, fx, partials<VT>(L));
+    });
+    CAD_HIP(hipGetLastError());
+}
+
+template <class VT, class ZT>
+void launch_scatter_reduce(ScatterPlan& sp, const LsqrScalars* div, int mode, ZT* z, hipStream_t s)
+{
+    ScatterLists& L = lists<VT>(sp);
+    if (L.reduce.empty()) return;
+    hipLaunchKernelGGL((k_scatter_reduce<VT, ZT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
+                       L.d_reduce.as<ReduceWork>(), (const VT*)partials<VT>(L), z, div, mode);
+    CAD_HIP(hipGetLastError());
+}
+
 template <class VT, class YT, class ZT, class View>
 void launch_scatter(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
                     const LsqrScalars* div, int mode, ZT* z, const GridPlan& g, int nsys, hipStream_t s)
 {
-    ScatterLists& L = lists<VT>(sp);
-    VT* part = partials<VT>(L);
-    if (!L.work.empty()) {
-        // the FP32 stored-index scatter accumulates in fixed point: outside an
-        // LSQR iteration (whose terms are bounded) it first needs max |y| per
-        // system
-        FixedScale fx{nullptr, L.cbits};
-        constexpr bool kFixed = std::is_same_v<VT, float> && (std::is_same_v<View, SlotMajor<uint16_t>> ||
-                                                              std::is_same_v<View, SlotMajor<uint32_t>>);
-        if (kFixed && mode != kScatterLsqr && !g.rows.empty()) {
-            hipLaunchKernelGGL(k_absmax_blocks<YT>, dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s,
-                               g.d_rows.as<RowWork>(), y, sp.d_ymax_part.as<unsigned>());
-            hipLaunchKernelGGL(k_absmax_systems, dim3(unsigned(nsys)), dim3(kRowBlock), 0, s,
-                               g.d_row_first.as<int32_t>(), g.d_row_count.as<int32_t>(),
-                               (const unsigned*)sp.d_ymax_part.as<unsigned>(), sp.d_ymax.as<unsigned>());
-            CAD_HIP(hipGetLastError());
-            fx.ymax = sp.d_ymax.as<unsigned>();
-        }
-        const size_t lds = size_t(L.widest) * sizeof(double);
-        with_scatter_kernel<VT, YT, View>(rpt, [&](auto kern) {
-            allow_lds(kern, lds);
-            hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
-                               L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
-                               sp.d_slot_list.as<uint16_t>(), sys_col, y, div, mode, fx, part);
-        });
-        CAD_HIP(hipGetLastError());
-    }
-    if (!L.reduce.empty()) {
-        hipLaunchKernelGGL((k_scatter_reduce<VT, ZT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
-                           L.d_reduce.as<ReduceWork>(), (const VT*)part, z, div, mode);
-        CAD_HIP(hipGetLastError());
-    }
+    if (fixed_point_scatter<VT, View>() && mode != kScatterLsqr && !lists<VT>(sp).work.empty())
+        launch_absmax(sp, y, g, nsys, s);
+    launch_scatter_tiles<VT, YT>(sp, view, rpt, sys_col, y, div, mode, s);
+    launch_scatter_reduce<VT>(sp, div, mode, z, s);
 }
 
 template <class VT, class YT>
@@ -1030,6 +1054,33 @@ void set_schedule(cad_matrix* m, const TuneParams& tp)
     build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
                        m->mf ? 2 : m->ibytes, *m->ctx, tp);
     CAD_HIP(hipStreamSynchronize(m->ctx->stream));
+}
+
+void kernel_times(cad_matrix* m, int reps, double* us, int n)
+{
+    hipStream_t s = m->ctx->stream;
+    CAD_HIP(hipStreamSynchronize(s));
+    const int64_t R = std::max<int64_t>(1, m->rows), Cn = std::max<int64_t>(1, m->cols);
+    DBuf x4(4 * Cn), y4(4 * R), z4(4 * Cn);
+    fill<float>(x4.as<float>(), Cn, 1.0f, s);
+    fill<float>(y4.as<float>(), R, 1.0f, s);
+    EventTimer t;
+    double v[4] = {0, 0, 0, 0};
+    v[0] = t.us([&] { spmv<float>(m, x4.as<float>(), y4.as<float>(), s); }, s, reps);
+    with_view(m, [&](auto view) {
+        using View = decltype(view);
+        ScatterPlan& sp = m->splan;
+        partials<float>(lists<float>(sp));
+        const uint32_t* sc = m->d_sys_col.as<uint32_t>();
+        const float* y = y4.as<float>();
+        if (fixed_point_scatter<float, View>())
+            v[3] = t.us([&] { launch_absmax(sp, y, m->grid, m->nsys, s); }, s, reps);
+        v[1] = t.us([&] { launch_scatter_tiles<float, float>(sp, view, m->tune.srpt, sc, y, nullptr, kScatterAll, s); },
+                    s, reps);
+        v[2] = t.us([&] { launch_scatter_reduce<float>(sp, nullptr, kScatterAll, z4.as<float>(), s); }, s, reps);
+    });
+    CAD_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < std::min(n, 4); i++) us[i] = v[i];
 }
 
 std::string autotune(cad_matrix* m)

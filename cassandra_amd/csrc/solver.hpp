@@ -43,6 +43,10 @@ void matrix_targets(cad_matrix* m, const double* xt, uint64_t seed, double sn, d
 // Times the candidate schedules of the FP32 products on this matrix and
 // keeps the fastest (rebuilds the plans; drops captured solver graphs).
 std::string autotune(cad_matrix* m);
+// Average launch times (us) of the FP32 product kernels, HIP events on the
+// context's stream: A x, the A^T y window tiles, their reduction, the max |y|
+// pre-pass.
+void kernel_times(cad_matrix* m, int reps, double* us, int n);
 // Sets the schedule (validated; drops the captured graphs, rebuilds the plans).
 void set_schedule(cad_matrix* m, const TuneParams& tp);
 

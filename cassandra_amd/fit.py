@@ -211,6 +211,13 @@ class Matrix:
         call("cad_matrix_autotune", self.h, buf, len(buf))
         return buf.value.decode()
 
+    def kernel_times(self, reps: int = 20) -> dict:
+        """Average launch times (us) of the FP32 product kernels (HIP events on
+        the library's stream): gather, scatter tiles, scatter reduce, absmax."""
+        out = (C.c_double * 4)()
+        call("cad_matrix_kernel_times", self.h, int(reps), C.cast(out, C.c_void_p), 4)
+        return dict(zip(("gather", "scatter_tiles", "scatter_reduce", "absmax"), list(out)))
+
     def schedule(self) -> dict:
         """The product schedule in use (cad_schedule fields)."""
         sc = capi.Schedule()

@@ -132,6 +132,13 @@ int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size);
  * Every schedule computes the same products (see cad_matrix_autotune);
  * CAD_ERR_ARGUMENT for a value outside these or a tile gather the matrix
  * cannot use. */
+/* No counterpart in the reference.  Average launch time in microseconds of
+ * each FP32 product kernel of this matrix with its current schedule, timed
+ * with HIP events on the context's stream over reps launches after a warm
+ * one (inputs: all ones): us[0] the A x gather, us[1] the A^T y window tiles,
+ * us[2] their column reduction, us[3] the max |y| pre-pass of the fixed-point
+ * windows (0 where not used).  The first n of them are written. */
+int cad_matrix_kernel_times(cad_matrix* m, int reps, double* us, int n);
 typedef struct cad_schedule {
     int gather_f32_rpt, gather_f32_block;
     int gather_f64_rpt, gather_f64_block;
