@@ -392,6 +392,8 @@ def main():
                     help="only launch the measured kernels a few times and a 1 GiB device copy (for counter runs)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default product schedules instead of timing the candidates")
+    ap.add_argument("--schedule", default="",
+                    help="pin product schedule fields after the autotuner, e.g. scatter_order=1,scatter_rpt=4")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -417,6 +419,14 @@ def main():
     t = time.perf_counter()
     tune_log = A.autotune() if args.autotune else ""
     log(f"autotune ({time.perf_counter() - t:.2f} s):\n{tune_log}")
+    if args.schedule:
+        knobs = {}
+        for kv in args.schedule.split(","):
+            key, val = kv.split("=")
+            knobs[key.strip()] = float(val) if "." in val else int(val)
+        A.set_schedule(**knobs)
+        tune_log = (tune_log or "") + f"pinned: {A.schedule()}\n"
+        log(f"schedule pinned: {A.schedule()}")
     if args.traffic:
         traffic_launches(ctx, A)
         ctx.close()

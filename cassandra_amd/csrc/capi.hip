@@ -147,8 +147,8 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s,
-                           RowMajor32{idx, p->group->k()}, L.d_work.as<ScatterWork>(), sp.d_groups.as<SlotGroup>(),
-                           sp.d_slot_list.as<uint16_t>(), p->d_syscol0.as<uint32_t>(), g,
+                           RowMajor32{idx, p->group->k()}, L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(),
+                           L.d_slot_list.as<uint16_t>(), p->d_syscol0.as<uint32_t>(), g,
                            (const LsqrScalars*)nullptr, kScatterAll, FixedScale{nullptr, 0}, part);
         CAD_HIP(hipGetLastError());
     }

@@ -104,6 +104,32 @@ void make_groups(const std::vector<uint32_t>& lo, const std::vector<uint32_t>& h
     }
 }
 
+// The scatter groups of a window size (ScatterLists::groups).  Pattern
+// groups come in column order and never overlap.
+void pack_groups(const ScatterPlan& sp, uint32_t wmax, ScatterLists& L)
+{
+    L.groups.clear();
+    L.slot_list.clear();
+    size_t i = 0;
+    while (i < sp.groups.size()) {
+        SlotGroup g = sp.groups[i];
+        std::vector<uint16_t> slots(sp.slot_list.begin() + g.first, sp.slot_list.begin() + g.first + g.nslots);
+        size_t j = i + 1;
+        if (g.col_hi - g.col_lo <= wmax)
+            for (; j < sp.groups.size(); j++) {
+                const SlotGroup& h = sp.groups[j];
+                if (h.col_lo < g.col_hi || h.col_hi - g.col_lo > wmax || g.nslots + h.nslots > kMaxSegSlots) break;
+                slots.insert(slots.end(), sp.slot_list.begin() + h.first, sp.slot_list.begin() + h.first + h.nslots);
+                g.col_hi = h.col_hi;
+                g.nslots = uint16_t(g.nslots + h.nslots);
+            }
+        g.first = uint16_t(L.slot_list.size());
+        L.slot_list.insert(L.slot_list.end(), slots.begin(), slots.end());
+        L.groups.push_back(g);
+        i = j;
+    }
+}
+
 // Reduce items of one window ([wl, wh) of system s, chunk slices at off with
 // stride W) and of the columns outside every window (written as zeros).
 void push_reduce(ScatterLists& L, int s, uint32_t sys_col, uint32_t wl, uint32_t wh, int64_t off, int64_t C)
@@ -139,8 +165,8 @@ void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& s
     struct Win { uint32_t lo, hi; int group; int nslots; };
     std::vector<Win> wins;
     double work_per_row = 0, width = 0;
-    for (size_t gi = 0; gi < sp.groups.size(); gi++) {
-        const SlotGroup& g = sp.groups[gi];
+    for (size_t gi = 0; gi < L.groups.size(); gi++) {
+        const SlotGroup& g = L.groups[gi];
         for (uint32_t wl = g.col_lo; wl < g.col_hi; wl += wmax) {
             const uint32_t wh = std::min(g.col_hi, wl + wmax);
             wins.push_back(Win{wl, wh, int(gi), g.nslots});
@@ -224,10 +250,11 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
     L.work.clear();
     L.reduce.clear();
     L.widest = 0;
+    pack_groups(sp, wmax, L);
     const int nsys = int(sys_col.size());
     double total = 0;
     for (int s = 0; s < nsys; s++)
-        for (auto& g : sp.groups) {
+        for (auto& g : L.groups) {
             const int64_t R = sys_rows[s + 1] - sys_rows[s];
             const uint32_t nwin = (g.col_hi - g.col_lo + wmax - 1) / wmax;
             total += double(R) * g.nslots * nwin;
@@ -241,8 +268,8 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
     for (int s = 0; s < nsys; s++) {
         const int64_t R = sys_rows[s + 1] - sys_rows[s];
         std::vector<std::pair<uint32_t, uint32_t>> covered;
-        for (size_t gi = 0; gi < sp.groups.size(); gi++) {
-            const SlotGroup& g = sp.groups[gi];
+        for (size_t gi = 0; gi < L.groups.size(); gi++) {
+            const SlotGroup& g = L.groups[gi];
             for (uint32_t wl = g.col_lo; wl < g.col_hi; wl += wmax) {
                 const uint32_t wh = std::min(g.col_hi, wl + wmax);
                 const uint32_t W = wh - wl;
@@ -334,7 +361,7 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
     for (ScatterLists* L : {&sp.f4, &sp.f8}) {
         L->cbits = 0;
         for (const ScatterWork& w : L->work) {
-            uint64_t n = uint64_t(std::max<int64_t>(0, w.row_end - w.row_begin)) * sp.groups[size_t(w.group)].nslots;
+            uint64_t n = uint64_t(std::max<int64_t>(0, w.row_end - w.row_begin)) * L->groups[size_t(w.group)].nslots;
             int b = 0;
             while (n) b++, n >>= 1;
             L->cbits = std::max(L->cbits, b);
@@ -343,6 +370,8 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
     sp.d_groups = to_device(sp.groups, ctx.stream);
     sp.d_slot_list = to_device(sp.slot_list, ctx.stream);
     for (ScatterLists* L : {&sp.f4, &sp.f8}) {
+        L->d_groups = to_device(L->groups, ctx.stream);
+        L->d_slot_list = to_device(L->slot_list, ctx.stream);
         L->d_work = to_device(L->work, ctx.stream);
         L->d_reduce = to_device(L->reduce, ctx.stream);
         L->part = DBuf();

@@ -50,6 +50,12 @@ DBuf to_device(const std::vector<T>& v, hipStream_t s)
 // The two-stage scatter of one value size: tiles (window x row chunk) that
 // write partial slices, and the column reduction over the slices.
 struct ScatterLists {
+    // the slot groups of these lists: the pattern groups, with consecutive
+    // groups narrower than a window merged while they fit one window together
+    // (at most kMaxSegSlots slots), so one pass over a row chunk serves them
+    std::vector<SlotGroup> groups;
+    std::vector<uint16_t> slot_list;
+    DBuf d_groups, d_slot_list;
     std::vector<ScatterWork> work;
     std::vector<ReduceWork> reduce;
     DBuf d_work, d_reduce;
