@@ -180,6 +180,28 @@ struct EpiStore {
     static constexpr bool kReduce = false;
     __device__ bool begin(int) { return true; }
     __device__ double row(int64_t r, VT a) { y[r] = a; return 0.0; }
+    // RPT consecutive rows from a (a multiple of RPT), all of them valid:
+    // 16-byte stores where y + a is 16-byte aligned
+    template <int RPT>
+    __device__ double rows(int64_t a, const VT (&acc)[RPT])
+    {
+        constexpr int Q = 16 / sizeof(VT);
+        if constexpr (RPT % Q == 0) {
+            if ((reinterpret_cast<uintptr_t>(y + a) & 15) == 0) {
+                using V4 = std::conditional_t<sizeof(VT) == 4, float4, double2>;
+#pragma unroll
+                for (int i = 0; i < RPT; i += Q) {
+                    V4 v;
+                    __builtin_memcpy(&v, &acc[i], 16);
+                    *reinterpret_cast<V4*>(y + a + i) = v;
+                }
+                return 0.0;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RPT; i++) y[a + i] = acc[i];
+        return 0.0;
+    }
     __device__ void finish(int, double) {}
 };
 
@@ -206,6 +228,50 @@ struct EpiLsqrU {
         u[r] = v;
         return double(v) * double(v);
     }
+    // RPT consecutive valid rows from a: 16-byte loads and stores of u where
+    // aligned; the squares add up in row order, as row() would
+    template <int RPT>
+    __device__ double rows(int64_t a, const VT (&acc)[RPT])
+    {
+        VT o[RPT];
+        constexpr int Q = 16 / sizeof(VT);
+        bool vec = false;
+        if constexpr (RPT % Q == 0) vec = (reinterpret_cast<uintptr_t>(u + a) & 15) == 0;
+        using V4 = std::conditional_t<sizeof(VT) == 4, float4, double2>;
+        if constexpr (RPT % Q == 0) {
+            if (vec) {
+#pragma unroll
+                for (int i = 0; i < RPT; i += Q) {
+                    const V4 t = *reinterpret_cast<const V4*>(u + a + i);
+                    __builtin_memcpy(&o[i], &t, 16);
+                }
+            }
+        }
+        if (!vec) {
+#pragma unroll
+            for (int i = 0; i < RPT; i++) o[i] = u[a + i];
+        }
+        double red = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            o[i] = acc[i] - f * o[i];
+            red += double(o[i]) * double(o[i]);
+        }
+        if constexpr (RPT % Q == 0) {
+            if (vec) {
+#pragma unroll
+                for (int i = 0; i < RPT; i += Q) {
+                    V4 t;
+                    __builtin_memcpy(&t, &o[i], 16);
+                    *reinterpret_cast<V4*>(u + a + i) = t;
+                }
+                return red;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RPT; i++) u[a + i] = o[i];
+        return red;
+    }
     __device__ void finish(int sys, double s)
     {
         scal[sys].beta = sqrt(s);
@@ -231,6 +297,14 @@ struct EpiResidual {
         if (u) u[r] = VT(v);
         return v * v;
     }
+    template <int RPT>
+    __device__ double rows(int64_t a, const double (&acc)[RPT])
+    {
+        double red = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) red += row(a + i, acc[i]);
+        return red;
+    }
     __device__ void finish(int sys, double s)
     {
         scal[sys].reduce_a = s;
@@ -253,6 +327,14 @@ struct EpiTargets {
         const double e = 2.0 * floor(v * 0.5 + 0.5);
         b[r] = fmin(64.0, fmax(-64.0, e));
         return 0.0;
+    }
+    template <int RPT>
+    __device__ double rows(int64_t a, const double (&acc)[RPT])
+    {
+        double red = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) red += row(a + i, acc[i]);
+        return red;
     }
     __device__ void finish(int, double) {}
 };
@@ -567,10 +649,19 @@ __global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const Row
             }
         st = nx;
     }
-    double red = 0.0;
+    // all RPT rows valid (every thread but the tile's first and last): the
+    // epilogue's vector form; else row by row
+    bool all = true;
 #pragma unroll
-    for (int i = 0; i < RPT; i++)
-        if (ok[i]) red += epi.row(a + i, acc[i]);
+    for (int i = 0; i < RPT; i++) all = all && ok[i];
+    double red = 0.0;
+    if (all) {
+        red = epi.template rows<RPT>(a, acc);
+    } else {
+#pragma unroll
+        for (int i = 0; i < RPT; i++)
+            if (ok[i]) red += epi.row(a + i, acc[i]);
+    }
     reduce_tail<BS>(epi, red, w.sys, g);
 }
 
