@@ -36,8 +36,17 @@ def test_autotune_keeps_results(ctx, oracle, free):
     assert torch.equal(A.spmtv(g), z0)
     x1, _ = A.solve(b, precision=capi.PREC_FP64, tol=0.0, max_iterations=10, check_every=5)
     po, p1 = oracle.spmv(idx, x0.cpu().numpy()), oracle.spmv(idx, x1.cpu().numpy())
-    # FP64 scatter order may change with the chunking: agreement to rounding
-    assert np.linalg.norm(p1 - po) <= 1e-9 * np.linalg.norm(po)
+    # the FP64 A^T y adds its LDS windows in the atomics' arrival order, which
+    # the chunking changes; ten LSQR steps amplify those ulp-level differences
+    # to 1e-9 .. 1e-8 of the predictions on this matrix (measured), so both
+    # iterates are held to 1e-7, and to the oracle's
+    assert np.linalg.norm(p1 - po) <= 1e-7 * np.linalg.norm(po)
+    D = oracle.jacobi(idx, A.cols).astype(np.float64)
+    okd = np.isfinite(D) & (D > 0)
+    D = np.where(okd, 1.0 / np.sqrt(np.rint(1.0 / np.where(okd, D, 1.0))), 1.0)
+    xo, _, _ = oracle.lsqr(idx, A.cols, b.cpu().numpy(), 10, colscale=D)
+    pr = oracle.spmv(idx, xo)
+    assert np.linalg.norm(p1 - pr) <= 1e-7 * np.linalg.norm(pr)
     xm, rep = A.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
     assert rep["converged"] == 1
     assert oracle.rel_normal_residual(idx, A.cols, b.cpu().numpy(), xm.cpu().numpy())[0] <= 1.5e-10
