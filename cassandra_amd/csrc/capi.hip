@@ -484,7 +484,8 @@ int cad_matrix_get_schedule(const cad_matrix* m, cad_schedule* out)
     return guarded([&] {
         need(m != nullptr && out != nullptr, "null argument");
         const TuneParams& t = m->tune;
-        *out = cad_schedule{t.rpt4, t.bs4, t.rpt8, t.bs8, t.srpt, t.scatter_order, t.blocks_per_cu, t.partial_cap};
+        *out = cad_schedule{t.rpt4, t.bs4, t.sub4, t.rpt8, t.bs8, t.sub8, t.srpt, t.scatter_order, t.blocks_per_cu,
+                            t.partial_cap};
     });
 }
 
@@ -495,8 +496,10 @@ int cad_matrix_set_schedule(cad_matrix* m, const cad_schedule* sc)
         TuneParams t;
         t.rpt4 = sc->gather_f32_rpt;
         t.bs4 = sc->gather_f32_block;
+        t.sub4 = sc->gather_f32_sub;
         t.rpt8 = sc->gather_f64_rpt;
         t.bs8 = sc->gather_f64_block;
+        t.sub8 = sc->gather_f64_sub;
         t.srpt = sc->scatter_rpt;
         t.scatter_order = sc->scatter_order;
         t.blocks_per_cu = sc->scatter_tiles_per_cu;

@@ -535,12 +535,13 @@ __device__ __forceinline__ uint32_t line_head(const VT* p)
     return uint32_t(reinterpret_cast<uintptr_t>(p) / sizeof(VT)) & (16 / sizeof(VT) - 1);
 }
 
+// One tile per block (no sub-tiles): the body of k_gather_tile<..., 1>.
 template <class VT, int RPT, class IdxT, class Epi, int BS>
-__global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const RowWork* __restrict__ work,
-                                                            const uint32_t* __restrict__ sys_col,
-                                                            const GatherStep* __restrict__ steps, int nsteps,
-                                                            const PhaseSeg* __restrict__ phases, uint32_t wmax,
-                                                            const VT* __restrict__ x, Epi epi, GridSys g)
+__device__ __forceinline__ void gather_tile_single(const SlotMajor<IdxT>& m, const RowWork* __restrict__ work,
+                                                   const uint32_t* __restrict__ sys_col,
+                                                   const GatherStep* __restrict__ steps, int nsteps,
+                                                   const PhaseSeg* __restrict__ phases, uint32_t wmax,
+                                                   const VT* __restrict__ x, Epi& epi, const GridSys& g)
 {
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
@@ -663,6 +664,183 @@ __global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const Row
             if (ok[i]) red += epi.row(a + i, acc[i]);
     }
     reduce_tail<BS>(epi, red, w.sys, g);
+}
+
+// S sub-tiles per block: the body of k_gather_tile<..., S > 1>.
+template <class VT, int RPT, class IdxT, class Epi, int BS, int S>
+__device__ __forceinline__ void gather_tile_sub(const SlotMajor<IdxT>& m, const RowWork* __restrict__ work,
+                                                const uint32_t* __restrict__ sys_col,
+                                                const GatherStep* __restrict__ steps, int nsteps,
+                                                const PhaseSeg* __restrict__ phases, uint32_t wmax,
+                                                const VT* __restrict__ x, Epi& epi, const GridSys& g)
+{
+    using V = IdxVec<IdxT, RPT>;
+    using VecT = typename V::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    VT* xs = reinterpret_cast<VT*>(smem_raw);
+    const RowWork w = work[blockIdx.x];
+    if (!epi.begin(w.sys)) return;
+    const VT* __restrict__ xb = x + sys_col[w.sys];
+    // S sub-tiles of BS * RPT rows: in sub-tile u the thread owns rows
+    // a0 + u * kSub + i, i < RPT; every staging serves all of them
+    constexpr int64_t kSub = int64_t(RPT) * BS;
+    const int64_t a0 = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x;
+    bool okr[S][RPT];  // that row is in the tile
+    VT acc[S][RPT];
+#pragma unroll
+    for (int u = 0; u < S; u++)
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const int64_t r = a0 + u * kSub + i;
+            okr[u][i] = r >= w.row_begin && r < w.row_end;
+            acc[u][i] = VT(0);
+        }
+    auto ok = [&](int u, int i) { return okr[u][i]; };
+    // slot s of sub-tile u's rows: vec_of(u)[s * vld] (rows past the tile read row 0)
+    const int64_t vld = m.ld / RPT;
+    auto vec_of = [&](int u) {
+        const int64_t au = a0 + u * kSub;
+        return reinterpret_cast<const VecT*>(m.idx + (au < w.row_end ? au : 0));
+    };
+    // Where the registers allow it (up to 8 bytes of entries per slot) the
+    // entries of the next (step, sub-tile) are loaded while this one sums;
+    // otherwise each sub-tile loads its own (the first ahead of the staging).
+    constexpr bool kPrefetch = sizeof(VecT) <= 8;
+    VecT q[kMaxSegSlots], qn[kPrefetch ? kMaxSegSlots : 1];
+    GatherStep st = nsteps > 0 ? step_at(steps, 0) : GatherStep{};
+    if (kPrefetch && nsteps > 0) load_step<IdxT, RPT, true>(q, vec_of(0), vld, st);
+    for (int k = 0; k < nsteps; k++) {
+        const bool more = k + 1 < nsteps;
+        const GatherStep nx = step_at(steps, more ? k + 1 : k);
+        // the pair after (k, u) is (k, u + 1), or (k + 1, 0)
+        auto prefetch_after = [&](int u) {
+            if constexpr (kPrefetch) {
+                if (u + 1 < S)
+                    load_step<IdxT, RPT, true>(qn, vec_of(u + 1), vld, st);
+                else if (more)
+                    load_step<IdxT, RPT, true>(qn, vec_of(0), vld, nx);
+            }
+        };
+        auto advance = [&](int u) {
+            if constexpr (kPrefetch)
+                if (u + 1 < S || more) {
+#pragma unroll
+                    for (int j = 0; j < kMaxSegSlots; j++) q[j] = qn[j];
+                }
+        };
+        if (st.nwin > 1) {
+            // a segment wider than the LDS, at most kMaxSegSlots slots: each
+            // sub-tile walks its windows
+#pragma unroll
+            for (int u = 0; u < S; u++) {
+                if constexpr (!kPrefetch) load_step<IdxT, RPT, false>(q, vec_of(u), vld, st);
+                VT v[RPT][kMaxSegSlots];
+#pragma unroll
+                for (int j = 0; j < kMaxSegSlots; j++)
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) v[i][j] = VT(0);
+                for (uint32_t wl = 0; wl < st.width; wl += wmax) {
+                    const uint32_t W = min(wmax, st.width - wl);
+                    // the next entries load with the last window's staging
+                    if (wl + W >= st.width) prefetch_after(u);
+                    __syncthreads();
+                    const uint32_t h = stage_direct<VT, BS>(xs, xb + st.col_lo + wl, W);
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < kMaxSegSlots; j++) {
+                        if (j < st.nslots) {
+                            // 32-bit offset: base - col_lo wraps for u32 storage
+                            // (base 0) and the sum with the entry unwraps it
+                            const uint32_t off = st.off[j] - wl;
+#pragma unroll
+                            for (int i = 0; i < RPT; i++) {
+                                const uint32_t c = off + V::get(q[j], i);
+                                const bool in = ok(u, i) && c < W;
+                                const VT t = xs[h + (in ? c : 0u)];
+                                v[i][j] = in ? t : v[i][j];
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kMaxSegSlots; j++)
+                    if (j < st.nslots) {
+#pragma unroll
+                        for (int i = 0; i < RPT; i++) acc[u][i] += v[i][j];
+                    }
+                advance(u);
+            }
+        } else {
+            if constexpr (!kPrefetch) load_step<IdxT, RPT, false>(q, vec_of(0), vld, st);
+            if (st.phase_count > 0) {
+                __syncthreads();
+                for (int p = st.phase_first; p < st.phase_first + st.phase_count; p++) {
+                    const PhaseSeg ps = phases[p];
+                    stage_direct<VT, BS>(xs + ps.lds_off, xb + ps.col_lo, ps.width);
+                }
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+            const uint32_t h = line_head(xb + st.col_lo);
+#pragma unroll
+            for (int u = 0; u < S; u++) {
+                if constexpr (!kPrefetch) {
+                    if (u > 0) load_step<IdxT, RPT, false>(q, vec_of(u), vld, st);
+                } else if (u > 0 || st.phase_count == 0) {
+                    // this pair's entries (loaded during the last pair's sums)
+                    // must be in before the next pair's loads go out: the
+                    // compiler's own wait for them would be a vmcnt(0) after
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
+                prefetch_after(u);
+#pragma unroll
+                for (int j = 0; j < kMaxSegSlots; j++) {
+                    if (j < st.nslots) {
+                        const uint32_t off = st.off[j] + h;
+#pragma unroll
+                        for (int i = 0; i < RPT; i++) {
+                            const VT t = xs[ok(u, i) ? off + V::get(q[j], i) : 0u];
+                            acc[u][i] += ok(u, i) ? t : VT(0);
+                        }
+                    }
+                }
+                advance(u);
+            }
+        }
+        st = nx;
+    }
+    // a sub-tile's RPT rows all in the tile (every thread but the tile's first
+    // and last): the epilogue's vector form; else row by row
+    double red = 0.0;
+#pragma unroll
+    for (int u = 0; u < S; u++) {
+        const int64_t au = a0 + u * kSub;
+        bool all = true;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) all = all && okr[u][i];
+        if (all) {
+            red += epi.template rows<RPT>(au, acc[u]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+                if (ok(u, i)) red += epi.row(au + i, acc[u][i]);
+        }
+    }
+    reduce_tail<BS>(epi, red, w.sys, g);
+}
+
+template <class VT, int RPT, class IdxT, class Epi, int BS, int S>
+__global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const RowWork* __restrict__ work,
+                                                    const uint32_t* __restrict__ sys_col,
+                                                    const GatherStep* __restrict__ steps, int nsteps,
+                                                    const PhaseSeg* __restrict__ phases, uint32_t wmax,
+                                                    const VT* __restrict__ x, Epi epi, GridSys g)
+{
+    if constexpr (S == 1)
+        gather_tile_single<VT, RPT, IdxT, Epi, BS>(m, work, sys_col, steps, nsteps, phases, wmax, x, epi, g);
+    else
+        gather_tile_sub<VT, RPT, IdxT, Epi, BS, S>(m, work, sys_col, steps, nsteps, phases, wmax, x, epi, g);
 }
 
 // The LDS-staged gather of a matrix-free pattern matrix: as k_gather_tile,

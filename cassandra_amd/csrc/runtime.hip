@@ -382,15 +382,17 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
 // slots that follow each other in slot order (true for pattern matrices, whose
 // groups are the patterns); groups wider than the LDS may hold at most
 // kMaxSegSlots slots.
-void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int bs)
+void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int bs, int sub)
 {
     const Context& ctx = *m->ctx;
     const ScatterPlan& sp = m->splan;
     gp.ok = false;
     gp.rpt = rpt;
     gp.bs = bs;
+    gp.sub = sub;
     if (rpt <= 0) return;  // the row kernel
     if (bs != kTileBlock && (bs != kTileBlock / 2 || m->mf)) return;
+    if (sub < 1 || sub > 3 || (sub > 1 && (bs != kTileBlock || m->mf))) return;
     const int per_cu = kTileBlock / bs;  // blocks resident per CU
     // A staged run of columns starts at its element offset inside a 16-byte
     // line (up to Q - 1 elements) so that whole lines map to whole LDS lines:
@@ -470,11 +472,11 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int b
         }
     }
     // Row tiles: a thread owns rpt consecutive rows counted from the tile's
-    // first row rounded down to rpt, so a tile spans at most bs * rpt
+    // first row rounded down to rpt, so a tile spans at most sub * bs * rpt
     // rows from there; tiles of a system are a multiple of rpt rows long (their
     // misalignment is the system's).  A single system is cut into a multiple of
     // the CU count so that every CU gets the same number of tiles.
-    const int64_t cap = int64_t(bs) * rpt - (rpt - 1);
+    const int64_t cap = int64_t(sub) * bs * rpt - (rpt - 1);
     gp.tiles.clear();
     gp.first.assign(m->nsys, 0);
     gp.count.assign(m->nsys, 0);
@@ -514,8 +516,8 @@ void build_plans(cad_matrix* m)
     // (a matrix-free matrix sizes its scatter partials as a u16 matrix would)
     build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
                        m->mf ? 2 : m->ibytes, ctx, m->tune);
-    build_gather_plan(m, m->gplan4, 4, m->tune.rpt4, m->tune.bs4);
-    build_gather_plan(m, m->gplan8, 8, m->tune.rpt8, m->tune.bs8);
+    build_gather_plan(m, m->gplan4, 4, m->tune.rpt4, m->tune.bs4, m->tune.sub4);
+    build_gather_plan(m, m->gplan8, 8, m->tune.rpt8, m->tune.bs8, m->tune.sub8);
 
     GridPlan& g = m->grid;
     g.rows.clear();

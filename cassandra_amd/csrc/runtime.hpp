@@ -80,6 +80,7 @@ struct GatherPlan {
     bool ok = false;  // the matrix has the pattern structure the tile kernel needs
     int rpt = 0;      // rows per thread
     int bs = 1024;    // threads per block (1024, or 512: two blocks and half the LDS each per CU)
+    int sub = 1;      // sub-tiles of bs * rpt rows that share each staging
     uint32_t wmax = 0;
     uint32_t lds_elems = 0;
     std::vector<GatherSeg> segs;     // matrix-free kernel: one staging per segment
@@ -109,6 +110,7 @@ struct TuneParams {
     int rpt4 = 4;               // rows per thread of the FP32 tile gather (0: row kernel)
     int rpt8 = 2;               // rows per thread of the FP64 tile gather (0: row kernel)
     int bs4 = 1024, bs8 = 1024; // threads per block of the FP32 / FP64 tile gather (1024 or 512)
+    int sub4 = 1, sub8 = 1;     // sub-tiles per staging of the FP32 / FP64 tile gather (1..3; 1 with 512)
     int srpt = 4;               // consecutive rows per step of the stored-index scatter (4, or 8 for u16)
     double blocks_per_cu = 1.0; // scatter tiles per CU the row chunks aim at
     double partial_cap = 1.0;   // partial-slice traffic cap as a fraction of index traffic
@@ -175,7 +177,7 @@ struct cad_matrix {
 
 namespace cad {
 void build_plans(cad_matrix* m);
-void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int bs);
+void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int bs, int sub);
 void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
                         const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
                         int64_t cols_per_sys, int ibytes, const Context& ctx, const TuneParams& tune);

@@ -219,14 +219,33 @@ void with_rpt(int rpt, F&& f)
     throw Error(CAD_ERR_ARGUMENT, "unsupported rows per thread of the tile gather");
 }
 
-// Threads per block of the stored-index tile gather (a schedule knob):
-// kTileBlock, or half of it with two blocks per CU.
-template <class F>
-void with_tile_block(int bs, F&& f)
+// Shape of the stored-index tile gather (schedule knobs): threads per block
+// (kTileBlock, or half of it with two blocks per CU) and sub-tiles per staging
+// (2 or 3 with full blocks, for the variants whose per-thread rows fit 8 bytes
+// of entries: the others would spill); f receives both as compile-time
+// constants.
+template <bool kSubTiles, class F>
+void with_tile_shape(int bs, int sub, F&& f)
 {
-    if (bs == kTileBlock) return f(std::integral_constant<int, kTileBlock>{});
-    if (bs == kTileBlock / 2) return f(std::integral_constant<int, kTileBlock / 2>{});
-    throw Error(CAD_ERR_ARGUMENT, "unsupported block size of the tile gather");
+    using std::integral_constant;
+    if (bs == kTileBlock) {
+        if (sub == 1) return f(integral_constant<int, kTileBlock>{}, integral_constant<int, 1>{});
+        if constexpr (kSubTiles) {
+            if (sub == 2) return f(integral_constant<int, kTileBlock>{}, integral_constant<int, 2>{});
+            if (sub == 3) return f(integral_constant<int, kTileBlock>{}, integral_constant<int, 3>{});
+        }
+    } else if (bs == kTileBlock / 2 && sub == 1) {
+        return f(integral_constant<int, kTileBlock / 2>{}, integral_constant<int, 1>{});
+    }
+    throw Error(CAD_ERR_ARGUMENT, "unsupported block size / sub-tiles of the tile gather");
+}
+
+// the tile gathers that take sub-tiles: FP32 with 2 rows per thread, FP64
+// with 1, and at most 4 bytes of entries per slot and thread
+template <class VT, class IdxT>
+constexpr bool sub_tiles_ok(int rpt)
+{
+    return rpt * int(sizeof(VT)) == 8 && rpt * int(sizeof(IdxT)) <= 4;
 }
 
 // Row gather with an epilogue: the LDS-staged tile kernel for pattern
@@ -250,9 +269,9 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
                                        gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
                                        gp.d_segs.as<GatherSeg>(), int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
                 } else {
-                    with_tile_block(gp.bs, [&](auto bc) {
+                    with_tile_shape<sub_tiles_ok<VT, IdxT>(RPT)>(gp.bs, gp.sub, [&](auto bc, auto sc) {
                         constexpr int BS = decltype(bc)::value;
-                        auto kern = k_gather_tile<VT, RPT, IdxT, Epi, BS>;
+                        auto kern = k_gather_tile<VT, RPT, IdxT, Epi, BS, decltype(sc)::value>;
                         allow_lds(kern, lds);
                         hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(BS), lds, s, view,
                                            gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
@@ -461,8 +480,8 @@ static void prepare_gather(cad_matrix* m)
             if constexpr (std::is_void_v<IdxT>)
                 allow_lds(k_gather_tile_mf<VT, RPT, Epi>, size_t(gp.lds_elems) * sizeof(VT));
             else
-                with_tile_block(gp.bs, [&](auto bc) {
-                    allow_lds(k_gather_tile<VT, RPT, IdxT, Epi, decltype(bc)::value>,
+                with_tile_shape<sub_tiles_ok<VT, IdxT>(RPT)>(gp.bs, gp.sub, [&](auto bc, auto sc) {
+                    allow_lds(k_gather_tile<VT, RPT, IdxT, Epi, decltype(bc)::value, decltype(sc)::value>,
                               size_t(gp.lds_elems) * sizeof(VT));
                 });
         });
@@ -1036,6 +1055,14 @@ void set_schedule(cad_matrix* m, const TuneParams& tp)
     for (int b : {tp.bs4, tp.bs8})
         if (!(b == kTileBlock || (b == kTileBlock / 2 && !m->mf)))
             throw Error(CAD_ERR_ARGUMENT, "unsupported block size of the tile gather");
+    // sub-tiles: 1, or 2..3 for the full-block stored-index variants that take
+    // them (rpt * value bytes == 8, rpt * index bytes <= 4)
+    auto sub_ok = [&](int bs, int sub, int rpt, int vbytes) {
+        return sub == 1 || (sub >= 2 && sub <= 3 && bs == kTileBlock && !m->mf && rpt * vbytes == 8 &&
+                            rpt * m->ibytes <= 4);
+    };
+    if (!sub_ok(tp.bs4, tp.sub4, tp.rpt4, 4) || !sub_ok(tp.bs8, tp.sub8, tp.rpt8, 8))
+        throw Error(CAD_ERR_ARGUMENT, "unsupported sub-tiles of the tile gather");
     if (!(u16 ? one_of(tp.srpt, {4, 8}) : tp.srpt == 4) || !one_of(tp.scatter_order, {0, 1}) ||
         !(tp.blocks_per_cu > 0 && tp.blocks_per_cu <= 64) || !(tp.partial_cap > 0 && tp.partial_cap <= 1))
         throw Error(CAD_ERR_ARGUMENT, "unsupported scatter schedule");
@@ -1043,12 +1070,12 @@ void set_schedule(cad_matrix* m, const TuneParams& tp)
     drop_graphs(m);
     const TuneParams keep = m->tune;
     m->tune = tp;
-    build_gather_plan(m, m->gplan4, 4, tp.rpt4, tp.bs4);
-    build_gather_plan(m, m->gplan8, 8, tp.rpt8, tp.bs8);
+    build_gather_plan(m, m->gplan4, 4, tp.rpt4, tp.bs4, tp.sub4);
+    build_gather_plan(m, m->gplan8, 8, tp.rpt8, tp.bs8, tp.sub8);
     if ((tp.rpt4 > 0 && !m->gplan4.ok) || (tp.rpt8 > 0 && !m->gplan8.ok)) {
         m->tune = keep;
-        build_gather_plan(m, m->gplan4, 4, keep.rpt4, keep.bs4);
-        build_gather_plan(m, m->gplan8, 8, keep.rpt8, keep.bs8);
+        build_gather_plan(m, m->gplan4, 4, keep.rpt4, keep.bs4, keep.sub4);
+        build_gather_plan(m, m->gplan8, 8, keep.rpt8, keep.bs8, keep.sub8);
         throw Error(CAD_ERR_ARGUMENT, "this matrix cannot use the tile gather");
     }
     build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
@@ -1098,29 +1125,41 @@ std::string autotune(cad_matrix* m)
     char line[160];
     TuneParams& tp = m->tune;
     // gathers: tile rows per thread (0: the row kernel) x threads per block
-    // (1024, or 512 with two blocks per CU; stored indices only)
-    const std::vector<int> blocks = m->mf ? std::vector<int>{kTileBlock} : std::vector<int>{kTileBlock, kTileBlock / 2};
-    auto tune_gather = [&](int vbytes, const std::vector<int>& rpts, int& rpt, int& bs, auto&& run) {
+    // (1024, or 512 with two blocks per CU) x sub-tiles per staging (stored
+    // indices only)
+    struct GShape { int rpt, bs, sub; };
+    auto tune_gather = [&](int vbytes, const std::vector<int>& rpts, int& rpt, int& bs, int& sub, auto&& run) {
         GatherPlan& gp = vbytes == 4 ? m->gplan4 : m->gplan8;
-        int best_r = rpt, best_b = bs;
-        double best_us = 1e300;
-        for (int r : rpts)
-            for (int b : blocks) {
-                if (r == 0 && b != kTileBlock) continue;
-                build_gather_plan(m, gp, vbytes, r, b);
-                if (r > 0 && !gp.ok) continue;
-                const double us = t.us(run, s);
-                std::snprintf(line, sizeof line, "gather f%d rpt %d block %d: %.1f us\n", 8 * vbytes, r, b, us);
-                log += line;
-                if (us < best_us) best_us = us, best_r = r, best_b = b;
+        std::vector<GShape> cands;
+        for (int r : rpts) {
+            if (r == 0) {
+                cands.push_back(GShape{0, kTileBlock, 1});
+                continue;
             }
-        rpt = best_r;
-        bs = best_b;
-        build_gather_plan(m, gp, vbytes, rpt, bs);
+            for (int sb : {1, 2, 3})
+                if (sb == 1 || (!m->mf && r * vbytes == 8 && r * m->ibytes <= 4))
+                    cands.push_back(GShape{r, kTileBlock, sb});
+            if (!m->mf) cands.push_back(GShape{r, kTileBlock / 2, 1});
+        }
+        GShape best{rpt, bs, sub};
+        double best_us = 1e300;
+        for (const GShape& c : cands) {
+            build_gather_plan(m, gp, vbytes, c.rpt, c.bs, c.sub);
+            if (c.rpt > 0 && !gp.ok) continue;
+            const double us = t.us(run, s);
+            std::snprintf(line, sizeof line, "gather f%d rpt %d block %d sub %d: %.1f us\n", 8 * vbytes, c.rpt, c.bs,
+                          c.sub, us);
+            log += line;
+            if (us < best_us) best_us = us, best = c;
+        }
+        rpt = best.rpt;
+        bs = best.bs;
+        sub = best.sub;
+        build_gather_plan(m, gp, vbytes, rpt, bs, sub);
     };
     tune_gather(4, (!m->mf && m->ibytes == 2) ? std::vector<int>{4, 8, 2, 0} : std::vector<int>{4, 2, 0}, tp.rpt4,
-                tp.bs4, [&] { spmv<float>(m, x4.as<float>(), y4.as<float>(), s); });
-    tune_gather(8, std::vector<int>{2, 1, 0}, tp.rpt8, tp.bs8,
+                tp.bs4, tp.sub4, [&] { spmv<float>(m, x4.as<float>(), y4.as<float>(), s); });
+    tune_gather(8, std::vector<int>{2, 1, 0}, tp.rpt8, tp.bs8, tp.sub8,
                 [&] { spmv<double>(m, x8.as<double>(), y8.as<double>(), s); });
     // scatter: work order, rows per step, tiles per CU and the partial-traffic
     // cap (FP32 timing)
@@ -1163,9 +1202,10 @@ std::string autotune(cad_matrix* m)
     CAD_HIP(hipStreamSynchronize(s));
     m->tuned = true;
     std::snprintf(line, sizeof line,
-                  "chosen: gather f32 rpt %d block %d, f64 rpt %d block %d, scatter order %d rpt %d blocks/CU %.0f "
-                  "cap %.2f\n",
-                  tp.rpt4, tp.bs4, tp.rpt8, tp.bs8, tp.scatter_order, tp.srpt, tp.blocks_per_cu, tp.partial_cap);
+                  "chosen: gather f32 rpt %d block %d sub %d, f64 rpt %d block %d sub %d, scatter order %d rpt %d "
+                  "blocks/CU %.0f cap %.2f\n",
+                  tp.rpt4, tp.bs4, tp.sub4, tp.rpt8, tp.bs8, tp.sub8, tp.scatter_order, tp.srpt, tp.blocks_per_cu,
+                  tp.partial_cap);
     log += line;
     return log;
 }

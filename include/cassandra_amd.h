@@ -126,7 +126,10 @@ int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size);
  * solvers' captured graphs are dropped).  Rows per thread of the FP32 tile
  * gather 2, 4 or 8 (8 for u16 indices only), of the FP64 one 1 or 2, 0 for
  * the row kernel; threads per gather block 1024, or 512 with two blocks per CU
- * (stored indices only); scatter rows per step 4, or 8 for u16 indices;
+ * (stored indices only); sub-tiles of rpt x block rows sharing each LDS
+ * staging 1..3 (2 and 3 only for FP32 with rpt 2 and FP64 with rpt 1, full
+ * blocks and stored indices); scatter
+ * rows per step 4, or 8 for u16 indices;
  * scatter order 0 (chunks per window) or 1 (chunks shared by the windows,
  * XCD-major); scatter tiles per CU > 0 and partial-traffic cap in (0, 1].
  * Every schedule computes the same products (see cad_matrix_autotune);
@@ -140,8 +143,8 @@ int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size);
  * windows (0 where not used).  The first n of them are written. */
 int cad_matrix_kernel_times(cad_matrix* m, int reps, double* us, int n);
 typedef struct cad_schedule {
-    int gather_f32_rpt, gather_f32_block;
-    int gather_f64_rpt, gather_f64_block;
+    int gather_f32_rpt, gather_f32_block, gather_f32_sub;
+    int gather_f64_rpt, gather_f64_block, gather_f64_sub;
     int scatter_rpt, scatter_order;
     double scatter_tiles_per_cu, scatter_partial_cap;
 } cad_schedule;

@@ -16,8 +16,10 @@ from cassandra_amd.fit import Patterns
 
 pytestmark = pytest.mark.gpu
 
-GATHER32 = [(8, 1024), (4, 1024), (2, 1024), (8, 512), (4, 512), (2, 512), (0, 1024)]
-GATHER64 = [(2, 1024), (1, 1024), (2, 512), (1, 512), (0, 1024)]
+# (rows per thread, threads per block, sub-tiles per staging)
+GATHER32 = [(8, 1024, 1), (4, 1024, 1), (2, 1024, 1), (8, 512, 1), (4, 512, 1), (2, 512, 1), (0, 1024, 1),
+            (2, 1024, 2), (2, 1024, 3)]
+GATHER64 = [(2, 1024, 1), (1, 1024, 1), (2, 512, 1), (1, 512, 1), (0, 1024, 1), (1, 1024, 2), (1, 1024, 3)]
 SCATTER = [(4, 0, 2.0, 0.25), (8, 1, 4.0, 0.25), (4, 1, 1.0, 1.0), (8, 0, 4.0, 0.1), (4, 0, 1.0, 1.0)]
 
 
@@ -45,12 +47,12 @@ def schedules():
 def test_schedule_products(ctx, oracle, fit8, g32, g64, sc):
     idx, A = fit8
     say("schedule", g32, g64, sc)
-    A.set_schedule(gather_f32_rpt=g32[0], gather_f32_block=g32[1], gather_f64_rpt=g64[0],
-                   gather_f64_block=g64[1], scatter_rpt=sc[0], scatter_order=sc[1], scatter_tiles_per_cu=sc[2],
-                   scatter_partial_cap=sc[3])
+    A.set_schedule(gather_f32_rpt=g32[0], gather_f32_block=g32[1], gather_f32_sub=g32[2], gather_f64_rpt=g64[0],
+                   gather_f64_block=g64[1], gather_f64_sub=g64[2], scatter_rpt=sc[0], scatter_order=sc[1],
+                   scatter_tiles_per_cu=sc[2], scatter_partial_cap=sc[3])
     got = A.schedule()
-    assert (got["gather_f32_rpt"], got["gather_f32_block"], got["gather_f64_rpt"], got["gather_f64_block"]) == \
-        (g32[0], g32[1], g64[0], g64[1])
+    assert (got["gather_f32_rpt"], got["gather_f32_block"], got["gather_f32_sub"]) == g32
+    assert (got["gather_f64_rpt"], got["gather_f64_block"], got["gather_f64_sub"]) == g64
     gen = torch.Generator(device="cuda").manual_seed(7)
     x = torch.randn(A.cols, dtype=torch.float32, device="cuda", generator=gen)
     assert np.array_equal(A.spmv(x).cpu().numpy(), oracle.spmv(idx, x.cpu().numpy()))
@@ -65,12 +67,20 @@ def test_schedule_solver_and_errors(ctx, oracle, fit8):
     idx, A = fit8
     say("schedule solver + errors")
     _, b = A.make_targets(2007, 3007)
-    A.set_schedule(gather_f32_rpt=4, gather_f32_block=512, gather_f64_rpt=2, gather_f64_block=512)
+    A.set_schedule(gather_f32_rpt=2, gather_f32_block=1024, gather_f32_sub=3, gather_f64_rpt=1,
+                   gather_f64_block=1024, gather_f64_sub=2)
+    xm, rep = A.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
+    assert rep["converged"] == 1
+    assert oracle.rel_normal_residual(idx, A.cols, b.cpu().numpy(), xm.cpu().numpy())[0] <= 1.5e-10
+    A.set_schedule(gather_f32_rpt=4, gather_f32_block=512, gather_f32_sub=1, gather_f64_rpt=2, gather_f64_block=512,
+                   gather_f64_sub=1)
     xm, rep = A.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
     assert rep["converged"] == 1
     assert oracle.rel_normal_residual(idx, A.cols, b.cpu().numpy(), xm.cpu().numpy())[0] <= 1.5e-10
     before = A.schedule()
     for bad in (dict(gather_f32_rpt=3), dict(gather_f32_block=256), dict(gather_f64_rpt=4),
+                dict(gather_f32_rpt=8, gather_f32_sub=2), dict(gather_f32_rpt=2, gather_f32_sub=4),
+                dict(gather_f32_rpt=2, gather_f32_block=512, gather_f32_sub=2),
                 dict(scatter_rpt=2), dict(scatter_order=2), dict(scatter_tiles_per_cu=0.0),
                 dict(scatter_partial_cap=1.5)):
         with pytest.raises(capi.CadError) as e:
@@ -87,6 +97,8 @@ def test_matrix_free_keeps_full_blocks(ctx, oracle):
         F.set_schedule(gather_f32_block=512)
     with pytest.raises(capi.CadError):
         F.set_schedule(gather_f32_rpt=8)
+    with pytest.raises(capi.CadError):
+        F.set_schedule(gather_f32_rpt=2, gather_f32_sub=2)
     F.set_schedule(gather_f32_rpt=2, scatter_rpt=4)
     idx = oracle.Group(N.FIT8).indices(pos)
     x = torch.randn(F.cols, dtype=torch.float32, device="cuda")
