@@ -214,6 +214,7 @@ struct EpiLsqrU {
     LsqrScalars* scal;
     VT f;
     static constexpr bool kReduce = true;
+    static constexpr bool kFinishKernel = true;
     __device__ bool begin(int sys)
     {
         const LsqrScalars* st = &scal[sys];
@@ -289,6 +290,7 @@ struct EpiResidual {
     VT* __restrict__ u;
     LsqrScalars* scal;
     static constexpr bool kReduce = true;
+    static constexpr bool kFinishKernel = false;
     __device__ bool begin(int sys) { return scal[sys].active != 0 && scal[sys].restart != 0; }
     __device__ double row(int64_t r, double a)
     {
@@ -340,6 +342,19 @@ struct EpiTargets {
 };
 
 // Block reduction of the epilogue shares, then the per-system grid finish.
+// An epilogue with kFinishKernel only stores its block partials; the
+// per-system finish runs in k_epi_finish, the next launch.  That avoids the
+// device-scope fences of the last-block ticket (on gfx950 an L2 write-back
+// and invalidate in every block), which the tile gather pays once per tile.
+template <class Epi>
+constexpr bool finish_kernel()
+{
+    if constexpr (Epi::kReduce)
+        return Epi::kFinishKernel;
+    else
+        return false;
+}
+
 template <int BS, class Epi>
 __device__ __forceinline__ void reduce_tail(Epi& epi, double acc, int sys, const GridSys& g)
 {
@@ -347,6 +362,7 @@ __device__ __forceinline__ void reduce_tail(Epi& epi, double acc, int sys, const
         __shared__ double sh[BS / 64];
         acc = block_sum<BS>(acc, sh);
         if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
+        if constexpr (finish_kernel<Epi>()) return;
         if (arrive_last(&g.counters[sys], g.blk_count[sys])) {
             double s = 0.0;
             for (int i = threadIdx.x; i < g.blk_count[sys]; i += BS) s += g.partials[2 * (g.blk_first[sys] + i)];
@@ -354,6 +370,21 @@ __device__ __forceinline__ void reduce_tail(Epi& epi, double acc, int sys, const
             if (threadIdx.x == 0) epi.finish(sys, s);
         }
     }
+}
+
+// One block per system: the sum of the system's block partials (the order
+// of the last-block ticket's sum) handed to the epilogue's finish.  Systems
+// whose gather blocks returned early (begin() false) are skipped the same way.
+template <class Epi>
+__global__ __launch_bounds__(kRowBlock) void k_epi_finish(Epi epi, GridSys g)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const int sys = blockIdx.x;
+    if (!epi.begin(sys)) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < g.blk_count[sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[sys] + i)];
+    s = block_sum<kRowBlock>(s, sh);
+    if (threadIdx.x == 0) epi.finish(sys, s);
 }
 
 template <class VT, class View>

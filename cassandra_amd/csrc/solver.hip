@@ -289,6 +289,15 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
         }
     });
     CAD_HIP(hipGetLastError());
+    // the epilogue's per-system finish, from the block partials the gather left
+    if constexpr (finish_kernel<Epi>()) {
+        const bool empty = gp.ok ? gp.tiles.empty() : m->grid.rows.empty();
+        if (!empty) {
+            hipLaunchKernelGGL(k_epi_finish<Epi>, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, epi,
+                               gp.ok ? gp.sys() : m->grid.row_sys());
+            CAD_HIP(hipGetLastError());
+        }
+    }
 }
 
 template <class VT>
