@@ -1417,8 +1417,10 @@ __global__ void k_colscale(const uint32_t* __restrict__ cnt, int64_t n, double* 
 }
 
 // ------------------------------------------------------------ LSQR cols ----
-// LSQR step 2 (Math/Solver.h:162) and the rotation (:165-171):
-//   vbar = D z - beta v, alpha = |vbar|, then rho, c, s, theta, phi, phibar.
+// LSQR step 2 (Math/Solver.h:162): vbar = D z - beta v and the block partials
+// of |vbar|^2; k_lsqr_rot finishes alpha and the rotation.  (A grid-wide
+// last-block ticket here would cost every block an L2 write-back and
+// invalidate on gfx950.)
 template <class VT>
 __global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict__ work,
                                                       const double* __restrict__ D, const VT* __restrict__ z,
@@ -1426,14 +1428,11 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict_
 {
     __shared__ double sh[kRowBlock / 64];
     const ColWork w = work[blockIdx.x];
-    LsqrScalars* st = &scal[w.sys];
+    const LsqrScalars* st = &scal[w.sys];
     if (!st->active) return;
-    // every block reads stop before it arrives, so the last block's update
-    // below is seen only by the next launch
-    const bool stopped = st->stop != 0;
     const double beta = st->beta;
     double acc = 0.0;
-    if (!stopped)
+    if (st->stop == 0)
         for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
             const double vb = D[c] * double(z[c]) - beta * v[c];
             v[c] = vb;
@@ -1441,36 +1440,47 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict_
         }
     acc = block_sum<kRowBlock>(acc, sh);
     if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
-    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
-        s = block_sum<kRowBlock>(s, sh);
-        if (threadIdx.x == 0 && stopped) {
-            // the iteration after the stop: no further step
-            st->stop = 2;
-            st->step_x = 0.0;
-            st->step_w = 0.0;
-        } else if (threadIdx.x == 0) {
-            const double alpha = sqrt(s);
-            const double rho = sqrt(st->rhobar * st->rhobar + beta * beta);
-            const double c = rho != 0.0 ? st->rhobar / rho : 0.0;
-            const double sn = rho != 0.0 ? beta / rho : 0.0;
-            const double theta = sn * alpha;
-            const double phi = c * st->phibar;
-            st->rhobar = -c * alpha;
-            st->phibar = sn * st->phibar;
-            st->step_x = rho != 0.0 ? phi / rho : 0.0;
-            st->step_w = rho != 0.0 ? theta / rho : 0.0;
-            st->alpha = alpha;
-            st->nres = st->phibar * alpha * fabs(c);
-            st->rnorm = st->phibar;
-            st->anorm2 += s;
-            st->iters += 1;
-            // met the target: this iteration's step is the last (a rank-
-            // deficient system that continued would iterate on rounding noise)
-            if (st->target > 0.0 && st->nres <= st->target * st->nres0) st->stop = 1;
-        }
+}
+
+// The rotation (Math/Solver.h:165-171), one block per system: alpha = |vbar|
+// from k_lsqr_v's partials, then rho, c, s, theta, phi, phibar and the
+// device-side stop.
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_rot(GridSys g, LsqrScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const int sys = blockIdx.x;
+    LsqrScalars* st = &scal[sys];
+    if (!st->active) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < g.blk_count[sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[sys] + i)];
+    s = block_sum<kRowBlock>(s, sh);
+    if (threadIdx.x != 0) return;
+    if (st->stop != 0) {
+        // the iteration after the stop: no further step
+        st->stop = 2;
+        st->step_x = 0.0;
+        st->step_w = 0.0;
+        return;
     }
+    const double beta = st->beta;
+    const double alpha = sqrt(s);
+    const double rho = sqrt(st->rhobar * st->rhobar + beta * beta);
+    const double c = rho != 0.0 ? st->rhobar / rho : 0.0;
+    const double sn = rho != 0.0 ? beta / rho : 0.0;
+    const double theta = sn * alpha;
+    const double phi = c * st->phibar;
+    st->rhobar = -c * alpha;
+    st->phibar = sn * st->phibar;
+    st->step_x = rho != 0.0 ? phi / rho : 0.0;
+    st->step_w = rho != 0.0 ? theta / rho : 0.0;
+    st->alpha = alpha;
+    st->nres = st->phibar * alpha * fabs(c);
+    st->rnorm = st->phibar;
+    st->anorm2 += s;
+    st->iters += 1;
+    // met the target: this iteration's step is the last (a rank-deficient
+    // system that continued would iterate on rounding noise)
+    if (st->target > 0.0 && st->nres <= st->target * st->nres0) st->stop = 1;
 }
 
 // LSQR update (Math/Solver.h:173-175): v = vbar/alpha, vd = D v,

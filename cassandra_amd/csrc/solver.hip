@@ -469,6 +469,7 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     scatter<VT, VT, VT>(m, ws.u.as<VT>(), sc, ws.z.as<VT>(), s);
     hipLaunchKernelGGL(k_lsqr_v<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.D.as<double>(), ws.z.as<VT>(), ws.v.as<double>(), g.col_sys(), sc);
+    hipLaunchKernelGGL(k_lsqr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, g.col_sys(), sc);
     hipLaunchKernelGGL(k_lsqr_xw<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.D.as<double>(), ws.v.as<double>(), ws.vd.as<VT>(), ws.w.as<double>(), ws.y.as<double>(),
                        (const LsqrScalars*)sc);
