@@ -1636,14 +1636,35 @@ __global__ __launch_bounds__(kRowBlock) void k_pcg_alpha(const ColWork* __restri
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) acc += double(p[c]) * double(q[c]);
     acc = block_sum<kRowBlock>(acc, sh);
     if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
-    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[w.sys] + i)];
-        s = block_sum<kRowBlock>(s, sh);
-        if (threadIdx.x == 0) {
-            st->pq = s;
-            st->alpha = s != 0.0 ? st->rz / s : 0.0;
-        }
+}
+
+// Per system (one block each): the sums of the column kernels' block partials
+// (slot 0 and slot 1 of each block), in block order.  The column kernels
+// leave partials instead of ending with a last-block ticket, whose
+// device-scope fences cost every block an L2 write-back and invalidate.
+__device__ __forceinline__ void sum_partials(const GridSys& g, int sys, double& s0, double& s1, double* sh)
+{
+    s0 = 0.0;
+    s1 = 0.0;
+    for (int i = threadIdx.x; i < g.blk_count[sys]; i += kRowBlock) {
+        s0 += g.partials[2 * (g.blk_first[sys] + i)];
+        s1 += g.partials[2 * (g.blk_first[sys] + i) + 1];
+    }
+    s0 = block_sum<kRowBlock>(s0, sh);
+    s1 = block_sum<kRowBlock>(s1, sh);
+}
+
+// pq = p . q (slot 0), alpha = rz / pq.
+__global__ __launch_bounds__(kRowBlock) void k_pcg_alpha_fin(GridSys g, PcgScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    PcgScalars* st = &scal[blockIdx.x];
+    if (!st->active) return;
+    double s0, s1;
+    sum_partials(g, blockIdx.x, s0, s1, sh);
+    if (threadIdx.x == 0) {
+        st->pq = s0;
+        st->alpha = s0 != 0.0 ? st->rz / s0 : 0.0;
     }
 }
 
@@ -1676,20 +1697,21 @@ __global__ __launch_bounds__(kRowBlock) void k_pcg_update(const ColWork* __restr
         g.partials[2 * blockIdx.x] = a0;
         g.partials[2 * blockIdx.x + 1] = a1;
     }
-    if (arrive_last(&g.counters[w.sys], g.blk_count[w.sys])) {
-        double s0 = 0.0, s1 = 0.0;
-        for (int i = threadIdx.x; i < g.blk_count[w.sys]; i += kRowBlock) {
-            s0 += g.partials[2 * (g.blk_first[w.sys] + i)];
-            s1 += g.partials[2 * (g.blk_first[w.sys] + i) + 1];
-        }
-        s0 = block_sum<kRowBlock>(s0, sh);
-        s1 = block_sum<kRowBlock>(s1, sh);
-        if (threadIdx.x == 0) {
-            st->beta = st->rz != 0.0 ? s0 / st->rz : 0.0;
-            st->rz = s0;
-            st->rr = s1;
-            st->iters += 1;
-        }
+}
+
+// rz' (slot 0), rr (slot 1), beta = rz' / rz.
+__global__ __launch_bounds__(kRowBlock) void k_pcg_update_fin(GridSys g, PcgScalars* scal)
+{
+    __shared__ double sh[kRowBlock / 64];
+    PcgScalars* st = &scal[blockIdx.x];
+    if (!st->active) return;
+    double s0, s1;
+    sum_partials(g, blockIdx.x, s0, s1, sh);
+    if (threadIdx.x == 0) {
+        st->beta = st->rz != 0.0 ? s0 / st->rz : 0.0;
+        st->rz = s0;
+        st->rr = s1;
+        st->iters += 1;
     }
 }
 

@@ -872,9 +872,11 @@ static void pcg_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     scatter<VT, VT, VT>(m, ws.t.as<VT>(), nullptr, ws.q.as<VT>(), s);
     hipLaunchKernelGGL(k_pcg_alpha<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.p.as<VT>(), ws.q.as<VT>(), g.col_sys(), sc);
+    hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, g.col_sys(), sc);
     hipLaunchKernelGGL(k_pcg_update<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
                        g.d_cols.as<ColWork>(), ws.P.as<VT>(), ws.p.as<VT>(), ws.q.as<VT>(), ws.xr.as<VT>(),
                        ws.rr.as<VT>(), ws.zz.as<VT>(), g.col_sys(), sc);
+    hipLaunchKernelGGL(k_pcg_update_fin, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, g.col_sys(), sc);
     hipLaunchKernelGGL(k_pcg_p<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.zz.as<VT>(), ws.p.as<VT>(), ws.q.as<VT>(), (const PcgScalars*)sc);
     CAD_HIP(hipGetLastError());
