@@ -290,7 +290,7 @@ struct EpiResidual {
     VT* __restrict__ u;
     LsqrScalars* scal;
     static constexpr bool kReduce = true;
-    static constexpr bool kFinishKernel = false;
+    static constexpr bool kFinishKernel = true;
     __device__ bool begin(int sys) { return scal[sys].active != 0 && scal[sys].restart != 0; }
     __device__ double row(int64_t r, double a)
     {
