@@ -150,68 +150,93 @@ void push_reduce(ScatterLists& L, int s, uint32_t sys_col, uint32_t wl, uint32_t
 }
 
 // Chunk-major scatter lists: every window of a system uses the same row
-// chunks, and the tiles of one chunk (all its windows) are placed on one XCD
-// one after the other (workgroup i runs on XCD i mod 8), so the y values and
-// the index entries that the windows of a chunk share are read from HBM once
-// and then hit that XCD's L2.  Slots left over at the end of an XCD's lane
-// hold empty tiles (no rows, no columns).
+// chunks, and the tiles of one chunk (all its windows, the heavier ones split
+// into row parts of equal work) are placed on one XCD one after the other
+// (workgroup i runs on XCD i mod 8), so the y values and the index entries
+// that the windows of a chunk share are read from HBM once and then hit that
+// XCD's L2.  The chunk count is a multiple of the XCD count near the target
+// tile count.  Slots left over at the end of an XCD's lane hold empty tiles
+// (no rows, no columns).
 constexpr int kXcds = 8;
 
 void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
                             const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, uint32_t wmax, int vbytes,
                             int ibytes, double target, const TuneParams& tune, ScatterLists& L)
 {
+    (void)sp;
     const int nsys = int(sys_col.size());
-    struct Win { uint32_t lo, hi; int group; int nslots; };
+    // Windows of the slot groups; a window with more slots than the lightest
+    // is split into parts (row sub-ranges of the chunk), so that every tile
+    // of a chunk does about the same work and the chunk's tiles, running
+    // side by side on one XCD, stream its rows at the same pace.
+    struct Win { uint32_t lo, hi; int group; int nslots; int parts; };
     std::vector<Win> wins;
     double work_per_row = 0, width = 0;
+    int minslots = 1 << 30;
     for (size_t gi = 0; gi < L.groups.size(); gi++) {
         const SlotGroup& g = L.groups[gi];
         for (uint32_t wl = g.col_lo; wl < g.col_hi; wl += wmax) {
             const uint32_t wh = std::min(g.col_hi, wl + wmax);
-            wins.push_back(Win{wl, wh, int(gi), g.nslots});
+            wins.push_back(Win{wl, wh, int(gi), g.nslots, 1});
             L.widest = std::max(L.widest, wh - wl);
             work_per_row += g.nslots;
             width += wh - wl;
+            minslots = std::min(minslots, int(g.nslots));
         }
     }
     const int nw = int(wins.size());
+    int tiles_per_chunk = 0;
+    for (Win& w : wins) {
+        w.parts = std::max(1, int(std::lround(double(w.nslots) / std::max(1, minslots))));
+        tiles_per_chunk += w.parts;
+    }
     int64_t total_rows = 0;
     for (int s = 0; s < nsys; s++) total_rows += sys_rows[s + 1] - sys_rows[s];
+    // chunks: about target tiles in all, a multiple of the XCD count
+    int64_t chunks = std::max<int64_t>(1, std::llround(target / std::max(1, tiles_per_chunk)));
+    if (chunks > kXcds) chunks = chunks / kXcds * kXcds;
     std::vector<std::vector<ScatterWork>> lanes(kXcds);
     int64_t off = 0;
     int chunk_id = 0;
     for (int s = 0; s < nsys; s++) {
         const int64_t R = sys_rows[s + 1] - sys_rows[s];
         int64_t C = 0;
-        if (R > 0) {
+        std::vector<int64_t> woff(size_t(std::max(1, nw)));
+        if (R > 0 && nw > 0) {
             const double share = total_rows > 0 ? double(R) / double(total_rows) : 1.0;
-            C = std::max<int64_t>(1, std::llround(target * share / std::max(1, nw)));
-            const double cap = tune.partial_cap * double(R) * work_per_row * ibytes / (2.0 * width * vbytes);
+            C = std::max<int64_t>(1, std::llround(double(chunks) * share));
+            const double cap = tune.partial_cap * double(R) * work_per_row * ibytes /
+                               (2.0 * width * vbytes * double(tiles_per_chunk) / std::max(1, nw));
             C = std::max<int64_t>(1, std::min<int64_t>(C, int64_t(cap)));
             C = std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock);
             const int64_t chunk = (R + C - 1) / C;
             C = (R + chunk - 1) / chunk;
-            std::vector<int64_t> woff(nw);
             for (int j = 0; j < nw; j++) {
-                woff[j] = off;
-                off += C * int64_t(wins[j].hi - wins[j].lo);
+                woff[size_t(j)] = off;
+                off += C * wins[size_t(j)].parts * int64_t(wins[size_t(j)].hi - wins[size_t(j)].lo);
             }
             for (int64_t c = 0; c < C; c++, chunk_id++) {
                 auto& lane = lanes[size_t(chunk_id % kXcds)];
+                const int64_t c0 = c * chunk, c1 = std::min(R, (c + 1) * chunk);
                 for (int j = 0; j < nw; j++) {
-                    ScatterWork w;
-                    w.row_begin = sys_rows[s] + c * chunk;
-                    w.row_end = sys_rows[s] + std::min(R, (c + 1) * chunk);
-                    w.part_off = woff[j] + c * int64_t(wins[j].hi - wins[j].lo);
-                    w.win_lo = sys_col[s] + wins[j].lo;
-                    w.win_hi = sys_col[s] + wins[j].hi;
-                    w.group = wins[j].group;
-                    w.sys = s;
-                    lane.push_back(w);
+                    const Win& wj = wins[size_t(j)];
+                    const int64_t W = int64_t(wj.hi - wj.lo);
+                    for (int q = 0; q < wj.parts; q++) {
+                        ScatterWork w;
+                        w.row_begin = sys_rows[s] + c0 + (c1 - c0) * q / wj.parts;
+                        w.row_end = sys_rows[s] + c0 + (c1 - c0) * (q + 1) / wj.parts;
+                        w.part_off = woff[size_t(j)] + (c * wj.parts + q) * W;
+                        w.win_lo = sys_col[s] + wj.lo;
+                        w.win_hi = sys_col[s] + wj.hi;
+                        w.group = wj.group;
+                        w.sys = s;
+                        lane.push_back(w);
+                    }
                 }
             }
-            for (int j = 0; j < nw; j++) push_reduce(L, s, sys_col[s], wins[j].lo, wins[j].hi, woff[j], C);
+            for (int j = 0; j < nw; j++)
+                push_reduce(L, s, sys_col[s], wins[size_t(j)].lo, wins[size_t(j)].hi, woff[size_t(j)],
+                            C * wins[size_t(j)].parts);
         }
         // columns outside every window are written as zeros
         std::vector<std::pair<uint32_t, uint32_t>> covered;
