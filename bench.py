@@ -488,12 +488,16 @@ def main():
     # HBM bytes per launch of that kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE passes over `bench.py --traffic` (scaled by an
     # elementwise copy of 1 GiB; see profiles/traffic.json)
+    # (the scatter's traffic depends on its work order: measured for both)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
             tj = json.load(open(tpath))
-            traffic = tj.get(dom_name.split()[0], {}).get("total_bytes")
+            key = dom_name.split()[0]
+            if key == "k_scatter_tile":
+                key = f"{key}/order{A.schedule()['scatter_order']}"
+            traffic = tj.get(key, {}).get("total_bytes")
         except Exception:
             traffic = None
 
