@@ -136,7 +136,7 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
         return;
     }
     ScatterPlan& sp = const_cast<ScatterPlan&>(eval_plan(p, B));
-    ScatterLists& L = sp.f8;  // FP64 LDS windows (k_scatter_part accumulates in FP64)
+    ScatterLists& L = sp.f8;  // 8-byte LDS windows (k_scatter_part accumulates in FP64)
     const size_t need = sizeof(double) * size_t(std::max<int64_t>(1, L.part_elems));  // either precision
     if (L.part.bytes < need) L.part = DBuf(need);
     VT* part = L.part.as<VT>();

@@ -1100,9 +1100,10 @@ __global__ __launch_bounds__(kRowBlock) void k_absmax_systems(const int32_t* __r
 
 // A^T (y * scale), stage 1: each block owns a column window of one slot group
 // in LDS, scatters a chunk of rows into it and stores the window to its slice
-// of the partial buffer -- no global atomics.  The window accumulates in FP64
-// whatever VT is: LDS atomic adds on doubles run several times faster than on
-// floats on this GPU, and the sums are exact for integer data either way.
+// of the partial buffer -- no global atomics.  This generic form (the
+// evaluator's row-major indices) accumulates in FP64 whatever VT is: LDS
+// atomic adds on doubles run several times faster than on floats on this GPU,
+// and the sums are exact for integer data either way.
 // scale = 1 / div[sys].beta when div is given (0 when beta == 0), else 1.
 template <class VT, class YT, class View>
 __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const ScatterWork* __restrict__ work,
@@ -1208,7 +1209,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* win = reinterpret_cast<double*>(smem_raw);  // FP64 window, as in k_scatter_part
+    double* win = reinterpret_cast<double*>(smem_raw);  // 8-byte window words (FP64 or fixed point)
     const ScatterWork w = work[blockIdx.x];
     if (!scatter_selected(div, mode, w.sys)) return;
     const uint32_t W = w.win_hi - w.win_lo;

@@ -58,7 +58,8 @@ void allow_lds(K kernel, size_t bytes)
     }
 }
 
-// The scatter windows: every real-valued scatter accumulates in FP64 LDS, so
+// The scatter windows: every real-valued scatter accumulates in 8-byte LDS
+// words (FP64, or the 64-bit fixed point of the FP32 stored-index scatter), so
 // it uses the 8-byte windows whatever its value type (f4 serves the u32
 // multiplicity counts of the Jacobi diagonal).
 template <class VT>
