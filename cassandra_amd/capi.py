@@ -34,7 +34,7 @@ class Schedule(C.Structure):
         ("gather_f32_rpt", C.c_int), ("gather_f32_block", C.c_int), ("gather_f32_sub", C.c_int),
         ("gather_f64_rpt", C.c_int), ("gather_f64_block", C.c_int), ("gather_f64_sub", C.c_int),
         ("scatter_rpt", C.c_int), ("scatter_order", C.c_int),
-        ("scatter_tiles_per_cu", C.c_double), ("scatter_partial_cap", C.c_double),
+        ("scatter_tiles_per_cu", C.c_double), ("scatter_partial_cap", C.c_double), ("scatter_balance", C.c_double),
     ]
 
 

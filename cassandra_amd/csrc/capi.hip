@@ -485,7 +485,7 @@ int cad_matrix_get_schedule(const cad_matrix* m, cad_schedule* out)
         need(m != nullptr && out != nullptr, "null argument");
         const TuneParams& t = m->tune;
         *out = cad_schedule{t.rpt4, t.bs4, t.sub4, t.rpt8, t.bs8, t.sub8, t.srpt, t.scatter_order, t.blocks_per_cu,
-                            t.partial_cap};
+                            t.partial_cap, t.scatter_balance};
     });
 }
 
@@ -504,6 +504,7 @@ int cad_matrix_set_schedule(cad_matrix* m, const cad_schedule* sc)
         t.scatter_order = sc->scatter_order;
         t.blocks_per_cu = sc->scatter_tiles_per_cu;
         t.partial_cap = sc->scatter_partial_cap;
+        t.scatter_balance = sc->scatter_balance;
         set_schedule(m, t);
     });
 }

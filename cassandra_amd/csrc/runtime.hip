@@ -277,12 +277,20 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
     L.widest = 0;
     pack_groups(sp, wmax, L);
     const int nsys = int(sys_col.size());
+    // Work of a window per row: its slot loads, and its atomics -- the share
+    // of the group's entries that fall into the window, taken as its share of
+    // the group's columns.  scatter_balance weighs the two (1: loads only).
+    const double lam = std::min(1.0, std::max(0.0, tune.scatter_balance));
+    auto win_cost = [&](const SlotGroup& g, uint32_t W) {
+        const double frac = double(W) / double(std::max<uint32_t>(1, g.col_hi - g.col_lo));
+        return g.nslots * (lam + (1.0 - lam) * frac);
+    };
     double total = 0;
     for (int s = 0; s < nsys; s++)
         for (auto& g : L.groups) {
             const int64_t R = sys_rows[s + 1] - sys_rows[s];
-            const uint32_t nwin = (g.col_hi - g.col_lo + wmax - 1) / wmax;
-            total += double(R) * g.nslots * nwin;
+            for (uint32_t wl = g.col_lo; wl < g.col_hi; wl += wmax)
+                total += double(R) * win_cost(g, std::min(g.col_hi, wl + wmax) - wl);
         }
     const double target = tune.blocks_per_cu * std::max(cus, 1);
     int64_t off = 0;
@@ -302,7 +310,7 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
                 L.widest = std::max(L.widest, W);
                 int64_t C = 0;
                 if (R > 0) {
-                    const double share = total > 0 ? double(R) * g.nslots / total : 1.0;
+                    const double share = total > 0 ? double(R) * win_cost(g, W) / total : 1.0;
                     C = std::max<int64_t>(1, int64_t(std::llround(target * share)));
                     const double cap = tune.partial_cap * double(R) * g.nslots * ibytes / (2.0 * W * vbytes);
                     C = std::max<int64_t>(1, std::min<int64_t>(C, int64_t(cap)));

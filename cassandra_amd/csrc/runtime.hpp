@@ -115,6 +115,7 @@ struct TuneParams {
     double blocks_per_cu = 1.0; // scatter tiles per CU the row chunks aim at
     double partial_cap = 1.0;   // partial-slice traffic cap as a fraction of index traffic
     int scatter_order = 0;      // 0: chunks per window by work share; 1: shared chunks, XCD-major
+    double scatter_balance = 1.0; // order 0: weight of slot loads against atomics in a window's work (1: loads)
 };
 
 // Slot columns of a stored index matrix start at multiples of this many rows,

@@ -1077,7 +1077,8 @@ void set_schedule(cad_matrix* m, const TuneParams& tp)
     if (!sub_ok(tp.bs4, tp.sub4, tp.rpt4, 4) || !sub_ok(tp.bs8, tp.sub8, tp.rpt8, 8))
         throw Error(CAD_ERR_ARGUMENT, "unsupported sub-tiles of the tile gather");
     if (!(u16 ? one_of(tp.srpt, {4, 8}) : tp.srpt == 4) || !one_of(tp.scatter_order, {0, 1}) ||
-        !(tp.blocks_per_cu > 0 && tp.blocks_per_cu <= 64) || !(tp.partial_cap > 0 && tp.partial_cap <= 1))
+        !(tp.blocks_per_cu > 0 && tp.blocks_per_cu <= 64) || !(tp.partial_cap > 0 && tp.partial_cap <= 1) ||
+        !(tp.scatter_balance >= 0 && tp.scatter_balance <= 1))
         throw Error(CAD_ERR_ARGUMENT, "unsupported scatter schedule");
     CAD_HIP(hipStreamSynchronize(m->ctx->stream));
     drop_graphs(m);
@@ -1177,27 +1178,31 @@ std::string autotune(cad_matrix* m)
     // scatter: work order, rows per step, tiles per CU and the partial-traffic
     // cap (FP32 timing)
     {
-        struct Cand { int order, srpt; double bpc, cap; };
+        struct Cand { int order, srpt; double bpc, cap, bal; };
         std::vector<Cand> cands;
         const std::vector<int> srpts = (!m->mf && m->ibytes == 2) ? std::vector<int>{4, 8} : std::vector<int>{4};
         for (int order : {1, 0})
             for (int sr : srpts)
                 for (double bpc : {1.0, 2.0, 4.0})
-                    for (double cap : {0.1, 0.25, 1.0}) cands.push_back(Cand{order, sr, bpc, cap});
+                    for (double cap : {0.1, 0.25, 1.0})
+                        for (double bal : {1.0, 0.5, 0.25})
+                            if (order == 0 || bal == 1.0) cands.push_back(Cand{order, sr, bpc, cap, bal});
         const TuneParams keep = tp;
-        Cand best{tp.scatter_order, tp.srpt, tp.blocks_per_cu, tp.partial_cap};
+        Cand best{tp.scatter_order, tp.srpt, tp.blocks_per_cu, tp.partial_cap, tp.scatter_balance};
         double best_us = 1e300;
         for (const Cand& c : cands) {
             tp.scatter_order = c.order;
             tp.srpt = c.srpt;
             tp.blocks_per_cu = c.bpc;
             tp.partial_cap = c.cap;
+            tp.scatter_balance = c.bal;
             build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
                                m->mf ? 2 : m->ibytes, *m->ctx, tp);
             const double us =
                 t.us([&] { scatter<float, float, float>(m, y4.as<float>(), nullptr, z4.as<float>(), s); }, s);
-            std::snprintf(line, sizeof line, "scatter f32 order %d rpt %d blocks/CU %.0f cap %.2f (%zu tiles): %.1f us\n",
-                          c.order, c.srpt, c.bpc, c.cap, m->splan.f8.work.size(), us);
+            std::snprintf(line, sizeof line,
+                          "scatter f32 order %d rpt %d blocks/CU %.0f cap %.2f balance %.2f (%zu tiles): %.1f us\n",
+                          c.order, c.srpt, c.bpc, c.cap, c.bal, m->splan.f8.work.size(), us);
             log += line;
             // a later candidate must win by 2 %: the chunk-shared XCD-major
             // order comes first and, at equal time, moves half the HBM bytes
@@ -1209,6 +1214,7 @@ std::string autotune(cad_matrix* m)
         tp.srpt = best.srpt;
         tp.blocks_per_cu = best.bpc;
         tp.partial_cap = best.cap;
+        tp.scatter_balance = best.bal;
         build_scatter_plan(m->splan, m->slot_lo, m->slot_hi, m->sys_row_begin, m->sys_col, m->cols_per_sys,
                            m->mf ? 2 : m->ibytes, *m->ctx, tp);
     }
@@ -1216,9 +1222,9 @@ std::string autotune(cad_matrix* m)
     m->tuned = true;
     std::snprintf(line, sizeof line,
                   "chosen: gather f32 rpt %d block %d sub %d, f64 rpt %d block %d sub %d, scatter order %d rpt %d "
-                  "blocks/CU %.0f cap %.2f\n",
+                  "blocks/CU %.0f cap %.2f balance %.2f\n",
                   tp.rpt4, tp.bs4, tp.sub4, tp.rpt8, tp.bs8, tp.sub8, tp.scatter_order, tp.srpt, tp.blocks_per_cu,
-                  tp.partial_cap);
+                  tp.partial_cap, tp.scatter_balance);
     log += line;
     return log;
 }

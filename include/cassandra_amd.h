@@ -131,7 +131,8 @@ int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size);
  * blocks and stored indices); scatter
  * rows per step 4, or 8 for u16 indices;
  * scatter order 0 (chunks per window) or 1 (chunks shared by the windows,
- * XCD-major); scatter tiles per CU > 0 and partial-traffic cap in (0, 1].
+ * XCD-major); scatter tiles per CU > 0, partial-traffic cap in (0, 1] and
+ * balance in [0, 1].
  * Every schedule computes the same products (see cad_matrix_autotune);
  * CAD_ERR_ARGUMENT for a value outside these or a tile gather the matrix
  * cannot use. */
@@ -147,6 +148,7 @@ typedef struct cad_schedule {
     int gather_f64_rpt, gather_f64_block, gather_f64_sub;
     int scatter_rpt, scatter_order;
     double scatter_tiles_per_cu, scatter_partial_cap;
+    double scatter_balance; /* order 0: weight of slot loads against atomics in a window's work, in [0, 1] */
 } cad_schedule;
 int cad_matrix_get_schedule(const cad_matrix* m, cad_schedule* out);
 int cad_matrix_set_schedule(cad_matrix* m, const cad_schedule* schedule);

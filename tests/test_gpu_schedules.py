@@ -20,7 +20,9 @@ pytestmark = pytest.mark.gpu
 GATHER32 = [(8, 1024, 1), (4, 1024, 1), (2, 1024, 1), (8, 512, 1), (4, 512, 1), (2, 512, 1), (0, 1024, 1),
             (2, 1024, 2), (2, 1024, 3)]
 GATHER64 = [(2, 1024, 1), (1, 1024, 1), (2, 512, 1), (1, 512, 1), (0, 1024, 1), (1, 1024, 2), (1, 1024, 3)]
-SCATTER = [(4, 0, 2.0, 0.25), (8, 1, 4.0, 0.25), (4, 1, 1.0, 1.0), (8, 0, 4.0, 0.1), (4, 0, 1.0, 1.0)]
+# (rows per step, order, tiles per CU, partial cap, balance)
+SCATTER = [(4, 0, 2.0, 0.25, 1.0), (8, 1, 4.0, 0.25, 1.0), (4, 1, 1.0, 1.0, 1.0), (8, 0, 4.0, 0.1, 0.25),
+           (4, 0, 1.0, 1.0, 0.5)]
 
 
 def say(*a):
@@ -49,7 +51,7 @@ def test_schedule_products(ctx, oracle, fit8, g32, g64, sc):
     say("schedule", g32, g64, sc)
     A.set_schedule(gather_f32_rpt=g32[0], gather_f32_block=g32[1], gather_f32_sub=g32[2], gather_f64_rpt=g64[0],
                    gather_f64_block=g64[1], gather_f64_sub=g64[2], scatter_rpt=sc[0], scatter_order=sc[1],
-                   scatter_tiles_per_cu=sc[2], scatter_partial_cap=sc[3])
+                   scatter_tiles_per_cu=sc[2], scatter_partial_cap=sc[3], scatter_balance=sc[4])
     got = A.schedule()
     assert (got["gather_f32_rpt"], got["gather_f32_block"], got["gather_f32_sub"]) == g32
     assert (got["gather_f64_rpt"], got["gather_f64_block"], got["gather_f64_sub"]) == g64
@@ -82,7 +84,7 @@ def test_schedule_solver_and_errors(ctx, oracle, fit8):
                 dict(gather_f32_rpt=8, gather_f32_sub=2), dict(gather_f32_rpt=2, gather_f32_sub=4),
                 dict(gather_f32_rpt=2, gather_f32_block=512, gather_f32_sub=2),
                 dict(scatter_rpt=2), dict(scatter_order=2), dict(scatter_tiles_per_cu=0.0),
-                dict(scatter_partial_cap=1.5)):
+                dict(scatter_partial_cap=1.5), dict(scatter_balance=1.5)):
         with pytest.raises(capi.CadError) as e:
             A.set_schedule(**bad)
         assert e.value.code == capi.ERR_ARGUMENT
