@@ -149,7 +149,7 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
         hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s,
                            RowMajor32{idx, p->group->k()}, L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(),
                            L.d_slot_list.as<uint16_t>(), p->d_syscol0.as<uint32_t>(), g,
-                           (const LsqrScalars*)nullptr, kScatterAll, FixedScale{nullptr, 0}, part);
+                           (const LsqrScalars*)nullptr, kScatterAll, FixedScale{nullptr, 0}, 0, part);
         CAD_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL((k_scatter_reduce<VT, VT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,

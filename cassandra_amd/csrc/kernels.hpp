@@ -1111,7 +1111,8 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
                                                                 const uint16_t* __restrict__ slot_list,
                                                                 const uint32_t* __restrict__ sys_col,
                                                                 const YT* __restrict__ y, const LsqrScalars* div,
-                                                                int mode, FixedScale fx, VT* __restrict__ part)
+                                                                int mode, FixedScale fx, int y_padded,
+                                                                VT* __restrict__ part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* win = reinterpret_cast<double*>(smem_raw);
@@ -1141,7 +1142,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
 // repeat slot 0's loads and add nothing).  Every load is unconditional (rows
 // clamped into the chunk), and the next step's y and entries are loaded
 // before this step's atomics, so their latency overlaps the LDS work.
-template <int NS, class WT, class VT, class YT, class IdxT, int RPT>
+template <int NS, class WT, bool VY, class VT, class YT, class IdxT, int RPT>
 __device__ __forceinline__ void scatter_rows(const SlotMajor<IdxT>& m, const ScatterWork& w, int ns,
                                              const int64_t (&slot)[kMaxSegSlots],
                                              const uint32_t (&off)[kMaxSegSlots], const YT* __restrict__ y,
@@ -1156,8 +1157,21 @@ __device__ __forceinline__ void scatter_rows(const SlotMajor<IdxT>& m, const Sca
         const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + min(at, alast));
 #pragma unroll
         for (int j = 0; j < NS; j++) q[j] = vec[slot[j]];
+        if constexpr (VY) {
+            // y readable through the padding to a multiple of 64 rows: the
+            // RPT values in 16-byte loads (rows past the chunk are masked)
+            constexpr int kPer = 16 / int(sizeof(YT));
+            using Y16 = std::conditional_t<sizeof(YT) == 4, float4, double2>;
+            const Y16* __restrict__ yv = reinterpret_cast<const Y16*>(y + min(at, alast));
 #pragma unroll
-        for (int i = 0; i < RPT; i++) yy[i] = y[min(at + i, last)];
+            for (int i = 0; i < RPT; i += kPer) {
+                const Y16 t = yv[i / kPer];
+                __builtin_memcpy(&yy[i], &t, 16);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < RPT; i++) yy[i] = y[min(at + i, last)];
+        }
     };
     int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x;
     VecT q[NS];
@@ -1204,7 +1218,8 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
                                                                 const uint16_t* __restrict__ slot_list,
                                                                 const uint32_t* __restrict__ sys_col,
                                                                 const YT* __restrict__ y, const LsqrScalars* div,
-                                                                int mode, FixedScale fx, VT* __restrict__ part)
+                                                                int mode, FixedScale fx, int y_padded,
+                                                                VT* __restrict__ part)
 {
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
@@ -1232,18 +1247,30 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
             slot[j] = int64_t(sj) * vld;
             off[j] = cb + m.base[sj] - w.win_lo;
         }
-        auto rows = [&](auto* wn) {
+        auto rows_y = [&](auto* wn, auto vy) {
             using WT = std::remove_pointer_t<decltype(wn)>;
+            constexpr bool VY = decltype(vy)::value;
             if (ns == 4)
-                scatter_rows<4, WT, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<4, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
             else if (ns == 8)
-                scatter_rows<8, WT, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<8, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
             else if (ns <= 2)
-                scatter_rows<2, WT, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<2, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
             else if (ns <= 4)
-                scatter_rows<4, WT, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<4, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
             else
-                scatter_rows<8, WT, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<8, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+        };
+        // 16-byte y loads need RPT values to fill them and y padded
+        constexpr bool kVecY = RPT * sizeof(YT) % 16 == 0;
+        auto rows = [&](auto* wn) {
+            if constexpr (kVecY) {
+                if (y_padded) {
+                    rows_y(wn, std::true_type{});
+                    return;
+                }
+            }
+            rows_y(wn, std::false_type{});
         };
         if constexpr (std::is_same_v<VT, float>) {
             if (fscale > 0.0)
@@ -1298,7 +1325,8 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView
                                                                    const uint16_t* __restrict__ slot_list,
                                                                    const uint32_t* __restrict__ sys_col,
                                                                    const YT* __restrict__ y, const LsqrScalars* div,
-                                                                   int mode, FixedScale fx, VT* __restrict__ part)
+                                                                   int mode, FixedScale fx, int y_padded,
+                                                                VT* __restrict__ part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* win = reinterpret_cast<double*>(smem_raw);

@@ -124,7 +124,7 @@ void launch_absmax(ScatterPlan& sp, const YT* y, const GridPlan& g, int nsys, hi
 
 template <class VT, class YT, class View>
 void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
-                          const LsqrScalars* div, int mode, hipStream_t s)
+                          const LsqrScalars* div, int mode, hipStream_t s, bool y_padded = false)
 {
     ScatterLists& L = lists<VT>(sp);
     if (L.work.empty()) return;
@@ -137,7 +137,7 @@ void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint
         allow_lds(kern, lds);
         hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
                            L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(),
-                           sys_col, y, div, mode, fx, partials<VT>(L));
+                           sys_col, y, div, mode, fx, int(y_padded), partials<VT>(L));
     });
     CAD_HIP(hipGetLastError());
 }
@@ -154,11 +154,12 @@ void launch_scatter_reduce(ScatterPlan& sp, const LsqrScalars* div, int mode, ZT
 
 template <class VT, class YT, class ZT, class View>
 void launch_scatter(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
-                    const LsqrScalars* div, int mode, ZT* z, const GridPlan& g, int nsys, hipStream_t s)
+                    const LsqrScalars* div, int mode, ZT* z, const GridPlan& g, int nsys, hipStream_t s,
+                    bool y_padded = false)
 {
     if (fixed_point_scatter<VT, View>() && mode != kScatterLsqr && !lists<VT>(sp).work.empty())
         launch_absmax(sp, y, g, nsys, s);
-    launch_scatter_tiles<VT, YT>(sp, view, rpt, sys_col, y, div, mode, s);
+    launch_scatter_tiles<VT, YT>(sp, view, rpt, sys_col, y, div, mode, s, y_padded);
     launch_scatter_reduce<VT>(sp, div, mode, z, s);
 }
 
@@ -174,14 +175,15 @@ void prepare_scatter(cad_matrix* m)
 }
 
 // A^T y for every system (div == nullptr), or, in an LSQR iteration, for the
-// iterating systems with y scaled by 1 / beta.
+// iterating systems with y scaled by 1 / beta.  y_padded: y is readable up
+// to padded_rows(rows) (the library's own row vectors).
 template <class VT, class YT, class ZT>
-void scatter(cad_matrix* m, const YT* y, const LsqrScalars* div, ZT* z, hipStream_t s)
+void scatter(cad_matrix* m, const YT* y, const LsqrScalars* div, ZT* z, hipStream_t s, bool y_padded = false)
 {
     const int mode = div ? kScatterLsqr : kScatterAll;
     with_view(m, [&](auto view) {
         launch_scatter<VT, YT, ZT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), y, div, mode, z,
-                                   m->grid, m->nsys, s);
+                                   m->grid, m->nsys, s, y_padded);
     });
 }
 
@@ -467,7 +469,7 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
     gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s);
-    scatter<VT, VT, VT>(m, ws.u.as<VT>(), sc, ws.z.as<VT>(), s);
+    scatter<VT, VT, VT>(m, ws.u.as<VT>(), sc, ws.z.as<VT>(), s, true);
     hipLaunchKernelGGL(k_lsqr_v<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.D.as<double>(), ws.z.as<VT>(), ws.v.as<double>(), g.col_sys(), sc);
     hipLaunchKernelGGL(k_lsqr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, g.col_sys(), sc);
@@ -549,7 +551,7 @@ static void lsqr_start_from_residual(cad_matrix* m, Workspace& ws, const double*
 template <class VT>
 static void lsqr_alloc(cad_matrix* m, Workspace& ws, hipStream_t s)
 {
-    ensure(ws.u, sizeof(VT) * m->rows);
+    ensure(ws.u, sizeof(VT) * size_t(padded_rows(m->rows)));  // padded: the scatter reads it in 16-byte groups
     ensure(ws.r64, sizeof(double) * m->rows);
     ensure(ws.v, sizeof(double) * m->cols);
     ensure(ws.w, sizeof(double) * m->cols);
