@@ -69,6 +69,15 @@ struct GatherSeg {
 // handles with hoisted slot offsets).
 constexpr int kMaxSegSlots = 8;
 
+// Slots of one step of a segment wider than the LDS in the stored-index tile
+// gather: the step holds every slot's value of each of its rows until the last
+// window, and with more than 8 bytes of entries per slot (rows per thread x
+// index bytes) the registers hold the values of half as many slots.
+__host__ __device__ constexpr int wide_step_slots(int entry_bytes)
+{
+    return entry_bytes <= 8 ? kMaxSegSlots : kMaxSegSlots / 2;
+}
+
 // One step of the stored-index tile gather: up to kMaxSegSlots consecutive
 // slots of one segment.  A step with phase_count > 0 first stages the phase's
 // segments (phases[phase_first ..]) into LDS side by side; the segment's

@@ -516,8 +516,8 @@ __device__ __forceinline__ uint32_t stage_direct(VT* __restrict__ dst, const VT*
 // phase, and the random reads of x then hit LDS instead of pulling an L2 line
 // per element.  Thread t owns the RPT consecutive rows starting at the tile's
 // first row rounded down to RPT plus RPT * t, so one aligned vector load brings
-// a slot's entries of all its rows; a step's entry loads go out before its
-// staging, so both latencies overlap.  Segments wider than the LDS (windows of
+// a slot's entries of all its rows; the next step's entry loads go out once a
+// step's staging has landed, so they are in flight during its sums.  Segments wider than the LDS (windows of
 // wmax columns) are staged window by window: each slot's value is kept in
 // registers until every window was seen, then the segment's slots are added
 // in slot order, so every row is the sequential sum in slot order
@@ -593,36 +593,42 @@ __device__ __forceinline__ void gather_tile_single(const SlotMajor<IdxT>& m, con
     // slot s of this thread's rows: vec[s * vld]
     const VecT* __restrict__ vec = reinterpret_cast<const VecT*>(m.idx + (live ? a : 0));
     const int64_t vld = m.ld / RPT;
-    // Where the registers allow it (up to 8 bytes of entries per slot) the
-    // next step's entries are loaded once this step's staging has landed, so
-    // they are in flight while this step's sums run; otherwise each step
-    // loads its own entries ahead of its staging.
-    constexpr bool kPrefetch = sizeof(VecT) <= 8;
-    VecT q[kMaxSegSlots], qn[kPrefetch ? kMaxSegSlots : 1];
+    // The next step's entries are loaded once this step's staging has landed,
+    // so they are in flight while this step's sums run.  A segment wider than
+    // the LDS holds its slots' values of every row until its last window; with
+    // more than 8 bytes of entries per slot the registers do not allow holding
+    // the next step's entries too, and the step after it loads its own.
+    constexpr bool kPrefetchWide = sizeof(VecT) <= 8;
+    constexpr int kWide = wide_step_slots(int(sizeof(VecT)));  // slots of a wide step (the plan's split)
+    VecT q[kMaxSegSlots], qn[kMaxSegSlots];
     GatherStep st = nsteps > 0 ? step_at(steps, 0) : GatherStep{};
-    if (kPrefetch && nsteps > 0) load_step<IdxT, RPT, true>(q, vec, vld, st);
+    bool have = false;  // q holds this step's entries (loaded by the step before)
     for (int k = 0; k < nsteps; k++) {
         const bool more = k + 1 < nsteps;
         const GatherStep nx = step_at(steps, more ? k + 1 : k);
-        if constexpr (!kPrefetch) load_step<IdxT, RPT, false>(q, vec, vld, st);
+        if (!have) load_step<IdxT, RPT, true>(q, vec, vld, st);
+        have = false;
         if (st.nwin > 1) {
-            // a segment wider than the LDS, at most kMaxSegSlots slots
-            VT v[RPT][kMaxSegSlots];
+            // a segment wider than the LDS, at most kWide slots per step
+            VT v[RPT][kWide];
 #pragma unroll
-            for (int j = 0; j < kMaxSegSlots; j++)
+            for (int j = 0; j < kWide; j++)
 #pragma unroll
                 for (int i = 0; i < RPT; i++) v[i][j] = VT(0);
             for (uint32_t wl = 0; wl < st.width; wl += wmax) {
                 const uint32_t W = min(wmax, st.width - wl);
                 // the next step's entries load with the last window's staging
-                if constexpr (kPrefetch)
-                    if (more && wl + W >= st.width) load_step<IdxT, RPT, true>(qn, vec, vld, nx);
+                if constexpr (kPrefetchWide)
+                    if (more && wl + W >= st.width) {
+                        load_step<IdxT, RPT, true>(qn, vec, vld, nx);
+                        have = true;
+                    }
                 __syncthreads();
                 const uint32_t h = stage_direct<VT, BS>(xs, xb + st.col_lo + wl, W);
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
 #pragma unroll
-                for (int j = 0; j < kMaxSegSlots; j++) {
+                for (int j = 0; j < kWide; j++) {
                     if (j < st.nslots) {
                         // 32-bit offset: base - col_lo wraps for u32 storage
                         // (base 0) and the sum with the entry unwraps it
@@ -638,10 +644,15 @@ __device__ __forceinline__ void gather_tile_single(const SlotMajor<IdxT>& m, con
                 }
             }
 #pragma unroll
-            for (int j = 0; j < kMaxSegSlots; j++)
+            for (int j = 0; j < kWide; j++)
                 if (j < st.nslots) {
 #pragma unroll
                     for (int i = 0; i < RPT; i++) acc[i] += v[i][j];
+                }
+            if constexpr (kPrefetchWide)
+                if (have) {
+#pragma unroll
+                    for (int j = 0; j < kMaxSegSlots; j++) q[j] = qn[j];
                 }
         } else {
             if (st.phase_count > 0) {
@@ -652,14 +663,16 @@ __device__ __forceinline__ void gather_tile_single(const SlotMajor<IdxT>& m, con
                 }
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
-            } else if constexpr (kPrefetch) {
-                // this step's entries (loaded during the last step's sums)
-                // must be in before the next step's loads go out: the
-                // compiler's own wait for them would be a vmcnt(0) after
+            } else {
+                // this step's entries must be in before the next step's loads
+                // go out: the compiler's own wait for them would be a vmcnt(0)
+                // after those
                 __builtin_amdgcn_s_waitcnt(0);
             }
-            if constexpr (kPrefetch)
-                if (more) load_step<IdxT, RPT, true>(qn, vec, vld, nx);
+            if (more) {
+                load_step<IdxT, RPT, true>(qn, vec, vld, nx);
+                have = true;
+            }
             const uint32_t h = line_head(xb + st.col_lo);
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++) {
@@ -673,12 +686,13 @@ __device__ __forceinline__ void gather_tile_single(const SlotMajor<IdxT>& m, con
                     }
                 }
             }
-        }
-        if constexpr (kPrefetch)
-            if (more) {
+            // (the copy in each branch: qn is not live across a wide step
+            // that does not load it)
+            if (have) {
 #pragma unroll
                 for (int j = 0; j < kMaxSegSlots; j++) q[j] = qn[j];
             }
+        }
         st = nx;
     }
     // all RPT rows valid (every thread but the tile's first and last): the

@@ -471,10 +471,11 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int b
             open_step = -1;
             phase_w = 0;
         }
-        for (int j0 = 0; j0 < sg.nslots; j0 += kMaxSegSlots) {
+        const int per_step = wide ? wide_step_slots(rpt * m->ibytes) : kMaxSegSlots;
+        for (int j0 = 0; j0 < sg.nslots; j0 += per_step) {
             GatherStep st{};
             st.slot_first = uint16_t(sg.slot_first + j0);
-            st.nslots = uint16_t(std::min<int>(kMaxSegSlots, sg.nslots - j0));
+            st.nslots = uint16_t(std::min<int>(per_step, sg.nslots - j0));
             st.col_lo = sg.col_lo;
             st.width = sg.width;
             st.nwin = sg.nwin;
