@@ -85,7 +85,11 @@ __host__ __device__ constexpr int wide_step_slots(int entry_bytes)
 // the LDS, staged window by window by the step itself.  off[j] is slot
 // (slot_first + j)'s offset from its stored entry to the staged column:
 // base - col_lo (+ lds_off for a phased step), modulo 2^32.  64 bytes, read
-// by one scalar load.
+// by one scalar load.  A wide segment split into several steps is walked
+// window 0 up, then down, and so on (wflags): each step after the first starts
+// on the window its predecessor ended on, which is still staged.
+constexpr uint16_t kWideReverse = 1;  // windows from the last to the first
+constexpr uint16_t kWideStaged = 2;   // the first window visited is in LDS already
 struct GatherStep {
     uint32_t off[kMaxSegSlots];
     uint16_t slot_first, nslots;
@@ -93,7 +97,7 @@ struct GatherStep {
     uint32_t col_lo;   // the segment's first column (system-local)
     uint32_t lds_off;  // the segment's offset in the staged phase
     uint32_t width;    // the segment's width (windows of a wide segment)
-    uint16_t nwin, pad;
+    uint16_t nwin, wflags;
     uint32_t pad2[2];
 };
 static_assert(sizeof(GatherStep) == 64, "GatherStep is one 64-byte scalar load");

@@ -615,18 +615,26 @@ __device__ __forceinline__ void gather_tile_single(const SlotMajor<IdxT>& m, con
             for (int j = 0; j < kWide; j++)
 #pragma unroll
                 for (int i = 0; i < RPT; i++) v[i][j] = VT(0);
-            for (uint32_t wl = 0; wl < st.width; wl += wmax) {
+            const bool rev = st.wflags & kWideReverse;
+            for (uint32_t n = 0; n < st.nwin; n++) {
+                const uint32_t wl = (rev ? st.nwin - 1 - n : n) * wmax;
                 const uint32_t W = min(wmax, st.width - wl);
                 // the next step's entries load with the last window's staging
                 if constexpr (kPrefetchWide)
-                    if (more && wl + W >= st.width) {
+                    if (more && n + 1 == st.nwin) {
                         load_step<IdxT, RPT, true>(qn, vec, vld, nx);
                         have = true;
                     }
-                __syncthreads();
-                const uint32_t h = stage_direct<VT, BS>(xs, xb + st.col_lo + wl, W);
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
+                uint32_t h;
+                if (n == 0 && (st.wflags & kWideStaged)) {
+                    // staged by the step before (same segment, same window)
+                    h = line_head(xb + st.col_lo + wl);
+                } else {
+                    __syncthreads();
+                    h = stage_direct<VT, BS>(xs, xb + st.col_lo + wl, W);
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __syncthreads();
+                }
 #pragma unroll
                 for (int j = 0; j < kWide; j++) {
                     if (j < st.nslots) {

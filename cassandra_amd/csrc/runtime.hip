@@ -485,6 +485,11 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int b
                 const int slot = st.slot_first + j;
                 st.off[j] = (m->ibytes == 2 ? m->slot_lo[size_t(slot)] : 0u) - sg.col_lo;
             }
+            if (wide && j0 > 0) {
+                // the windows of this step in the other direction, from the
+                // one the step before ended on
+                st.wflags = uint16_t(kWideStaged | ((j0 / per_step) % 2 ? kWideReverse : 0));
+            }
             if (!wide) {
                 if (j0 == 0) {
                     if (open_step < 0) {
