@@ -1393,6 +1393,28 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView
 // Stage 2: z[c] = sum of the window's chunk slices, in chunk order
 // (the order of the slices is fixed; the sums inside a slice took the LDS
 // atomics' arrival order); columns outside every window get 0.
+// Column c's sum over the slices of its window: integer counts stay
+// integer; real sums add up in FP64, in chunk order, eight slices loaded
+// ahead of their additions.
+template <class T>
+__device__ __forceinline__ std::conditional_t<std::is_integral_v<T>, T, double>
+reduce_column(const T* __restrict__ part, const ReduceWork& w, int64_t c)
+{
+    using AccT = std::conditional_t<std::is_integral_v<T>, T, double>;
+    const T* __restrict__ p = part + w.part_off + (c - w.col_begin);
+    AccT s = AccT(0);
+    int j = 0;
+    for (; j + 8 <= w.nchunk; j += 8) {
+        T t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = p[int64_t(j + q) * w.stride];
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += AccT(t[q]);
+    }
+    for (; j < w.nchunk; j++) s += AccT(p[int64_t(j) * w.stride]);
+    return s;
+}
+
 template <class T, class ZT>
 __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* __restrict__ work,
                                                               const T* __restrict__ part, ZT* __restrict__ z,
@@ -1400,23 +1422,7 @@ __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* 
 {
     const ReduceWork w = work[blockIdx.x];
     if (!scatter_selected(div, mode, w.sys)) return;
-    // integer counts stay integer; real sums add up in FP64, in chunk order,
-    // eight slices loaded ahead of their additions
-    using AccT = std::conditional_t<std::is_integral_v<T>, T, double>;
-    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
-        const T* __restrict__ p = part + w.part_off + (c - w.col_begin);
-        AccT s = AccT(0);
-        int j = 0;
-        for (; j + 8 <= w.nchunk; j += 8) {
-            T t[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) t[q] = p[int64_t(j + q) * w.stride];
-#pragma unroll
-            for (int q = 0; q < 8; q++) s += AccT(t[q]);
-        }
-        for (; j < w.nchunk; j++) s += AccT(p[int64_t(j) * w.stride]);
-        z[c] = ZT(s);
-    }
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) z[c] = ZT(reduce_column(part, w, c));
 }
 
 // Jacobi of A^T A (Math/MatrixCSR.cpp:21-47): diag_c = sum over rows of the
@@ -1468,24 +1474,29 @@ __global__ void k_colscale(const uint32_t* __restrict__ cnt, int64_t n, double* 
 }
 
 // ------------------------------------------------------------ LSQR cols ----
-// LSQR step 2 (Math/Solver.h:162): vbar = D z - beta v and the block partials
-// of |vbar|^2; k_lsqr_rot finishes alpha and the rotation.  (A grid-wide
+// LSQR step 2 (Math/Solver.h:162), fused with the column reduction of
+// A^T (u / beta): per column z = the slices' sum rounded to VT (as
+// k_scatter_reduce writes it), then vbar = D z - beta v and the block
+// partials of |vbar|^2, one block per reduce item (g: the reduce items of
+// each system); k_lsqr_rot finishes alpha and the rotation.  (A grid-wide
 // last-block ticket here would cost every block an L2 write-back and
 // invalidate on gfx950.)
 template <class VT>
-__global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict__ work,
-                                                      const double* __restrict__ D, const VT* __restrict__ z,
-                                                      double* __restrict__ v, GridSys g, LsqrScalars* scal)
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_reduce_v(const ReduceWork* __restrict__ work,
+                                                             const VT* __restrict__ part,
+                                                             const double* __restrict__ D, double* __restrict__ v,
+                                                             GridSys g, const LsqrScalars* scal)
 {
     __shared__ double sh[kRowBlock / 64];
-    const ColWork w = work[blockIdx.x];
+    const ReduceWork w = work[blockIdx.x];
     const LsqrScalars* st = &scal[w.sys];
     if (!st->active) return;
     const double beta = st->beta;
     double acc = 0.0;
     if (st->stop == 0)
         for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
-            const double vb = D[c] * double(z[c]) - beta * v[c];
+            const double zc = double(VT(reduce_column(part, w, c)));
+            const double vb = D[c] * zc - beta * v[c];
             v[c] = vb;
             acc += vb * vb;
         }
@@ -1494,7 +1505,7 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_v(const ColWork* __restrict_
 }
 
 // The rotation (Math/Solver.h:165-171), one block per system: alpha = |vbar|
-// from k_lsqr_v's partials, then rho, c, s, theta, phi, phibar and the
+// from k_lsqr_reduce_v's partials, then rho, c, s, theta, phi, phibar and the
 // device-side stop.
 __global__ __launch_bounds__(kRowBlock) void k_lsqr_rot(GridSys g, LsqrScalars* scal)
 {

@@ -402,13 +402,32 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
     }
     sp.d_groups = to_device(sp.groups, ctx.stream);
     sp.d_slot_list = to_device(sp.slot_list, ctx.stream);
+    const int nsys = int(sys_rows.size()) - 1;
     for (ScatterLists* L : {&sp.f4, &sp.f8}) {
+        std::stable_sort(L->reduce.begin(), L->reduce.end(),
+                         [](const ReduceWork& a, const ReduceWork& b) { return a.sys < b.sys; });
+        L->red_first.assign(size_t(nsys), 0);
+        L->red_count.assign(size_t(nsys), 0);
+        for (size_t i = L->reduce.size(); i-- > 0;) {
+            L->red_first[size_t(L->reduce[i].sys)] = int32_t(i);
+            L->red_count[size_t(L->reduce[i].sys)]++;
+        }
         L->d_groups = to_device(L->groups, ctx.stream);
         L->d_slot_list = to_device(L->slot_list, ctx.stream);
         L->d_work = to_device(L->work, ctx.stream);
         L->d_reduce = to_device(L->reduce, ctx.stream);
+        L->d_red_first = to_device(L->red_first, ctx.stream);
+        L->d_red_count = to_device(L->red_count, ctx.stream);
+        L->d_red_counters = to_device(std::vector<unsigned>(size_t(std::max(1, nsys)), 0u), ctx.stream);
+        L->d_red_partials = DBuf(2 * sizeof(double) * std::max<size_t>(1, L->reduce.size()));
         L->part = DBuf();
     }
+}
+
+GridSys ScatterLists::reduce_sys() const
+{
+    return GridSys{d_red_first.as<int32_t>(), d_red_count.as<int32_t>(), d_red_counters.as<unsigned>(),
+                   d_red_partials.as<double>()};
 }
 
 // Segments of the tiled gather: the slot groups must be runs of consecutive

@@ -63,6 +63,11 @@ struct ScatterLists {
     int64_t part_elems = 0;  // partial buffer length
     DBuf part;               // partial buffer (allocated on first use)
     int cbits = 0;           // a window column gets fewer than 2^cbits terms per work item
+    // reduce items of each system (the list is ordered by system), for a
+    // reduce that also sums over its columns (block partials per item)
+    std::vector<int32_t> red_first, red_count;
+    DBuf d_red_first, d_red_count, d_red_counters, d_red_partials;
+    GridSys reduce_sys() const;
 };
 
 // Slot groups + per-slot column ranges -> scatter work lists.

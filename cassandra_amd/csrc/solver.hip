@@ -469,10 +469,18 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
     gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s);
-    scatter<VT, VT, VT>(m, ws.u.as<VT>(), sc, ws.z.as<VT>(), s, true);
-    hipLaunchKernelGGL(k_lsqr_v<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
-                       ws.D.as<double>(), ws.z.as<VT>(), ws.v.as<double>(), g.col_sys(), sc);
-    hipLaunchKernelGGL(k_lsqr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, g.col_sys(), sc);
+    // A^T (u / beta): the tiles, then the column reduction fused with vbar =
+    // D z - beta v (u is padded to whole 16-byte groups)
+    ScatterLists& L = lists<VT>(m->splan);
+    with_view(m, [&](auto view) {
+        launch_scatter_tiles<VT, VT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), ws.u.as<VT>(), sc,
+                                     kScatterLsqr, s, true);
+    });
+    if (!L.reduce.empty())
+        hipLaunchKernelGGL(k_lsqr_reduce_v<VT>, dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
+                           L.d_reduce.as<ReduceWork>(), (const VT*)partials<VT>(L), ws.D.as<double>(),
+                           ws.v.as<double>(), L.reduce_sys(), (const LsqrScalars*)sc);
+    hipLaunchKernelGGL(k_lsqr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, L.reduce_sys(), sc);
     hipLaunchKernelGGL(k_lsqr_xw<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
                        ws.D.as<double>(), ws.v.as<double>(), ws.vd.as<VT>(), ws.w.as<double>(), ws.y.as<double>(),
                        (const LsqrScalars*)sc);
