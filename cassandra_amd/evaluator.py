@@ -93,7 +93,8 @@ def huber_fit(patterns: Patterns, idx: torch.Tensor, target: torch.Tensor, epoch
     """Adam on the Huber loss (delta) over minibatches of rows; returns the
     trained evaluator and the per-epoch mean loss."""
     model = PatternEvaluator(patterns)
-    opt = torch.optim.Adam(model.parameters(), lr=lr)
+    # the fused step: one kernel per minibatch instead of a chain of small ones
+    opt = torch.optim.Adam(model.parameters(), lr=lr, fused=True)
     n = idx.shape[0]
     gen = torch.Generator(device=idx.device).manual_seed(seed)
     losses = []
