@@ -118,8 +118,13 @@ const ScatterPlan& eval_plan(cad_patterns* p, int64_t B)
     if (it != p->eval_plans.end()) return *it->second;
     if (p->eval_plans.size() >= 16) p->eval_plans.clear();
     std::unique_ptr<ScatterPlan> sp(new ScatterPlan);
+    // A minibatch has few rows per window: the row chunks are sized to give
+    // every CU a tile (partial slices up to 8x the index bytes; the default
+    // cap of 1x left a 65,536-row minibatch of 4 patterns 28 tiles).
+    TuneParams tp;
+    tp.partial_cap = 8.0;
     build_scatter_plan(*sp, p->slot_lo, p->slot_hi, std::vector<int64_t>{0, B}, std::vector<uint32_t>{0u},
-                       int64_t(p->group->cols()), 4, *p->ctx, TuneParams{});
+                       int64_t(p->group->cols()), 4, *p->ctx, tp);
     const ScatterPlan& ref = *sp;
     p->eval_plans[B] = std::move(sp);
     return ref;
