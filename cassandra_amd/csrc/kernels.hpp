@@ -1168,7 +1168,7 @@ template <int NS, class WT, bool VY, class VT, class YT, class IdxT, int RPT>
 __device__ __forceinline__ void scatter_rows(const SlotMajor<IdxT>& m, const ScatterWork& w, int ns,
                                              const int64_t (&slot)[kMaxSegSlots],
                                              const uint32_t (&off)[kMaxSegSlots], const YT* __restrict__ y,
-                                             VT sc, double fscale, uint32_t W, WT* win)
+                                             VT sc, double fscale, uint32_t W, bool full, WT* win)
 {
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
@@ -1210,13 +1210,27 @@ __device__ __forceinline__ void scatter_rows(const SlotMajor<IdxT>& m, const Sca
             ok[i] = a + i >= w.row_begin && a + i < w.row_end;
             yv[i] = window_term<WT>(double(VT(yr[i]) * sc), fscale);
         }
+        // A wave whose rows all lie inside the chunk, over a group whose
+        // columns all lie in the window, adds without per-lane conditions
+        // (entries past the window, which valid data never holds, are held
+        // inside it rather than skipped)
+        if (__all(full && a >= w.row_begin && a + RPT <= w.row_end)) {
 #pragma unroll
-        for (int j = 0; j < NS; j++) {
-            if (j < ns) {
+            for (int j = 0; j < NS; j++) {
+                if (j < ns) {
 #pragma unroll
-                for (int i = 0; i < RPT; i++) {
-                    const uint32_t uc = off[j] + V::get(q[j], i);
-                    if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
+                    for (int i = 0; i < RPT; i++) atomicAdd(&win[min(off[j] + V::get(q[j], i), W - 1)], yv[i]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                if (j < ns) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) {
+                        const uint32_t uc = off[j] + V::get(q[j], i);
+                        if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
+                    }
                 }
             }
         }
@@ -1269,19 +1283,21 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
             slot[j] = int64_t(sj) * vld;
             off[j] = cb + m.base[sj] - w.win_lo;
         }
+        // every column of the group inside this window
+        const bool full = w.win_lo == cb + gr.col_lo && gr.col_hi - gr.col_lo <= W;
         auto rows_y = [&](auto* wn, auto vy) {
             using WT = std::remove_pointer_t<decltype(wn)>;
             constexpr bool VY = decltype(vy)::value;
             if (ns == 4)
-                scatter_rows<4, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<4, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, full, wn);
             else if (ns == 8)
-                scatter_rows<8, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<8, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, full, wn);
             else if (ns <= 2)
-                scatter_rows<2, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<2, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, full, wn);
             else if (ns <= 4)
-                scatter_rows<4, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<4, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, full, wn);
             else
-                scatter_rows<8, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, wn);
+                scatter_rows<8, WT, VY, VT, YT, IdxT, RPT>(m, w, ns, slot, off, y, sc, fscale, W, full, wn);
         };
         // 16-byte y loads need RPT values to fill them and y padded
         constexpr bool kVecY = RPT * sizeof(YT) % 16 == 0;
