@@ -165,6 +165,24 @@ def test_full_size_properties(ctx):
     assert np.all((idx >= lo) & (idx < hi))
 
 
+def test_full_size_products_bit_exact(ctx, oracle):
+    # the bench's 5M x 139,644 matrix (8 patterns, u16 entries) with the default
+    # schedule: A x in FP32 and FP64 bit-identical to the oracle's slot-order
+    # sums, A^T y exact on integer y
+    say("full size products")
+    P = Patterns(ctx, N.FIT8)
+    pos = ctx.random_play(1002, 5_000_000, 24)
+    A = P.matrix(pos)
+    idx = A.download()
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.randn(A.cols, dtype=torch.float32, device="cuda", generator=gen)
+    assert np.array_equal(A.spmv(x).cpu().numpy(), oracle.spmv(idx, x.cpu().numpy()))
+    x64 = torch.randn(A.cols, dtype=torch.float64, device="cuda", generator=gen)
+    assert np.array_equal(A.spmv(x64).cpu().numpy(), oracle.spmv(idx, x64.cpu().numpy()))
+    y = torch.randint(-64, 65, (A.rows,), device="cuda", generator=gen).float()
+    assert np.array_equal(A.spmtv(y).cpu().numpy(), oracle.spmtv(idx, y.cpu().numpy(), A.cols))
+
+
 def test_reference_golden_vectors(ctx):
     # outputs of the reference's compiled Math sources (tests/golden/make_reference_math.py)
     import os
