@@ -48,6 +48,23 @@ def test_products_bit_identical(ctx, oracle, name, masks):
     assert torch.equal(F.jacobi(), S.jacobi())
 
 
+def test_full_size_matches_stored(ctx):
+    # the bench's matrix-free workload: 20M positions over 15..45 empties, 12
+    # patterns (k = 54); the matrix-free A x bit-identical to the stored one
+    # (FP32 and FP64), A^T y equal on integer y
+    say("matrix-free full size")
+    P = Patterns(ctx, N.FIT12)
+    pos = ctx.random_play(1005, 20_000_000, 15, 45)
+    F = P.matrix_free(pos)
+    S = P.matrix(pos)
+    gen = torch.Generator(device="cuda").manual_seed(13)
+    x = torch.randn(S.cols, dtype=torch.float32, device="cuda", generator=gen)
+    assert torch.equal(F.spmv(x), S.spmv(x))
+    assert torch.equal(F.spmv(x.double()), S.spmv(x.double()))
+    g = torch.randint(-64, 65, (S.rows,), device="cuda", generator=gen).float()
+    assert torch.equal(F.spmtv(g), S.spmtv(g))
+
+
 def colscale(oracle, idx, cols):
     """LSQR's column scaling 1/|a_c| from the exact multiplicity counts."""
     d = oracle.jacobi(idx, cols).astype(np.float64)
