@@ -51,6 +51,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_info():
+    """The host CPU: model name, physical cores and logical CPUs of the
+    machine, and the CPUs this process may run on."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "physical_cores": len(cores) or None, "logical_cpus": os.cpu_count(),
+            "cpus_allowed": allowed}
+
+
+def omp_threads():
+    """OpenMP threads of the CPU baseline: OMP_NUM_THREADS, else the CPUs this
+    process may run on."""
+    v = os.environ.get("OMP_NUM_THREADS", "")
+    if v.isdigit() and int(v) > 0:
+        return int(v)
+    return cpu_info()["cpus_allowed"]
+
+
 def event_time_ms(ctx, fn, reps):
     """Average GPU time of fn() (which enqueues on the library stream)."""
     import torch
@@ -85,8 +124,8 @@ def cpu_baseline(A, b, fp64_iterations, budget_s, x_gpu):
     t0 = time.perf_counter()
     O.lsqr(idx, A.cols, bh, 2, colscale=D)
     per = (time.perf_counter() - t0) / 2
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    out = {"unit": "s", "cores": cores, "kind": "port"}
+    cores = omp_threads()
+    out = {"unit": "s", "cores": cores, "kind": "port", **cpu_info()}
     if per * fp64_iterations <= budget_s:
         t0 = time.perf_counter()
         xc, _, _ = O.lsqr(idx, A.cols, bh, fp64_iterations, colscale=D)
@@ -161,7 +200,7 @@ def cpu_baseline_pcg(idx, cols, bh, x_gpu, iterations):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    out = {"cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    out = {"cores": omp_threads()}
     t0 = time.perf_counter()
     xo, _ = O.pcg(idx, cols, bh, iterations)
     out["cpu_oracle_fp64_seconds"] = time.perf_counter() - t0
@@ -201,9 +240,9 @@ def leg_buckets(ctx):
     A = P.matrix_systems(torch.cat(parts), rb)
     _, b = A.make_targets(2003, 3003)
     log(f"buckets: {A.nsys} systems, {A.rows} rows, k={A.k}, {A.cols} columns")
-    # some small buckets are slow to reach 1e-10 in the normal-residual metric:
-    # both runs stop at the same iteration budget and report what each bucket reached
-    opts = dict(precision=capi.PREC_MIXED, tol=1e-10, max_iterations=4000)
+    # the small buckets (fewer rows than columns) are ill-conditioned and need
+    # thousands of iterations; both runs get the same generous budget
+    opts = dict(precision=capi.PREC_MIXED, tol=1e-10, max_iterations=20000)
     A.solve_systems(b, **dict(opts, max_iterations=50))  # warm: workspace, graph
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -211,6 +250,10 @@ def leg_buckets(ctx):
     ctx.synchronize()
     t_batch = time.perf_counter() - t0
     log(f"buckets batched: {t_batch:.4f} s, {rep}, {int(np.sum(rel <= 1e-10))} buckets at 1e-10")
+    off_tol = [{"system": int(i), "empties": empties[i], "rows": int(rows[i]), "rel": float(rel[i]),
+                "iterations": int(its[i])} for i in np.flatnonzero(rel > 1e-10)]
+    if off_tol:
+        log(f"buckets above 1e-10: {off_tol}")
     # the same fits one system at a time (each warmed first)
     mats = [P.matrix(p_) for p_ in parts]
     del parts
@@ -236,7 +279,7 @@ def leg_buckets(ctx):
             "buckets_at_tol": int(np.sum(rel <= 1e-10)), "max_rel_normal_residual": float(np.max(rel)),
             "median_rel_normal_residual": float(np.median(rel)), "median_iterations": float(np.median(its)),
             "buckets_at_tol_one_by_one": int(np.sum(seq_rel <= 1e-10)),
-            "max_rel_one_by_one": float(np.max(seq_rel))}
+            "max_rel_one_by_one": float(np.max(seq_rel)), "above_tol": off_tol}
 
 
 def leg_matrix_free(ctx):
@@ -293,22 +336,26 @@ def leg_matrix_free(ctx):
 
 
 def leg_evaluator(ctx):
-    """Huber/Adam training throughput and the gather / scatter bandwidth."""
+    """The differentiable evaluator: Huber/Adam training throughput, the
+    forward gather and backward scatter kernels of one minibatch (HIP events
+    around the C-ABI calls alone), the resident evaluator's batch latency, and
+    the Huber fit against the least-squares fit of the same data."""
+    import ctypes as C_
+
     import torch
 
     import cassandra_amd.capi as capi
     from cassandra_amd import names as N
-    from cassandra_amd.evaluator import huber_fit, pattern_eval
+    from cassandra_amd.evaluator import huber_fit
     from cassandra_amd.fit import Patterns
 
     P = Patterns(ctx, N.FIT4)
     n, B = 5_000_000, 65536
     pos = ctx.random_play(1004, n, 30)
     A = P.matrix(pos)
-    _, b = A.make_targets(2004, 3004)
-    del A
+    xt, b = A.make_targets(2004, 3004)
+    truth = A.spmv(xt)  # the noise-free targets A x_true
     idx = P.extract(pos, capi.LAYOUT_ROW_MAJOR)
-    del pos
     target = b.float()
     huber_fit(P, idx[: 4 * B], target[: 4 * B], epochs=1, batch=B)  # warm
     torch.cuda.synchronize()
@@ -317,31 +364,65 @@ def leg_evaluator(ctx):
     torch.cuda.synchronize()
     t_epoch = time.perf_counter() - t0
     log(f"evaluator: epoch {t_epoch:.3f} s, loss {losses}")
-    # one minibatch: forward gather and backward scatter on torch's stream
-    w = torch.randn(P.cols, dtype=torch.float32, device="cuda", requires_grad=True)
+
+    # one minibatch through the C-ABI on the library's stream, kernels alone
+    L = capi.lib()
+    hp = lambda t_: C_.c_void_p(t_.data_ptr())  # noqa: E731
+    w = torch.randn(P.cols, dtype=torch.float32, device="cuda")
     sub = idx[:B].contiguous()
     g = torch.randn(B, dtype=torch.float32, device="cuda")
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = torch.empty(B, dtype=torch.float32, device="cuda")
+    grad = torch.empty(P.cols, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
     reps = 50
-
-    def timed(fn):
-        fn()
-        e0.record()
-        for _ in range(reps):
-            fn()
-        e1.record()
-        e1.synchronize()
-        return e0.elapsed_time(e1) / reps
-
-    with torch.no_grad():
-        fwd_ms = timed(lambda: pattern_eval(w, sub, P))
-    out = pattern_eval(w, sub, P)
-    bwd_ms = timed(lambda: torch.autograd.grad(out, w, g, retain_graph=True))
+    fwd_ms = event_time_ms(ctx, lambda: L.cad_eval_forward_f32(P.h, hp(sub), B, hp(w), hp(out)), reps)
+    bwd_ms = event_time_ms(ctx, lambda: L.cad_eval_backward_f32(P.h, hp(sub), B, hp(g), hp(grad)), reps)
     nbytes = B * (4 * P.k + 4) + 4 * P.cols
+    # the resident evaluator (expanded tables kept on the device): positions
+    # in, scores out -- on device positions, and host batches with the copies
+    E = P.evaluator(w)
+    pos_b = pos[:B].contiguous()
+    res_ms = event_time_ms(ctx, lambda: L.cad_evaluator_eval(E.h, hp(pos_b), B, hp(out)), reps)
+    host_pos = pos_b.cpu().numpy().view("uint64")
+    host_lat = {}
+    for nb in (64, 4096, B):
+        E.eval_host(host_pos[:nb])
+        t0 = time.perf_counter()
+        for _ in range(20):
+            E.eval_host(host_pos[:nb])
+        host_lat[str(nb)] = (time.perf_counter() - t0) / 20 * 1e6
+    del E, pos_b
+
+    # the Huber(4) fit of several epochs against the least-squares fit (mixed
+    # LSQR to 1e-10) of the same rows: errors against the targets and against
+    # the noise-free A x_true
+    epochs = 8
+    model, hl = huber_fit(P, idx, target, epochs=epochs, batch=B)
+    with torch.no_grad():
+        p_h = model(idx).double()
+    x_ls, rep_ls = A.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
+    p_ls = A.spmv(x_ls)
+
+    def quality(p):
+        r = b - p
+        e = truth - p
+        hub = torch.nn.functional.huber_loss(p, b, delta=4.0)
+        return {"rmse_vs_targets": float(torch.sqrt(torch.mean(r * r))),
+                "mae_vs_targets": float(torch.mean(torch.abs(r))), "huber4_vs_targets": float(hub),
+                "rmse_vs_noise_free": float(torch.sqrt(torch.mean(e * e)))}
+
+    fit_cmp = {"huber_adam": dict(quality(p_h), epochs=epochs, lr=0.05, minibatch=B, final_epoch_loss=hl[-1]),
+               "least_squares": dict(quality(p_ls), solver="mixed LSQR", iterations=rep_ls["iterations"],
+                                     rel_normal_residual=rep_ls["rel_normal_residual"])}
+    log(f"evaluator fits: {fit_cmp}")
     return {"positions": n, "minibatch": B, "epoch_seconds": t_epoch, "positions_per_s": n / t_epoch,
-            "loss_epoch": losses[0], "forward_us": fwd_ms * 1e3, "backward_us": bwd_ms * 1e3,
+            "loss_epoch": losses[0], "bytes_per_minibatch": nbytes,
+            "kernel_us": {"forward": fwd_ms * 1e3, "backward": bwd_ms * 1e3,
+                          "resident_eval_positions": res_ms * 1e3},
             "forward_GBps": nbytes / (fwd_ms * 1e-3) / 1e9, "backward_GBps": nbytes / (bwd_ms * 1e-3) / 1e9,
-            "bytes_per_minibatch": nbytes}
+            "resident_eval_positions_per_s": B / (res_ms * 1e-3),
+            "resident_eval_host_batch_latency_us": host_lat,
+            "fit_comparison": fit_cmp}
 
 
 def traffic_launches(ctx, A):
@@ -414,8 +495,24 @@ def main():
     P = Patterns(ctx, N.FIT8)
     A = P.matrix(pos)
     xt, b = A.make_targets(2002, 3002)
-    del pos
     log(f"data: {A.rows} rows, k={A.k}, cols={A.cols}, index bytes {A.index_bytes} ({time.perf_counter() - t:.1f} s)")
+    # index extraction (GroupIndexer::InsertIndices over every row,
+    # PatternFit/PatternFit.cpp:19-21): positions -> u32 slot-major indices
+    extract = None
+    if not args.traffic:
+        L = capi.lib()
+        out_idx = torch.empty(A.rows * A.k, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        ex_ms = event_time_ms(ctx, lambda: L.cad_index_extract(P.h, C.c_void_p(pos.data_ptr()), A.rows,
+                                                             C.c_void_p(out_idx.data_ptr()),
+                                                             capi.LAYOUT_SLOT_MAJOR), 10)
+        ex_bytes = A.rows * (16 + 4 * A.k)
+        extract = {"kernel": "k_index_extract (u32 slot-major)", "launch_us": ex_ms * 1e3,
+                   "algorithmic_bytes": ex_bytes, "achieved_GBps": ex_bytes / (ex_ms * 1e-3) / 1e9,
+                   "frac": ex_bytes / (ex_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}
+        log(f"index extraction: {extract}")
+        del out_idx
+    del pos
     t = time.perf_counter()
     tune_log = A.autotune() if args.autotune else ""
     log(f"autotune ({time.perf_counter() - t:.2f} s):\n{tune_log}")
@@ -482,6 +579,11 @@ def main():
     # reduction are overhead, not algorithm).
     kt = A.kernel_times(args.reps)
     log(f"kernel times (us): {kt}")
+    # the Jacobi count pass (JacobiPreconditionerOfATA): the index stream once
+    jac_bytes = A.rows * ib * A.k + 4 * A.cols
+    jacobi = {"kernel": "k_jacobi_part", "launch_us": kt["jacobi_part"], "algorithmic_bytes": jac_bytes,
+              "achieved_GBps": jac_bytes / (kt["jacobi_part"] * 1e-6) / 1e9,
+              "frac": jac_bytes / (kt["jacobi_part"] * 1e-6) / 1e9 / HBM_PEAK_GBPS}
     cands = [("k_gather_tile (A x)", kt["gather"]), ("k_scatter_tile (A^T y windows)", kt["scatter_tiles"])]
     dom_name, dom_us = max(cands, key=lambda c: c[1])
     dom_gbps = bytes_pass / (dom_us * 1e-6) / 1e9
@@ -525,6 +627,8 @@ def main():
         "fp64_lsqr": None if fp64 is None else {"seconds": fp64["seconds"], "iterations": fp64["iterations"],
                                                  "rel_normal_residual": fp64["rel_normal_residual"]},
         "speedup_vs_fp64": None if fp64 is None else fp64["seconds"] / t_mixed,
+        "index_extract": extract,
+        "jacobi": jacobi,
         "kernels": {"spmv_us": spmv_ms * 1e3, "spmv_GBps": spmv_gbps, "spmtv_us": spmtv_ms * 1e3,
                     "spmtv_GBps": spmtv_gbps, "bytes_per_pass": bytes_pass, "index_bytes": ib,
                     "kernel_us": kt,

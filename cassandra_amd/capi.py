@@ -109,6 +109,14 @@ SIGNATURES = {
     "cad_eval_forward_f64": (C.c_int, [vp, vp, i64, vp, vp]),
     "cad_eval_backward_f32": (C.c_int, [vp, vp, i64, vp, vp]),
     "cad_eval_backward_f64": (C.c_int, [vp, vp, i64, vp, vp]),
+    "cad_patterns_expanded_size": (C.c_int, [vp, C.POINTER(i64)]),
+    "cad_weights_expand": (C.c_int, [vp, vp, vp]),
+    "cad_weights_compact": (C.c_int, [vp, vp, vp]),
+    "cad_evaluator_create": (C.c_int, [vp, vp, PP]),
+    "cad_evaluator_destroy": (C.c_int, [vp]),
+    "cad_evaluator_set_weights": (C.c_int, [vp, vp]),
+    "cad_evaluator_eval": (C.c_int, [vp, vp, i64, vp]),
+    "cad_evaluator_eval_host": (C.c_int, [vp, vp, i64, vp]),
     "cad_random_play": (C.c_int, [vp, u64, i64, C.c_int, C.c_int, vp]),
     "cad_make_targets": (C.c_int, [vp, u64, u64, C.c_double, C.c_double, vp, vp]),
     "cad_flip": (u64, [C.c_int, u64]),

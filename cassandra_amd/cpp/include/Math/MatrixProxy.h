@@ -2,7 +2,7 @@
 // Unlike the reference, a proxy keeps proxies it is built from by value (and
 // real matrices by reference), so `transposed(A) * A` stays valid after the
 // full expression that created it -- the reference's PCG keeps a reference to
-// that temporary (PatternFit/PatternFit.cpp:55, Math/MatrixProxy.h:166-169).
+// that temporary (PatternFit/PatternFit.cpp:31, Math/MatrixProxy.h:48-49).
 #pragma once
 #include <cstddef>
 #include <stdexcept>

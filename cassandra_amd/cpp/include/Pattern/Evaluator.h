@@ -1,9 +1,11 @@
 // Drop-in for Pattern/Evaluator.h:9-68.  ScoreEstimator expands the compact
 // weights into per-variation tables and sums them per position, as the
-// reference does; EvalBatch scores many positions on the GPU from the compact
-// weights (same terms, same order: the same floats).
+// reference does; EvalBatch scores many positions on the GPU from the same
+// tables, expanded once on the device and kept there (same terms, same order:
+// the same floats).
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <span>
 #include <vector>
 
@@ -19,6 +21,8 @@ class ScoreEstimator
     std::vector<uint64_t> pattern;            // transformed masks
     std::vector<uint64_t> input_pattern;      // masks as given
     std::vector<float> compact;               // compact weights (for EvalBatch)
+    // the device evaluator of EvalBatch, created on its first call and shared by copies
+    mutable std::shared_ptr<struct ScoreEstimatorDevice> device;
 
 public:
     ScoreEstimator(std::vector<uint64_t> pattern, std::span<const float> weights);

@@ -2,6 +2,7 @@
 #include "Pattern/Evaluator.h"
 
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 
 #include "Pattern/Helpers.h"
@@ -68,26 +69,47 @@ std::vector<float> ScoreEstimator::Weights() const
     return out;
 }
 
+// The pattern group and the expanded tables of one ScoreEstimator on the
+// device; calls on one estimator take turns (its staging buffers are shared).
+struct ScoreEstimatorDevice {
+    std::mutex mu;
+    cad_patterns* patterns = nullptr;
+    cad_evaluator* eval = nullptr;
+    ~ScoreEstimatorDevice()
+    {
+        if (eval) cad_evaluator_destroy(eval);
+        if (patterns) cad_patterns_destroy(patterns);
+    }
+};
+
 std::vector<float> ScoreEstimator::EvalBatch(std::span<const Position> pos) const
 {
     std::vector<float> out(pos.size());
     if (pos.empty()) return out;
-    cad_context* ctx = Context();
-    cad_patterns* p = nullptr;
-    Check(cad_patterns_create(ctx, input_pattern.data(), int(input_pattern.size()), &p));
-    std::unique_ptr<cad_patterns, int (*)(cad_patterns*)> guard(p, cad_patterns_destroy);
-    int k = 0;
-    int64_t cols = 0;
-    int np = 0;
-    Check(cad_patterns_info(p, &k, &cols, &np));
-    const std::size_t n = pos.size();
-    DevBuf dpos(16 * n), didx(sizeof(int32_t) * n * std::size_t(k)), dw(sizeof(float) * std::size_t(cols)),
-        dout(sizeof(float) * n);
-    Check(cad_memcpy_h2d(ctx, dpos.as<void>(), pos.data(), 16 * n));
-    Check(cad_memcpy_h2d(ctx, dw.as<void>(), compact.data(), sizeof(float) * std::size_t(cols)));
-    Check(cad_index_extract(p, dpos.as<uint64_t>(), int64_t(n), didx.as<uint32_t>(), CAD_LAYOUT_ROW_MAJOR));
-    Check(cad_eval_forward_f32(p, didx.as<int32_t>(), int64_t(n), dw.as<float>(), dout.as<float>()));
-    Check(cad_memcpy_d2h(ctx, out.data(), dout.as<void>(), sizeof(float) * n));
+    std::shared_ptr<ScoreEstimatorDevice> d;
+    {
+        static std::mutex create;
+        std::lock_guard<std::mutex> lock(create);
+        if (!device) {
+            auto nd = std::make_shared<ScoreEstimatorDevice>();
+            cad_context* ctx = Context();
+            Check(cad_patterns_create(ctx, input_pattern.data(), int(input_pattern.size()), &nd->patterns));
+            int k = 0, np = 0;
+            int64_t cols = 0;
+            Check(cad_patterns_info(nd->patterns, &k, &cols, &np));
+            if (compact.size() < std::size_t(cols)) throw std::runtime_error("Size mismatch.");
+            DevBuf dw(sizeof(float) * std::size_t(cols));
+            Check(cad_memcpy_h2d(ctx, dw.as<void>(), compact.data(), sizeof(float) * std::size_t(cols)));
+            Check(cad_evaluator_create(nd->patterns, dw.as<float>(), &nd->eval));
+            Check(cad_context_synchronize(ctx));  // the expansion has read dw
+            device = std::move(nd);
+        }
+        d = device;
+    }
+    std::lock_guard<std::mutex> lock(d->mu);
+    static_assert(sizeof(Position) == 16, "Position is {P, O}");
+    Check(cad_evaluator_eval_host(d->eval, reinterpret_cast<const uint64_t*>(pos.data()), int64_t(pos.size()),
+                                  out.data()));
     return out;
 }
 

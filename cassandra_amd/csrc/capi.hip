@@ -4,6 +4,7 @@
 #include <string>
 #include <vector>
 
+#include "evaluator.hpp"
 #include "kernels.hpp"
 #include "solver.hpp"
 
@@ -775,6 +776,96 @@ int cad_eval_backward_f64(cad_patterns* p, const int32_t* idx, int64_t B, const 
         need(p && grad && B >= 0 && (B == 0 || (idx && g)), "invalid argument");
         need(p->ctx != nullptr, "host-only pattern group (created without a context)");
         eval_backward<double>(p, idx, B, g, grad);
+    });
+}
+
+// ------------------------------------------------------------ tables ------
+int cad_patterns_expanded_size(cad_patterns* p, int64_t* n)
+{
+    return guarded([&] {
+        need(p && n, "null argument");
+        int64_t total = 0;
+        for (const auto& pi : p->group->patterns)
+            total += int64_t(pow3(__builtin_popcountll(pi.mask))) * int64_t(pi.variations.size());
+        *n = total;
+    });
+}
+
+int cad_weights_expand(cad_patterns* p, const float* w, float* tables)
+{
+    return guarded([&] {
+        need(p && w && tables, "null argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        weights_expand(p, w, tables);
+    });
+}
+
+int cad_weights_compact(cad_patterns* p, const float* tables, float* w)
+{
+    return guarded([&] {
+        need(p && w && tables, "null argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        for (const auto& pi : p->group->patterns)
+            need(__builtin_popcountll(pi.mask) <= 15, "compaction takes patterns of at most 15 squares");
+        weights_compact(p, tables, w);
+    });
+}
+
+int cad_evaluator_create(cad_patterns* p, const float* w, cad_evaluator** out)
+{
+    return guarded([&] {
+        need(p && w && out, "null argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        std::unique_ptr<cad_evaluator> e(new cad_evaluator);
+        e->ctx = p->ctx;
+        e->patterns = p;
+        e->tables = DBuf(sizeof(float) * size_t(std::max<int64_t>(1, expanded_layout(p).size)));
+        weights_expand(p, w, e->tables.as<float>());
+        *out = e.release();
+    });
+}
+
+int cad_evaluator_destroy(cad_evaluator* e)
+{
+    return guarded([&] {
+        if (!e) return;
+        (void)hipStreamSynchronize(e->ctx->stream);
+        delete e;
+    });
+}
+
+int cad_evaluator_set_weights(cad_evaluator* e, const float* w)
+{
+    return guarded([&] {
+        need(e && w, "null argument");
+        weights_expand(e->patterns, w, e->tables.as<float>());
+    });
+}
+
+int cad_evaluator_eval(cad_evaluator* e, const uint64_t* pos, int64_t n, float* out)
+{
+    return guarded([&] {
+        need(e && n >= 0 && (n == 0 || (pos && out)), "invalid argument");
+        eval_tables(e->patterns, e->tables.as<float>(), pos, n, out);
+    });
+}
+
+int cad_evaluator_eval_host(cad_evaluator* e, const uint64_t* pos, int64_t n, float* out)
+{
+    return guarded([&] {
+        need(e && n >= 0 && (n == 0 || (pos && out)), "invalid argument");
+        if (n == 0) return;
+        hipStream_t s = e->ctx->stream;
+        if (n > e->staged) {
+            CAD_HIP(hipStreamSynchronize(s));
+            e->d_pos = DBuf(sizeof(uint64_t) * 2 * size_t(n));
+            e->d_out = DBuf(sizeof(float) * size_t(n));
+            e->staged = n;
+        }
+        CAD_HIP(hipMemcpyAsync(e->d_pos.p, pos, sizeof(uint64_t) * 2 * size_t(n), hipMemcpyHostToDevice, s));
+        eval_tables(e->patterns, e->tables.as<float>(), e->d_pos.as<uint64_t>(), n, e->d_out.as<float>());
+        CAD_HIP(hipMemcpyAsync(out, e->d_out.p, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, s));
+        CAD_HIP(hipStreamSynchronize(s));
     });
 }
 

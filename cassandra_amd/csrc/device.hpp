@@ -115,6 +115,17 @@ struct SlotGroup {
     uint16_t nslots, first;   // slots: slot_list[first .. first+nslots)
 };
 
+// One variation of the expanded-table evaluator: its mask's squares in
+// ascending order (FastIndex digits), 3^n, and where its table starts.
+struct EvalSlot {
+    uint8_t sq[16];
+    uint8_t n;
+    uint8_t pad[3];
+    uint32_t n3;
+    uint64_t table_offset;
+};
+static_assert(sizeof(EvalSlot) == 32, "EvalSlot layout");
+
 // Per-system state of the device-resident LSQR recurrence (FP64 scalars).
 struct LsqrScalars {
     double alpha, beta;       // current bidiagonalisation scalars

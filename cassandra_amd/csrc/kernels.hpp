@@ -471,14 +471,19 @@ struct IdxVec<uint32_t, 4> {
 // barriers before and after a staging keep the LDS accesses in order around
 // it.  Declared without a memory clobber: one would mark every later load of
 // the kernel's read-only tables as possibly overwritten and turn their scalar
-// loads into vector loads with full vmcnt waits.  No code in these kernels
-// keeps a value in M0 across it (checked in the ISA).
+// loads into vector loads with full vmcnt waits.  M0 (the LDS address of
+// the load) is declared clobbered, so the compiler assumes no value of its
+// own survives in it across the asm (clang warns that a reserved register is
+// not saved for the caller, which is not wanted here).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void load_line_to_lds(const void* src, void* lds)
 {
     const uint32_t la = __builtin_amdgcn_readfirstlane(
         uint32_t(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) void*)lds)));
-    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la), "v"(src));
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(la), "v"(src) : "m0");
 }
+#pragma clang diagnostic pop
 
 // Copies the W elements at src into LDS at dst + h, h = the element offset of
 // src inside its 16-byte line (dst is 16-byte aligned), and returns h: the
@@ -1442,9 +1447,10 @@ __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* 
 }
 
 // Jacobi of A^T A (Math/MatrixCSR.cpp:21-47): diag_c = sum over rows of the
-// squared multiplicity of c in the row.  Each (row, slot) adds the multiplicity
-// of its column inside the slot group (repeats can only occur within a group),
-// which sums to mult^2 per row.  Counts are exact 32-bit integers.
+// squared multiplicity of c in the row.  A row's columns of one slot group
+// (repeats can only occur within a group, of at most kMaxSegSlots slots) are
+// loaded once into registers; the first slot holding a column adds mult^2.
+// Counts are exact 32-bit integers.
 template <class View>
 __global__ __launch_bounds__(kScatterBlock) void k_jacobi_part(View m, const ScatterWork* __restrict__ work,
                                                                const SlotGroup* __restrict__ groups,
@@ -1460,14 +1466,23 @@ __global__ __launch_bounds__(kScatterBlock) void k_jacobi_part(View m, const Sca
     __syncthreads();
     const SlotGroup gr = groups[w.group];
     const uint32_t cb = sys_col[w.sys];
+    const int ns = gr.nslots;
     for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kScatterBlock) {
-        for (int j = 0; j < gr.nslots; j++) {
-            const uint32_t c = cb + m.col(slot_list[gr.first + j], r);
-            const uint32_t uc = c - w.win_lo;
-            if (uc >= W) continue;
+        uint32_t c[kMaxSegSlots];
+#pragma unroll
+        for (int j = 0; j < kMaxSegSlots; j++) c[j] = j < ns ? cb + m.col(slot_list[gr.first + j], r) : 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < kMaxSegSlots; j++) {
+            const uint32_t uc = c[j] - w.win_lo;
+            if (j >= ns || uc >= W) continue;
             uint32_t mult = 0;
-            for (int t = 0; t < gr.nslots; t++) mult += (cb + m.col(slot_list[gr.first + t], r)) == c;
-            atomicAdd(&win[uc], mult);
+            bool first = true;
+#pragma unroll
+            for (int t = 0; t < kMaxSegSlots; t++) {
+                mult += c[t] == c[j] ? 1u : 0u;
+                if (t < j && c[t] == c[j]) first = false;
+            }
+            if (first) atomicAdd(&win[uc], mult * mult);
         }
     }
     __syncthreads();
@@ -1654,17 +1669,34 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_init_w(const ColWork* __rest
     }
 }
 
-// x += D y (back to the unscaled space), y = 0.
+// x += D y (back to the unscaled space), y = 0, keeping the x before in
+// xprev; for the selected systems (active and restart).
 __global__ __launch_bounds__(kRowBlock) void k_lsqr_apply(const ColWork* __restrict__ work,
                                                           const double* __restrict__ D, double* __restrict__ y,
-                                                          double* __restrict__ x, const LsqrScalars* scal)
+                                                          double* __restrict__ x, double* __restrict__ xprev,
+                                                          const LsqrScalars* scal)
 {
     const ColWork w = work[blockIdx.x];
     if (!scal[w.sys].active || !scal[w.sys].restart) return;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
-        x[c] += D[c] * y[c];
+        const double xc = x[c];
+        xprev[c] = xc;
+        x[c] = xc + D[c] * y[c];
         y[c] = 0.0;
     }
+}
+
+// Undoes the last k_lsqr_apply of the selected systems: x = xprev.
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_undo(const ColWork* __restrict__ work,
+                                                         const double* __restrict__ D, double* __restrict__ y,
+                                                         double* __restrict__ x, double* __restrict__ xprev,
+                                                         const LsqrScalars* scal)
+{
+    (void)D;
+    (void)y;
+    const ColWork w = work[blockIdx.x];
+    if (!scal[w.sys].active || !scal[w.sys].restart) return;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) x[c] = xprev[c];
 }
 
 // |z|^2 per system into reduce_b (used for |A^T b| and |A^T r|).

@@ -131,6 +131,15 @@ inline int64_t padded_rows(int64_t rows) { return (rows + kIndexAlign - 1) / kIn
 
 struct Workspace;
 
+// Expanded tables of a pattern group: one table of 3^n floats per variation,
+// variation after variation in slot order (ScoreEstimator's w[i]).
+struct ExpandedLayout {
+    std::vector<EvalSlot> slots;
+    std::vector<int32_t> first_slot;  // first slot (variation 0) of each pattern
+    int64_t size = 0;                 // floats of all tables
+    DBuf d_slots, d_first_slot;
+};
+
 struct Context {
     int device = 0;
     hipStream_t own = nullptr;
@@ -152,6 +161,17 @@ struct cad_patterns {
     // scatter plans of the evaluator backward, one per minibatch size
     std::map<int64_t, std::unique_ptr<cad::ScatterPlan>> eval_plans;
     cad::DBuf d_base0, d_syscol0;            // zeros
+    std::unique_ptr<cad::ExpandedLayout> layout;  // expanded tables (built on first use)
+};
+
+// A resident evaluator: the expanded tables of one weight vector in HBM and
+// staging for host batches (ScoreEstimator on the device).
+struct cad_evaluator {
+    cad_context* ctx = nullptr;        // the pattern group's context
+    cad_patterns* patterns = nullptr;
+    cad::DBuf tables;
+    cad::DBuf d_pos, d_out;     // staging of cad_evaluator_eval_host
+    int64_t staged = 0;         // rows the staging holds
 };
 
 struct cad_matrix {

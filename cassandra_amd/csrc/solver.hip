@@ -124,10 +124,13 @@ void launch_absmax(ScatterPlan& sp, const YT* y, const GridPlan& g, int nsys, hi
 
 template <class VT, class YT, class View>
 void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
-                          const LsqrScalars* div, int mode, hipStream_t s, bool y_padded = false)
+                          const LsqrScalars* div, int mode, hipStream_t s, bool y_padded = false,
+                          const ActiveLists* al = nullptr)
 {
     ScatterLists& L = lists<VT>(sp);
-    if (L.work.empty()) return;
+    const ScatterWork* work = al ? al->scatter.as<ScatterWork>() : L.d_work.as<ScatterWork>();
+    const int nwork = al ? al->nscatter : int(L.work.size());
+    if (nwork == 0) return;
     // the fixed-point windows read max |y| per system outside an LSQR
     // iteration, whose terms are bounded by construction
     const FixedScale fx{fixed_point_scatter<VT, View>() && mode != kScatterLsqr ? sp.d_ymax.as<unsigned>() : nullptr,
@@ -135,9 +138,9 @@ void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint
     const size_t lds = size_t(L.widest) * sizeof(double);
     with_scatter_kernel<VT, YT, View>(rpt, [&](auto kern) {
         allow_lds(kern, lds);
-        hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
-                           L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(),
-                           sys_col, y, div, mode, fx, int(y_padded), partials<VT>(L));
+        hipLaunchKernelGGL(kern, dim3(unsigned(nwork)), dim3(kScatterBlock), lds, s, view, work,
+                           L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(), sys_col, y, div, mode, fx,
+                           int(y_padded), partials<VT>(L));
     });
     CAD_HIP(hipGetLastError());
 }
@@ -251,55 +254,71 @@ constexpr bool sub_tiles_ok(int rpt)
     return rpt * int(sizeof(VT)) == 8 && rpt * int(sizeof(IdxT)) <= 4;
 }
 
-// Row gather with an epilogue: the LDS-staged tile kernel for pattern
-// matrices, the generic row kernel otherwise.
-template <class VT, class Epi>
-void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s)
+// The row blocks a gather launches: the tile plan's or the grid plan's (all
+// systems), or those of an ActiveLists built for this value size.
+struct RowList {
+    const RowWork* p;
+    int n;
+    GridSys sys;
+};
+
+template <class VT>
+RowList row_list(cad_matrix* m, const ActiveLists* al)
 {
     GatherPlan& gp = gather_plan<VT>(m);
+    if (al) {
+        const GridSys base = gp.ok ? gp.sys() : m->grid.row_sys();
+        return RowList{al->tiles.as<RowWork>(), al->ntiles,
+                       GridSys{al->tfirst.as<int32_t>(), al->tcount.as<int32_t>(), base.counters, base.partials}};
+    }
+    if (gp.ok) return RowList{gp.d_tiles.as<RowWork>(), int(gp.tiles.size()), gp.sys()};
+    return RowList{m->grid.d_rows.as<RowWork>(), int(m->grid.rows.size()), m->grid.row_sys()};
+}
+
+// Row gather with an epilogue: the LDS-staged tile kernel for pattern
+// matrices, the generic row kernel otherwise.  al: launch only the blocks of
+// the systems it holds.
+template <class VT, class Epi>
+void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s, const ActiveLists* al = nullptr)
+{
+    GatherPlan& gp = gather_plan<VT>(m);
+    const RowList rl = row_list<VT>(m, al);
+    if (rl.n == 0) return;
     with_view(m, [&](auto view) {
         using View = decltype(view);
         using IdxT = typename View::index_type;
         if (gp.ok) {
-            if (gp.tiles.empty()) return;
             const size_t lds = size_t(gp.lds_elems) * sizeof(VT);
             with_rpt<VT, View>(gp.rpt, [&](auto rc) {
                 constexpr int RPT = decltype(rc)::value;
                 if constexpr (std::is_void_v<IdxT>) {
                     auto kern = k_gather_tile_mf<VT, RPT, Epi>;
                     allow_lds(kern, lds);
-                    hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(kTileBlock), lds, s, view,
-                                       gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
-                                       gp.d_segs.as<GatherSeg>(), int(gp.segs.size()), gp.wmax, x, epi, gp.sys());
+                    hipLaunchKernelGGL(kern, dim3(unsigned(rl.n)), dim3(kTileBlock), lds, s, view, rl.p,
+                                       m->d_sys_col.as<uint32_t>(), gp.d_segs.as<GatherSeg>(), int(gp.segs.size()),
+                                       gp.wmax, x, epi, rl.sys);
                 } else {
                     with_tile_shape<sub_tiles_ok<VT, IdxT>(RPT)>(gp.bs, gp.sub, [&](auto bc, auto sc) {
                         constexpr int BS = decltype(bc)::value;
                         auto kern = k_gather_tile<VT, RPT, IdxT, Epi, BS, decltype(sc)::value>;
                         allow_lds(kern, lds);
-                        hipLaunchKernelGGL(kern, dim3(unsigned(gp.tiles.size())), dim3(BS), lds, s, view,
-                                           gp.d_tiles.as<RowWork>(), m->d_sys_col.as<uint32_t>(),
-                                           gp.d_steps.as<GatherStep>(), int(gp.steps.size()),
-                                           gp.d_phases.as<PhaseSeg>(), gp.wmax, x, epi, gp.sys());
+                        hipLaunchKernelGGL(kern, dim3(unsigned(rl.n)), dim3(BS), lds, s, view, rl.p,
+                                           m->d_sys_col.as<uint32_t>(), gp.d_steps.as<GatherStep>(),
+                                           int(gp.steps.size()), gp.d_phases.as<PhaseSeg>(), gp.wmax, x, epi,
+                                           rl.sys);
                     });
                 }
             });
         } else {
-            const GridPlan& g = m->grid;
-            if (g.rows.empty()) return;
-            hipLaunchKernelGGL((k_gather_rows<VT, decltype(view), Epi>), dim3(unsigned(g.rows.size())),
-                               dim3(kRowBlock), 0, s, view, m->k, g.d_rows.as<RowWork>(),
-                               m->d_sys_col.as<uint32_t>(), x, epi, g.row_sys());
+            hipLaunchKernelGGL((k_gather_rows<VT, decltype(view), Epi>), dim3(unsigned(rl.n)), dim3(kRowBlock), 0,
+                               s, view, m->k, rl.p, m->d_sys_col.as<uint32_t>(), x, epi, rl.sys);
         }
     });
     CAD_HIP(hipGetLastError());
     // the epilogue's per-system finish, from the block partials the gather left
     if constexpr (finish_kernel<Epi>()) {
-        const bool empty = gp.ok ? gp.tiles.empty() : m->grid.rows.empty();
-        if (!empty) {
-            hipLaunchKernelGGL(k_epi_finish<Epi>, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, epi,
-                               gp.ok ? gp.sys() : m->grid.row_sys());
-            CAD_HIP(hipGetLastError());
-        }
+        hipLaunchKernelGGL(k_epi_finish<Epi>, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, epi, rl.sys);
+        CAD_HIP(hipGetLastError());
     }
 }
 
@@ -328,6 +347,24 @@ void matrix_targets(cad_matrix* m, const double* xt, uint64_t seed, double sn, d
     gather<double>(m, xt, EpiTargets{b, seed, sn}, m->ctx->stream);
 }
 
+// The first stage of the Jacobi counts: u32 multiplicity-squared windows
+// into the partial slices of the 4-byte scatter lists.
+static void launch_jacobi_part(cad_matrix* m, hipStream_t s)
+{
+    ScatterLists& L = m->splan.f4;
+    uint32_t* part = partials<uint32_t>(L);
+    if (L.work.empty()) return;
+    const size_t lds = size_t(L.widest) * 4;
+    with_view(m, [&](auto view) {
+        auto kern = k_jacobi_part<decltype(view)>;
+        allow_lds(kern, lds);
+        hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
+                           L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(),
+                           m->d_sys_col.as<uint32_t>(), part);
+    });
+    CAD_HIP(hipGetLastError());
+}
+
 const uint32_t* jacobi_counts(cad_matrix* m)
 {
     if (m->have_cnt) return m->d_jacobi_cnt.as<uint32_t>();
@@ -336,17 +373,7 @@ const uint32_t* jacobi_counts(cad_matrix* m)
     ScatterPlan& sp = m->splan;
     ScatterLists& L = sp.f4;
     uint32_t* part = partials<uint32_t>(L);
-    if (!L.work.empty()) {
-        const size_t lds = size_t(L.widest) * 4;
-        with_view(m, [&](auto view) {
-            auto kern = k_jacobi_part<decltype(view)>;
-            allow_lds(kern, lds);
-            hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s, view,
-                               L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(),
-                               L.d_slot_list.as<uint16_t>(), m->d_sys_col.as<uint32_t>(), part);
-        });
-        CAD_HIP(hipGetLastError());
-    }
+    launch_jacobi_part(m, s);
     hipLaunchKernelGGL((k_scatter_reduce<uint32_t, uint32_t>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0,
                        s, L.d_reduce.as<ReduceWork>(), (const uint32_t*)part, m->d_jacobi_cnt.as<uint32_t>(),
                        (const LsqrScalars*)nullptr, kScatterAll);
@@ -464,27 +491,105 @@ static std::vector<double> atb_norms(cad_matrix* m, Workspace& ws, const double*
 
 // ------------------------------------------------------------ LSQR --------
 template <class VT>
-static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
+static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s, const ActiveLists* al = nullptr)
 {
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
-    gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s);
+    gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s, al);
     // A^T (u / beta): the tiles, then the column reduction fused with vbar =
     // D z - beta v (u is padded to whole 16-byte groups)
     ScatterLists& L = lists<VT>(m->splan);
     with_view(m, [&](auto view) {
         launch_scatter_tiles<VT, VT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), ws.u.as<VT>(), sc,
-                                     kScatterLsqr, s, true);
+                                     kScatterLsqr, s, true, al);
     });
-    if (!L.reduce.empty())
-        hipLaunchKernelGGL(k_lsqr_reduce_v<VT>, dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
-                           L.d_reduce.as<ReduceWork>(), (const VT*)partials<VT>(L), ws.D.as<double>(),
-                           ws.v.as<double>(), L.reduce_sys(), (const LsqrScalars*)sc);
-    hipLaunchKernelGGL(k_lsqr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, L.reduce_sys(), sc);
-    hipLaunchKernelGGL(k_lsqr_xw<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
-                       ws.D.as<double>(), ws.v.as<double>(), ws.vd.as<VT>(), ws.w.as<double>(), ws.y.as<double>(),
-                       (const LsqrScalars*)sc);
+    const GridSys rsys = al ? GridSys{al->rfirst.as<int32_t>(), al->rcount.as<int32_t>(),
+                                      L.reduce_sys().counters, L.reduce_sys().partials}
+                            : L.reduce_sys();
+    const ReduceWork* red = al ? al->reduce.as<ReduceWork>() : L.d_reduce.as<ReduceWork>();
+    const int nred = al ? al->nreduce : int(L.reduce.size());
+    if (nred > 0)
+        hipLaunchKernelGGL(k_lsqr_reduce_v<VT>, dim3(unsigned(nred)), dim3(kRowBlock), 0, s, red,
+                           (const VT*)partials<VT>(L), ws.D.as<double>(), ws.v.as<double>(), rsys,
+                           (const LsqrScalars*)sc);
+    hipLaunchKernelGGL(k_lsqr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, rsys, sc);
+    const ColWork* cw = al ? al->cols.as<ColWork>() : g.d_cols.as<ColWork>();
+    const int ncw = al ? al->ncols : int(g.cols.size());
+    if (ncw > 0)
+        hipLaunchKernelGGL(k_lsqr_xw<VT>, dim3(unsigned(ncw)), dim3(kRowBlock), 0, s, cw, ws.D.as<double>(),
+                           ws.v.as<double>(), ws.vd.as<VT>(), ws.w.as<double>(), ws.y.as<double>(),
+                           (const LsqrScalars*)sc);
     CAD_HIP(hipGetLastError());
+}
+
+// The iteration lists of the systems in `on` (see ActiveLists).
+template <class VT>
+static void build_active(cad_matrix* m, ActiveLists& al, const std::vector<int>& on, hipStream_t s)
+{
+    const int n = m->nsys;
+    GatherPlan& gp = gather_plan<VT>(m);
+    const std::vector<RowWork>& tiles = gp.ok ? gp.tiles : m->grid.rows;
+    const std::vector<int32_t>& tf = gp.ok ? gp.first : m->grid.row_first;
+    const std::vector<int32_t>& tc = gp.ok ? gp.count : m->grid.row_count;
+    ScatterLists& L = lists<VT>(m->splan);
+    std::vector<RowWork> t;
+    std::vector<int32_t> t_first(n, 0), t_count(n, 0), r_first(n, 0), r_count(n, 0);
+    std::vector<ScatterWork> sw;
+    std::vector<ReduceWork> rw;
+    std::vector<ColWork> cw;
+    al.on.assign(n, 0);
+    al.rows = 0;
+    for (int i = 0; i < n; i++) {
+        if (!on[i]) continue;
+        al.on[i] = 1;
+        al.rows += m->sys_row_begin[i + 1] - m->sys_row_begin[i];
+        t_first[i] = int32_t(t.size());
+        t_count[i] = tc[i];
+        t.insert(t.end(), tiles.begin() + tf[i], tiles.begin() + tf[i] + tc[i]);
+        r_first[i] = int32_t(rw.size());
+        r_count[i] = L.red_count[i];
+        rw.insert(rw.end(), L.reduce.begin() + L.red_first[i], L.reduce.begin() + L.red_first[i] + L.red_count[i]);
+        cw.insert(cw.end(), m->grid.cols.begin() + m->grid.col_first[i],
+                  m->grid.cols.begin() + m->grid.col_first[i] + m->grid.col_count[i]);
+    }
+    for (const ScatterWork& w : L.work)
+        if (on[w.sys]) sw.push_back(w);
+    al.ntiles = int(t.size());
+    al.nscatter = int(sw.size());
+    al.nreduce = int(rw.size());
+    al.ncols = int(cw.size());
+    al.tiles = to_device(t, s);
+    al.tfirst = to_device(t_first, s);
+    al.tcount = to_device(t_count, s);
+    al.scatter = to_device(sw, s);
+    al.reduce = to_device(rw, s);
+    al.rfirst = to_device(r_first, s);
+    al.rcount = to_device(r_count, s);
+    al.cols = to_device(cw, s);
+    CAD_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    al.gen++;
+}
+
+// Before a run of iterations of a batch: launch only the systems still
+// iterating once they hold less than half the rows of the lists in use (and
+// always when a running system is missing from the active lists).
+template <class VT>
+static const ActiveLists* active_lists(cad_matrix* m, Workspace& ws, const std::vector<int>& running, hipStream_t s)
+{
+    if (m->nsys <= 1) return nullptr;
+    int64_t rows = 0;
+    bool missing = false;
+    for (int i = 0; i < m->nsys; i++) {
+        if (!running[i]) continue;
+        rows += m->sys_row_begin[i + 1] - m->sys_row_begin[i];
+        if (ws.use_active && !ws.active.on[i]) missing = true;
+    }
+    const int64_t in_use = ws.use_active ? ws.active.rows : m->rows;
+    if (missing || 2 * rows < in_use) {
+        build_active<VT>(m, ws.active, running, s);
+        ws.use_active = true;
+    }
+    return ws.use_active ? &ws.active : nullptr;
 }
 
 // Kernel attributes of a gather with epilogue Epi (set outside graph capture).
@@ -510,12 +615,14 @@ static void prepare_gather(cad_matrix* m)
 }
 
 // Runs n LSQR iterations, replaying a captured graph of `batch` iterations
-// where possible.
+// where possible (al: the active lists the iterations launch, or all).
 template <class VT>
-static void lsqr_run(cad_matrix* m, Workspace& ws, int n, int batch, bool graph, hipStream_t s)
+static void lsqr_run(cad_matrix* m, Workspace& ws, int n, int batch, bool graph, hipStream_t s,
+                     const ActiveLists* al = nullptr)
 {
+    const int gen = al ? al->gen : -1;
     if (graph && batch > 1 && s != nullptr) {
-        if (!ws.exec || ws.graph_iters != batch || ws.graph_prec != int(sizeof(VT))) {
+        if (!ws.exec || ws.graph_iters != batch || ws.graph_prec != int(sizeof(VT)) || ws.graph_gen != gen) {
             if (ws.exec) (void)hipGraphExecDestroy(ws.exec);
             if (ws.graph) (void)hipGraphDestroy(ws.graph);
             ws.exec = nullptr;
@@ -523,18 +630,19 @@ static void lsqr_run(cad_matrix* m, Workspace& ws, int n, int batch, bool graph,
             prepare_scatter<VT, VT>(m);
             prepare_gather<VT, EpiLsqrU<VT>>(m);
             CAD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (int i = 0; i < batch; i++) lsqr_iteration<VT>(m, ws, s);
+            for (int i = 0; i < batch; i++) lsqr_iteration<VT>(m, ws, s, al);
             CAD_HIP(hipStreamEndCapture(s, &ws.graph));
             CAD_HIP(hipGraphInstantiate(&ws.exec, ws.graph, nullptr, nullptr, 0));
             ws.graph_iters = batch;
             ws.graph_prec = int(sizeof(VT));
+            ws.graph_gen = gen;
         }
         while (n >= batch) {
             CAD_HIP(hipGraphLaunch(ws.exec, s));
             n -= batch;
         }
     }
-    for (int i = 0; i < n; i++) lsqr_iteration<VT>(m, ws, s);
+    for (int i = 0; i < n; i++) lsqr_iteration<VT>(m, ws, s, al);
 }
 
 // LSQR start from the current FP64 residual r = b - A x (x64): u = r / |r|,
@@ -681,9 +789,13 @@ struct RefineState {
 // batch advance independently: at each host check the systems whose inner
 // solve ended are restarted (apply, FP64 residual, new start, selected by the
 // restart flag) while the others keep iterating in the same launches.  A
-// system leaves the stage when it meets tol (act = 0), when a refinement cut
-// its metric by less than 4x (stalled = 1: products in VT cannot resolve its
-// residual further) or after max_refinements.
+// system leaves the stage when it meets tol (act = 0), when a refinement made
+// too little progress (stalled = 1) or after max_refinements.  Too little is
+// a cut of the metric by less than 4x with FP32 products (they cannot resolve
+// the residual further: the FP64 stage takes over) and by less than 10 %
+// with FP64 products.  A refinement that did not lower the metric at all is
+// undone (x goes back to the iterate before it), so a stage never hands on a
+// worse x than it got.
 template <class VT>
 static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const cad_solve_options& o, double floor_tol,
                          int max_refinements, RefineState& st, SystemDetail& det, cad_solve_report& rep)
@@ -691,9 +803,13 @@ static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const ca
     hipStream_t s = m->ctx->stream;
     const int n = m->nsys;
     const int batch = std::max(1, o.check_every);
-    std::vector<double> target(n, 0.0), prev(n, 0.0), best(n, 1.0);
+    const double keep_factor = floor_tol > 0 ? 0.25 : 0.9;  // stall unless rel <= keep_factor * prev
+    const char* prec = sizeof(VT) == 4 ? "fp32" : "fp64";
+    std::vector<double> target(n, 0.0), prev(n, 0.0), prev_rn2(n, 0.0), best(n, 1.0);
     std::vector<int> refs(n, 0), stall(n, 0), checks(n, 0), running(n, 0);
     std::vector<int> restart = st.act;
+    ensure(ws.xprev, sizeof(double) * m->cols);
+    ws.use_active = false;  // every system's blocks until most have finished
     set_active(m, ws, ws.scal, st.act, s);  // all scalars zero, restart = active
     const LsqrScalars* h = nullptr;
     bool first = true;
@@ -702,7 +818,7 @@ static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const ca
             if (v[i]) return true;
         return false;
     };
-    // writes the flags: active = running or restarting, restart as given
+    // writes the flags: active = running or selected, restart = selected
     auto write_flags = [&](const std::vector<int>& rs) {
         std::vector<LsqrScalars> hv(n);
         if (h) std::memcpy(hv.data(), h, sizeof(LsqrScalars) * n);
@@ -713,45 +829,79 @@ static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const ca
         }
         CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * n, hipMemcpyHostToDevice, s));
     };
+    auto cols_kernel = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(unsigned(m->grid.cols.size())), dim3(kRowBlock), 0, s,
+                           m->grid.d_cols.as<ColWork>(), ws.D.as<double>(), ws.y.as<double>(), ws.x64.as<double>(),
+                           ws.xprev.as<double>(), (const LsqrScalars*)ws.scal.as<LsqrScalars>());
+        CAD_HIP(hipGetLastError());
+    };
+    // x += D y for the systems in rs (saving x first), then the FP64 residual
+    // start of all of them; returns the new metric in st.rel / st.rn2 and
+    // undoes, for the systems whose refinement (refs > 0) did not lower the
+    // metric, the step just applied.  undone[i] = 1 for those.
+    auto apply_and_measure = [&](const std::vector<int>& rs, bool apply, std::vector<int>& undone) {
+        undone.assign(n, 0);
+        if (apply) {
+            write_flags(rs);
+            cols_kernel(k_lsqr_apply);
+        }
+        lsqr_start_from_residual<VT>(m, ws, b, s);
+        h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+        rep.explicit_checks++;
+        for (int i = 0; i < n; i++) {
+            if (!rs[i]) continue;
+            st.rel[i] = st.atb[i] > 0 ? std::sqrt(h[i].reduce_b) / st.atb[i] : std::sqrt(h[i].reduce_b);
+            st.rn2[i] = h[i].reduce_a;
+            if (apply && refs[i] > 0 && st.rel[i] >= prev[i] && !(o.tol > 0 && st.rel[i] <= o.tol)) {
+                undone[i] = 1;
+                if (o.verbose)
+                    std::fprintf(stderr, "[refine %s] sys %d it %d: rel %.3e not below %.3e, step undone\n", prec, i,
+                                 st.it, st.rel[i], prev[i]);
+                st.rel[i] = prev[i];
+                st.rn2[i] = prev_rn2[i];
+            }
+        }
+        if (any_of(undone)) {
+            std::vector<int> keep_running = running;
+            std::fill(running.begin(), running.end(), 0);
+            write_flags(undone);
+            cols_kernel(k_lsqr_undo);
+            running = keep_running;
+        }
+    };
+    std::vector<int> undone;
     for (;;) {
         if (any_of(restart)) {
-            if (!first) {
-                write_flags(restart);
-                hipLaunchKernelGGL(k_lsqr_apply, dim3(unsigned(m->grid.cols.size())), dim3(kRowBlock), 0, s,
-                                   m->grid.d_cols.as<ColWork>(), ws.D.as<double>(), ws.y.as<double>(),
-                                   ws.x64.as<double>(), (const LsqrScalars*)ws.scal.as<LsqrScalars>());
-                CAD_HIP(hipGetLastError());
-            }
-            lsqr_start_from_residual<VT>(m, ws, b, s);
-            h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
-            rep.explicit_checks++;
+            apply_and_measure(restart, !first, undone);
             for (int i = 0; i < n; i++) {
                 if (!restart[i]) continue;
-                st.rel[i] = st.atb[i] > 0 ? std::sqrt(h[i].reduce_b) / st.atb[i] : std::sqrt(h[i].reduce_b);
-                st.rn2[i] = h[i].reduce_a;
+                const char* why = nullptr;
                 if (o.tol > 0 && st.rel[i] <= o.tol) {
                     st.act[i] = 0;
                     det.iters[i] = st.it;
-                } else if (refs[i] > 0 && st.rel[i] > 0.25 * prev[i]) {
+                    why = "converged";
+                } else if (undone[i] || (refs[i] > 0 && st.rel[i] > keep_factor * prev[i])) {
                     st.stalled[i] = 1;
+                    why = "stalled";
                 } else if (refs[i] >= max_refinements) {
-                    // left to the caller: not converged
+                    why = "out of refinements";  // left to the caller: not converged
                 } else {
                     // the last refinement only needs to close the remaining gap
                     // (with a factor 3 of margin); products in VT cannot go much
                     // below floor_tol in one solve
                     target[i] = o.tol > 0 ? std::max(floor_tol, o.tol / (3.0 * st.rel[i])) : floor_tol;
                     prev[i] = st.rel[i];
+                    prev_rn2[i] = st.rn2[i];
                     refs[i]++;
                     st.refinements = std::max(st.refinements, refs[i]);
                     running[i] = 1;
                     best[i] = 1.0;
                     stall[i] = checks[i] = 0;
                 }
+                if (o.verbose)
+                    std::fprintf(stderr, "[refine %s] sys %d it %d rel %.3e: %s\n", prec, i, st.it, st.rel[i],
+                                 why ? why : "refine");
             }
-            if (o.verbose && n == 1)
-                std::fprintf(stderr, "[refine %s] it %d rel %.3e\n", sizeof(VT) == 4 ? "fp32" : "fp64", st.it,
-                             st.rel[0]);
             // the started systems iterate; targets on the device
             std::vector<LsqrScalars> hv(h, h + n);
             for (int i = 0; i < n; i++) {
@@ -768,13 +918,13 @@ static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const ca
         }
         if (!any_of(running) || st.it >= o.max_iterations) break;
         const int k = std::min(batch, o.max_iterations - st.it);
-        lsqr_run<VT>(m, ws, k, batch, o.use_graph != 0 && k == batch, s);
+        lsqr_run<VT>(m, ws, k, batch, o.use_graph != 0 && k == batch, s, active_lists<VT>(m, ws, running, s));
         st.it += k;
         h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
         // systems whose inner solve ended: met the target on the device, or
         // stuck (the estimate of |A^T r| is not monotone in LSQR and can
         // plateau, so only a best estimate that did not improve by 1% for as
-        // many checks as the solve has run, at least 16, counts as stuck)
+        // many checks as the solve has run, at least 32, counts as stuck)
         for (int i = 0; i < n; i++) {
             if (!running[i]) continue;
             const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
@@ -786,35 +936,27 @@ static void refine_stage(cad_matrix* m, Workspace& ws, const double* b, const ca
             } else {
                 stall[i]++;
             }
-            const bool stuck = stall[i] >= std::max(16, checks[i] / 2 + 1) && stall[i] * 2 >= checks[i];
+            const bool stuck = stall[i] >= std::max(32, checks[i] / 2 + 1) && stall[i] * 2 >= checks[i];
             if (h[i].stop != 0 || est <= target[i] || stuck) {
+                if (o.verbose)
+                    std::fprintf(stderr, "[refine %s] sys %d it %d: inner solve ended (est %.3e, %s)\n", prec, i,
+                                 st.it, est, stuck && h[i].stop == 0 && est > target[i] ? "stuck" : "target");
                 running[i] = 0;
                 restart[i] = 1;
             }
         }
     }
     // out of iterations with inner solves in flight: keep their corrections
-    // and measure where they got
+    // (unless the metric rose) and measure where they got
     if (any_of(running)) {
         restart = running;
         std::fill(running.begin(), running.end(), 0);
-        write_flags(restart);
-        hipLaunchKernelGGL(k_lsqr_apply, dim3(unsigned(m->grid.cols.size())), dim3(kRowBlock), 0, s,
-                           m->grid.d_cols.as<ColWork>(), ws.D.as<double>(), ws.y.as<double>(), ws.x64.as<double>(),
-                           (const LsqrScalars*)ws.scal.as<LsqrScalars>());
-        CAD_HIP(hipGetLastError());
-        lsqr_start_from_residual<VT>(m, ws, b, s);
-        h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
-        rep.explicit_checks++;
-        for (int i = 0; i < n; i++) {
-            if (!restart[i]) continue;
-            st.rel[i] = st.atb[i] > 0 ? std::sqrt(h[i].reduce_b) / st.atb[i] : std::sqrt(h[i].reduce_b);
-            st.rn2[i] = h[i].reduce_a;
-            if (o.tol > 0 && st.rel[i] <= o.tol) {
+        apply_and_measure(restart, true, undone);
+        for (int i = 0; i < n; i++)
+            if (restart[i] && o.tol > 0 && st.rel[i] <= o.tol) {
                 st.act[i] = 0;
                 det.iters[i] = st.it;
             }
-        }
     }
 }
 
@@ -1116,7 +1258,7 @@ void kernel_times(cad_matrix* m, int reps, double* us, int n)
     fill<float>(x4.as<float>(), Cn, 1.0f, s);
     fill<float>(y4.as<float>(), R, 1.0f, s);
     EventTimer t;
-    double v[4] = {0, 0, 0, 0};
+    double v[5] = {0, 0, 0, 0, 0};
     v[0] = t.us([&] { spmv<float>(m, x4.as<float>(), y4.as<float>(), s); }, s, reps);
     with_view(m, [&](auto view) {
         using View = decltype(view);
@@ -1130,8 +1272,9 @@ void kernel_times(cad_matrix* m, int reps, double* us, int n)
                     s, reps);
         v[2] = t.us([&] { launch_scatter_reduce<float>(sp, nullptr, kScatterAll, z4.as<float>(), s); }, s, reps);
     });
+    v[4] = t.us([&] { launch_jacobi_part(m, s); }, s, std::max(1, reps / 10));
     CAD_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < std::min(n, 4); i++) us[i] = v[i];
+    for (int i = 0; i < std::min(n, 5); i++) us[i] = v[i];
 }
 
 std::string autotune(cad_matrix* m)

@@ -7,16 +7,35 @@
 
 namespace cad {
 
+// The work lists of an LSQR iteration restricted to some systems of a batch.
+// Once most systems of a batch have finished, the iterations of the rest launch
+// only their own blocks (a block of a finished system would still be
+// scheduled, read its flag and exit).  Each list keeps the system order of
+// the full list; systems outside `on` have no blocks.
+struct ActiveLists {
+    std::vector<uint8_t> on;      // per system: in the lists
+    int64_t rows = 0;             // rows of those systems
+    int ntiles = 0, nscatter = 0, nreduce = 0, ncols = 0;
+    DBuf tiles, tfirst, tcount;   // gather blocks and each system's range of them
+    DBuf scatter;                 // A^T y window tiles
+    DBuf reduce, rfirst, rcount;  // column reduction items and each system's range
+    DBuf cols;                    // column blocks
+    int gen = 0;                  // bumped on every rebuild (captured graphs are keyed on it)
+};
+
 // Solver buffers cached on a matrix (one set per solver / precision), with
 // the captured iteration graph.
 struct Workspace {
     int key = 0;
-    DBuf u, r64, v, w, y, x64, D, z64, vd, z, scal, scal_chk;
+    DBuf u, r64, v, w, y, x64, xprev, D, z64, vd, z, scal, scal_chk;
     DBuf P, p, q, zz, xr, rr, t, bvt, pscal, jac;
     int vd_prec = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_iters = 0, graph_prec = 0;
+    int graph_gen = -1;       // ActiveLists generation the graph was captured with (-1: full lists)
+    ActiveLists active;
+    bool use_active = false;  // iterations launch the active lists
     void* host_scal = nullptr;
     ~Workspace();
 };

@@ -139,6 +139,33 @@ class Patterns:
         self.ctx.synchronize()
         return out
 
+    def expanded_size(self) -> int:
+        n = C.c_int64()
+        call("cad_patterns_expanded_size", self.h, C.byref(n))
+        return n.value
+
+    def expand(self, w: torch.Tensor) -> torch.Tensor:
+        """Compact weights -> the expanded tables, one of 3^n floats per
+        variation (ScoreEstimator's constructor, Pattern/Evaluator.cpp:11-29)."""
+        _dev(w, torch.float32)
+        out = torch.empty(self.expanded_size(), dtype=torch.float32, device=w.device)
+        torch.cuda.synchronize(w.device)
+        call("cad_weights_expand", self.h, _ptr(w), _ptr(out))
+        self.ctx.synchronize()
+        return out
+
+    def compact(self, tables: torch.Tensor) -> torch.Tensor:
+        """Expanded tables -> compact weights (ScoreEstimator::Weights,
+        Pattern/Evaluator.cpp:56-73)."""
+        _dev(tables, torch.float32)
+        out = torch.empty(self.cols, dtype=torch.float32, device=tables.device)
+        torch.cuda.synchronize(tables.device)
+        call("cad_weights_compact", self.h, _ptr(tables), _ptr(out))
+        return out
+
+    def evaluator(self, w: torch.Tensor) -> "Evaluator":
+        return Evaluator(self, w)
+
     def matrix(self, pos: torch.Tensor) -> "Matrix":
         _dev(pos, torch.int64)
         torch.cuda.synchronize(pos.device)
@@ -167,6 +194,54 @@ class Patterns:
         call("cad_matrix_from_positions_systems", self.h, _ptr(pos), rb.ctypes.data_as(C.c_void_p),
              rb.shape[0] - 1, C.byref(h))
         return Matrix(self.ctx, h)
+
+
+class Evaluator:
+    """A resident evaluator: the expanded tables of one weight vector kept on
+    the device (ScoreEstimator::Eval for batches of positions)."""
+
+    def __init__(self, patterns: Patterns, w: torch.Tensor):
+        _dev(w, torch.float32)
+        self.patterns = patterns
+        self.ctx = patterns.ctx
+        h = C.c_void_p()
+        torch.cuda.synchronize(w.device)
+        call("cad_evaluator_create", patterns.h, _ptr(w), C.byref(h))
+        self.ctx.synchronize()
+        self.h = h
+        self.ctx._adopt(self)
+
+    def _destroy(self):
+        if getattr(self, "h", None):
+            capi.lib().cad_evaluator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self._destroy()
+        except Exception:
+            pass
+
+    def set_weights(self, w: torch.Tensor):
+        _dev(w, torch.float32)
+        torch.cuda.synchronize(w.device)
+        call("cad_evaluator_set_weights", self.h, _ptr(w))
+        self.ctx.synchronize()
+
+    def __call__(self, pos: torch.Tensor) -> torch.Tensor:
+        _dev(pos, torch.int64)
+        out = torch.empty(pos.shape[0], dtype=torch.float32, device=pos.device)
+        torch.cuda.synchronize(pos.device)
+        call("cad_evaluator_eval", self.h, _ptr(pos), pos.shape[0], _ptr(out))
+        self.ctx.synchronize()
+        return out
+
+    def eval_host(self, PO) -> np.ndarray:
+        PO = np.ascontiguousarray(np.asarray(PO, dtype=np.uint64).reshape(-1, 2))
+        out = np.zeros(PO.shape[0], dtype=np.float32)
+        call("cad_evaluator_eval_host", self.h, PO.ctypes.data_as(C.c_void_p), PO.shape[0],
+             out.ctypes.data_as(C.c_void_p))
+        return out
 
 
 class Matrix:
@@ -213,10 +288,11 @@ class Matrix:
 
     def kernel_times(self, reps: int = 20) -> dict:
         """Average launch times (us) of the FP32 product kernels (HIP events on
-        the library's stream): gather, scatter tiles, scatter reduce, absmax."""
-        out = (C.c_double * 4)()
-        call("cad_matrix_kernel_times", self.h, int(reps), C.cast(out, C.c_void_p), 4)
-        return dict(zip(("gather", "scatter_tiles", "scatter_reduce", "absmax"), list(out)))
+        the library's stream): gather, scatter tiles, scatter reduce, absmax,
+        and the Jacobi count pass."""
+        out = (C.c_double * 5)()
+        call("cad_matrix_kernel_times", self.h, int(reps), C.cast(out, C.c_void_p), 5)
+        return dict(zip(("gather", "scatter_tiles", "scatter_reduce", "absmax", "jacobi_part"), list(out)))
 
     def schedule(self) -> dict:
         """The product schedule in use (cad_schedule fields)."""

@@ -40,6 +40,7 @@ extern "C" {
 typedef struct cad_context cad_context;
 typedef struct cad_patterns cad_patterns;
 typedef struct cad_matrix cad_matrix;
+typedef struct cad_evaluator cad_evaluator;
 
 /* ---- errors, context ---------------------------------------------------- */
 const char* cad_last_error(void);
@@ -141,7 +142,8 @@ int cad_matrix_autotune(cad_matrix* m, char* log, size_t log_size);
  * with HIP events on the context's stream over reps launches after a warm
  * one (inputs: all ones): us[0] the A x gather, us[1] the A^T y window tiles,
  * us[2] their column reduction, us[3] the max |y| pre-pass of the fixed-point
- * windows (0 where not used).  The first n of them are written. */
+ * windows (0 where not used), us[4] the first pass of the Jacobi counts
+ * (JacobiPreconditionerOfATA).  The first n of them are written. */
 int cad_matrix_kernel_times(cad_matrix* m, int reps, double* us, int n);
 typedef struct cad_schedule {
     int gather_f32_rpt, gather_f32_block, gather_f32_sub;
@@ -240,6 +242,39 @@ int cad_eval_backward_f32(cad_patterns* p, const int32_t* d_idx, int64_t B, cons
                           float* d_grad_w);
 int cad_eval_backward_f64(cad_patterns* p, const int32_t* d_idx, int64_t B, const double* d_g,
                           double* d_grad_w);
+
+/* ---- expanded tables (Pattern/Evaluator.cpp:11-29,56-73) ----------------
+ * The evaluator's form of the weights: one table of 3^n floats per variation
+ * (GroupIndexer::Variations order), table i at offset sum_{j<i} 3^n_j. */
+/* Floats of all tables of a pattern group (the sum of 3^n over the variations). */
+int cad_patterns_expanded_size(cad_patterns* p, int64_t* n);
+/* ScoreEstimator(pattern, weights) (Pattern/Evaluator.cpp:11-29): table i gets
+ * w_i[FastIndex(config, variation_i)] = w[Indices(config)[i]] for every
+ * configuration of variation i's mask.  d_w: cols floats, d_tables:
+ * cad_patterns_expanded_size floats. */
+int cad_weights_expand(cad_patterns* p, const float* d_w, float* d_tables);
+/* ScoreEstimator::Weights (Pattern/Evaluator.cpp:56-73): per pattern,
+ * w[Index(config)] = table 0 at FastIndex(config, pattern) for every
+ * configuration of the pattern's mask, the last one in Configurations order
+ * winning; synchronises the context's stream. */
+int cad_weights_compact(cad_patterns* p, const float* d_tables, float* d_w);
+
+/* ---- resident evaluator (ScoreEstimator::Eval, Pattern/Evaluator.cpp:48-54;
+ * the batched form of the leaf evaluations of
+ * Search/PrincipalVariation.cpp:224-229) --------------------------------------
+ * Holds the expanded tables of one compact weight vector in device memory,
+ * so that batches of positions are scored without re-uploading anything.
+ * Every score is the float sum over the variations in order from 0, the same
+ * bits as ScoreEstimator::Eval.  The pattern group must outlive it. */
+int cad_evaluator_create(cad_patterns* p, const float* d_w, cad_evaluator** out);
+int cad_evaluator_destroy(cad_evaluator* e);
+/* New compact weights (d_w: cols floats), expanded on the device. */
+int cad_evaluator_set_weights(cad_evaluator* e, const float* d_w);
+/* d_out[r] = Eval(position r) for n device positions (enqueued on the stream). */
+int cad_evaluator_eval(cad_evaluator* e, const uint64_t* d_positions, int64_t n, float* d_out);
+/* The same for host positions and scores (staged through buffers the
+ * evaluator keeps; synchronises the stream). */
+int cad_evaluator_eval_host(cad_evaluator* e, const uint64_t* h_positions, int64_t n, float* h_out);
 
 /* ---- synthetic inputs (Game/GameGenerator.cpp:3-28, Game/Player.cpp:3-16) */
 /* n positions from uniformly random legal play (empties drawn in [e_lo,e_hi]) */

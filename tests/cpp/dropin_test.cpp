@@ -361,9 +361,82 @@ TEST_CASE(fit_stages_equals_stagewise_fit_weights, true)
     CHECK(checked >= 2);
 }
 
+// ---- file modes for the oracle-parity tests in Python ----------------------
+// All little-endian: u64 counts, then the elements.
+template <class T>
+static T get(std::ifstream& f)
+{
+    T v{};
+    f.read(reinterpret_cast<char*>(&v), sizeof(T));
+    if (!f) throw std::runtime_error("truncated input");
+    return v;
+}
+template <class T>
+static std::vector<T> get_run(std::ifstream& f)
+{
+    std::vector<T> v(get<uint64_t>(f));
+    f.read(reinterpret_cast<char*>(v.data()), std::streamsize(sizeof(T) * v.size()));
+    if (!f) throw std::runtime_error("truncated input");
+    return v;
+}
+template <class T>
+static void put_run(std::ofstream& f, const std::vector<T>& v)
+{
+    const uint64_t n = v.size();
+    f.write(reinterpret_cast<const char*>(&n), sizeof n);
+    f.write(reinterpret_cast<const char*>(v.data()), std::streamsize(sizeof(T) * v.size()));
+}
+
+// fit-stages: in = i32 stage size, i32 iterations, run of u64 masks, run of
+// u64 {P, O} pairs, run of i8 scores; out = u64 stages, then per stage the
+// run of f32 compact weights FitStages gave it.
+static int fit_stages_file(const char* in_path, const char* out_path)
+{
+    std::ifstream in(in_path, std::ios::binary);
+    const int stage_size = get<int32_t>(in), iterations = get<int32_t>(in);
+    const auto masks = get_run<uint64_t>(in);
+    const auto po = get_run<uint64_t>(in);
+    const auto score = get_run<int8_t>(in);
+    if (po.size() != 2 * score.size()) throw std::runtime_error("positions and scores differ in count");
+    std::vector<ScoredPosition> sp(score.size());
+    for (std::size_t i = 0; i < sp.size(); i++) sp[i] = ScoredPosition{Position(po[2 * i], po[2 * i + 1]), score[i]};
+    PatternBasedEstimator est(stage_size, masks);
+    FitStages(est, sp, iterations);
+    std::ofstream out(out_path, std::ios::binary);
+    const uint64_t n = uint64_t(est.Stages());
+    out.write(reinterpret_cast<const char*>(&n), sizeof n);
+    for (int s = 0; s < est.Stages(); s++) put_run(out, est.score.estimators[s].Weights());
+    return out ? 0 : 1;
+}
+
+// eval-batch: in = run of u64 masks, run of f32 compact weights, run of u64
+// {P, O} pairs; out = run of f32 ScoreEstimator::EvalBatch values.
+static int eval_batch_file(const char* in_path, const char* out_path)
+{
+    std::ifstream in(in_path, std::ios::binary);
+    const auto masks = get_run<uint64_t>(in);
+    const auto w = get_run<float>(in);
+    const auto po = get_run<uint64_t>(in);
+    std::vector<Position> pos(po.size() / 2);
+    for (std::size_t i = 0; i < pos.size(); i++) pos[i] = Position(po[2 * i], po[2 * i + 1]);
+    ScoreEstimator est(masks, w);
+    const std::vector<float> v = est.EvalBatch(pos);
+    std::ofstream out(out_path, std::ios::binary);
+    put_run(out, v);
+    return out ? 0 : 1;
+}
+
 int main(int argc, char** argv)
 {
     const std::string mode = argc > 1 ? argv[1] : "host";
+    if ((mode == "fit-stages" || mode == "eval-batch") && argc == 4) {
+        try {
+            return mode == "fit-stages" ? fit_stages_file(argv[2], argv[3]) : eval_batch_file(argv[2], argv[3]);
+        } catch (const std::exception& e) {
+            std::printf("%s failed: %s\n", mode.c_str(), e.what());
+            return 1;
+        }
+    }
     if (mode == "copy-model" && argc == 4) {  // load a weight file and save it again
         try {
             Save(LoadPatternBasedEstimator(argv[2]), argv[3]);

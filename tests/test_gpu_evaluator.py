@@ -62,6 +62,33 @@ def test_backward_against_oracle_and_csr(ctx, oracle, c1):
     assert np.all(np.abs(w64.grad.cpu().numpy() - ref) <= 1e-13 * bound + 1e-300)
 
 
+def test_backward_full_minibatch_against_oracle(ctx, oracle):
+    # the training workload's minibatch: 65,536 rows (the backward's scatter
+    # plan for this size splits the rows into chunks that give every CU a tile)
+    say("eval backward, 65,536-row minibatch")
+    P = Patterns(ctx, N.FIT4)
+    pos = ctx.random_play(1005, 65536, 30)
+    idx = P.extract(pos, capi.LAYOUT_ROW_MAJOR)
+    oidx = oracle.Group(N.FIT4).indices(pos.cpu().numpy().view(np.uint64))
+    assert np.array_equal(idx.cpu().numpy().view(np.uint32), oidx)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    gi = torch.randint(-64, 65, (idx.shape[0],), device="cuda", generator=gen).float()
+    w = torch.zeros(P.cols, dtype=torch.float32, device="cuda", requires_grad=True)
+    pattern_eval(w, idx, P).backward(gi)
+    assert np.array_equal(w.grad.cpu().numpy(), oracle.spmtv(oidx, gi.cpu().numpy(), P.cols))  # exact
+    gr = torch.randn(idx.shape[0], dtype=torch.float32, device="cuda", generator=gen)
+    w.grad = None
+    pattern_eval(w, idx, P).backward(gr)
+    g64 = gr.cpu().numpy().astype(np.float64)
+    ref = oracle.spmtv(oidx, g64, P.cols)
+    bound = oracle.spmtv(oidx, np.abs(g64), P.cols)
+    # FP64 window sums rounded to float per row chunk, the chunks summed in FP64
+    assert np.all(np.abs(w.grad.cpu().numpy().astype(np.float64) - ref) <= 2.0 ** -23 * bound + 1e-30)
+    w64 = torch.zeros(P.cols, dtype=torch.float64, device="cuda", requires_grad=True)
+    pattern_eval(w64, idx, P).backward(gr.double())
+    assert np.all(np.abs(w64.grad.cpu().numpy() - ref) <= 1e-13 * bound + 1e-300)
+
+
 def test_gradcheck_small(ctx, oracle):
     say("gradcheck")
     masks = [N.test_a, N.test_d, N.test_vh, N.D4]

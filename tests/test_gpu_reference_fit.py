@@ -1,0 +1,76 @@
+"""The reference's own fit at its full size: 100,000 random-play positions at
+20 empties, {L02X, B5, Q3, D8} (k = 18, 102,222 columns), Jacobi-PCG on the
+normal equations from x = 0 (FitWeights, PatternFit/PatternFit.cpp:7-39;
+PCG of Math/Solver.h:78-117) -- the GPU in FP64 against the oracle's FP64
+PCG on the same indices and targets.
+
+The normal equations are singular, and PCG's iterates after 100 steps are
+sensitive to rounding: the test measures that sensitivity on the oracle itself
+(the same PCG on targets moved by one ulp each, with random signs) and
+requires the GPU's predictions to lie within 10 times that distance of the
+oracle's.  At 10 iterations, before the rounding has grown, the predictions
+agree to 1e-9."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import cassandra_amd.capi as capi
+from cassandra_amd import names as N
+from cassandra_amd.fit import Patterns
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def fit1(ctx, oracle):
+    pos = ctx.random_play(1001, 100_000, 20)
+    P = Patterns(ctx, N.FIT4)
+    A = P.matrix(pos)
+    _, b = A.make_targets(2001, 3001)
+    pos_h = pos.cpu().numpy().view(np.uint64)
+    assert np.array_equal(pos_h[:2000], oracle.random_play(1001, 2000, 20))
+    idx = oracle.Group(N.FIT4).indices(pos_h)
+    assert np.array_equal(A.download(), idx)
+    return A, b, idx
+
+
+def pcg_gpu(A, b, iterations):
+    x, rep = A.solve(b, solver=capi.SOLVER_PCG, precision=capi.PREC_FP64, tol=0.0, max_iterations=iterations,
+                     check_every=iterations)
+    assert rep["iterations"] == iterations
+    return x.cpu().numpy()
+
+
+def rel_dist(oracle, idx, x, xo):
+    po = oracle.spmv(idx, xo)
+    return float(np.linalg.norm(oracle.spmv(idx, x) - po) / np.linalg.norm(po))
+
+
+def test_pcg_10_iterations_match_oracle(ctx, oracle, fit1):
+    A, b, idx = fit1
+    say("reference fit: 10 PCG iterations")
+    xo, _ = oracle.pcg(idx, A.cols, b.cpu().numpy(), 10)
+    assert rel_dist(oracle, idx, pcg_gpu(A, b, 10), xo) <= 1e-9
+
+
+def test_pcg_100_iterations_within_oracle_sensitivity(ctx, oracle, fit1):
+    A, b, idx = fit1
+    say("reference fit: 100 PCG iterations")
+    bh = b.cpu().numpy()
+    xo, _ = oracle.pcg(idx, A.cols, bh, 100)
+    sign = np.random.default_rng(7).integers(0, 2, size=bh.shape[0])
+    bp = np.where(sign == 1, np.nextafter(bh, np.inf), np.nextafter(bh, -np.inf))
+    bp = np.where(bh == 0, bh, bp)  # zero targets stay zero (one ulp of 0 is subnormal)
+    xp, _ = oracle.pcg(idx, A.cols, bp, 100)
+    sens = rel_dist(oracle, idx, xp, xo)
+    dist = rel_dist(oracle, idx, pcg_gpu(A, b, 100), xo)
+    say(f"reference fit: oracle one-ulp sensitivity {sens:.3e}, GPU distance {dist:.3e}")
+    assert 0 < sens < 1e-2
+    assert dist <= 10 * sens
