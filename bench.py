@@ -377,6 +377,7 @@ def leg_evaluator(ctx):
     reps = 50
     fwd_ms = event_time_ms(ctx, lambda: L.cad_eval_forward_f32(P.h, hp(sub), B, hp(w), hp(out)), reps)
     bwd_ms = event_time_ms(ctx, lambda: L.cad_eval_backward_f32(P.h, hp(sub), B, hp(g), hp(grad)), reps)
+    bwd_na_ms = event_time_ms(ctx, lambda: L.cad_eval_backward_f32_noalloc(P.h, hp(sub), B, hp(g), hp(grad)), reps)
     nbytes = B * (4 * P.k + 4) + 4 * P.cols
     # the resident evaluator (expanded tables kept on the device): positions
     # in, scores out -- on device positions, and host batches with the copies
@@ -417,7 +418,7 @@ def leg_evaluator(ctx):
     log(f"evaluator fits: {fit_cmp}")
     return {"positions": n, "minibatch": B, "epoch_seconds": t_epoch, "positions_per_s": n / t_epoch,
             "loss_epoch": losses[0], "bytes_per_minibatch": nbytes,
-            "kernel_us": {"forward": fwd_ms * 1e3, "backward": bwd_ms * 1e3,
+            "kernel_us": {"forward": fwd_ms * 1e3, "backward": bwd_ms * 1e3, "backward_noalloc": bwd_na_ms * 1e3,
                           "resident_eval_positions": res_ms * 1e3},
             "forward_GBps": nbytes / (fwd_ms * 1e-3) / 1e9, "backward_GBps": nbytes / (bwd_ms * 1e-3) / 1e9,
             "resident_eval_positions_per_s": B / (res_ms * 1e-3),

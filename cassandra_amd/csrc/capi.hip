@@ -1,6 +1,7 @@
 // extern "C" boundary of libcassandra_amd.so (declared in include/cassandra_amd.h).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -8,11 +9,100 @@
 #include "kernels.hpp"
 #include "solver.hpp"
 
+// Scratch of cad_eval_backward_f32_noalloc: the FP64 partial column sums of
+// its row chunks, chunk after chunk (cols doubles each).  Every device the
+// library runs on holds its own copy until the process ends.  In host code the
+// name is only a stand-in: the device address comes from hipGetSymbolAddress
+// at context set-up (noalloc_scratch_init) and only that address is used.
+constexpr int64_t kNoAllocScratchDoubles = int64_t(4) << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
 namespace cad {
 
 thread_local std::string g_last_error;
 
 void set_error(const std::string& s) { g_last_error = s; }
+
+// ---- evaluator backward without allocation --------------------------------
+// grad_w = A^T g for row-major int32 indices in two launches that need no
+// plan: a block per (column window, row chunk) adds g[r] into an FP64 LDS
+// window for the slots of the window's pattern block and writes the window to
+// the chunk's partial slice of the scratch; then every column sums its chunk
+// slices in chunk order (FP64) and rounds once to float.
+constexpr int kNoAllocWindow = 16384;     // columns per LDS window (128 KB of doubles)
+constexpr int kNoAllocMaxWindows = 128;   // window table passed by value
+constexpr int kNoAllocBlock = 1024;
+constexpr int kMaxDevices = 64;
+
+struct NoAllocWindow {
+    int32_t c0, c1;  // global columns [c0, c1) of one pattern block
+    int32_t s0, s1;  // the slots whose entries lie in that block
+};
+struct NoAllocWindows {
+    NoAllocWindow w[kNoAllocMaxWindows];
+};
+
+__global__ __launch_bounds__(kNoAllocBlock) void k_noalloc_grad_window(const int32_t* __restrict__ idx, int64_t B,
+                                                                       int k, const float* __restrict__ g,
+                                                                       NoAllocWindows win, int64_t rows_per_chunk,
+                                                                       int64_t cols, double* __restrict__ part)
+{
+    extern __shared__ double lds[];
+    const NoAllocWindow w = win.w[blockIdx.x];
+    const int n = w.c1 - w.c0;
+    for (int i = threadIdx.x; i < n; i += kNoAllocBlock) lds[i] = 0.0;
+    __syncthreads();
+    const int64_t r0 = int64_t(blockIdx.y) * rows_per_chunk;
+    const int64_t r1 = r0 + rows_per_chunk < B ? r0 + rows_per_chunk : B;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kNoAllocBlock) {
+        const double gv = double(g[r]);
+        const int32_t* row = idx + r * k;
+        for (int s = w.s0; s < w.s1; s++) {
+            // entries outside the window (or outside [0, cols)) are not this block's
+            const uint32_t c = uint32_t(row[s]) - uint32_t(w.c0);
+            if (c < uint32_t(n)) atomicAdd(&lds[c], gv);
+        }
+    }
+    __syncthreads();
+    double* out = part + int64_t(blockIdx.y) * cols + w.c0;
+    for (int i = threadIdx.x; i < n; i += kNoAllocBlock) out[i] = lds[i];
+}
+
+__global__ __launch_bounds__(256) void k_noalloc_grad_reduce(const double* __restrict__ part, int nchunk,
+                                                             int64_t cols, float* __restrict__ grad)
+{
+    for (int64_t c = blockIdx.x * int64_t(256) + threadIdx.x; c < cols; c += int64_t(gridDim.x) * 256) {
+        double s = 0.0;
+        for (int ch = 0; ch < nchunk; ch++) s += part[int64_t(ch) * cols + c];
+        grad[c] = float(s);
+    }
+}
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current: the device address of the
+// scratch array and the LDS limit of the window kernel.
+void noalloc_scratch_init(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) throw Error(CAD_ERR_DEVICE, "device ordinal beyond 64");
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    CAD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(CAD_ERR_DEVICE, "no device address for the scratch array");
+    CAD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_noalloc_grad_window),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(kNoAllocWindow * sizeof(double))));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+}
+
+double* noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices || !g_scratch_ptr[device])
+        throw Error(CAD_ERR_DEVICE, "scratch of the no-allocation entry point not set up on this device");
+    return g_scratch_ptr[device];
+}
 
 
 namespace {
@@ -190,6 +280,7 @@ int cad_context_create(int device, cad_context** out)
         CAD_HIP(hipGetDeviceCount(&n));
         need(device >= 0 && device < n, "no such device", CAD_ERR_DEVICE);
         CAD_HIP(hipSetDevice(device));
+        noalloc_scratch_init(device);
         auto* c = new cad_context;
         c->device = device;
         hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
@@ -776,6 +867,54 @@ int cad_eval_backward_f64(cad_patterns* p, const int32_t* idx, int64_t B, const 
         need(p && grad && B >= 0 && (B == 0 || (idx && g)), "invalid argument");
         need(p->ctx != nullptr, "host-only pattern group (created without a context)");
         eval_backward<double>(p, idx, B, g, grad);
+    });
+}
+
+int cad_eval_backward_f32_noalloc(cad_patterns* p, const int32_t* idx, int64_t B, const float* g, float* grad)
+{
+    return guarded([&] {
+        need(p && grad && B >= 0 && (B == 0 || (idx && g)), "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        const int64_t cols = int64_t(p->group->cols());
+        const int k = p->group->k();
+        need(cols <= kNoAllocScratchDoubles,
+             "no-allocation backward: more than 4,194,304 columns (one partial slice beyond the 32 MiB scratch)");
+        double* part = noalloc_scratch(p->ctx->device);
+        hipStream_t s = p->ctx->stream;
+        if (B == 0) {
+            hipLaunchKernelGGL(k_fill<float>, dim3(grid1(cols, 256)), dim3(256), 0, s, grad, cols, 0.0f);
+            CAD_HIP(hipGetLastError());
+            return;
+        }
+        // windows: consecutive slots with the same column block form one
+        // pattern block, cut into windows of at most kNoAllocWindow columns
+        NoAllocWindows win;
+        int nwin = 0;
+        for (int s0 = 0; s0 < k;) {
+            int s1 = s0 + 1;
+            while (s1 < k && p->slot_lo[s1] == p->slot_lo[s0] && p->slot_hi[s1] == p->slot_hi[s0]) s1++;
+            for (int64_t c0 = p->slot_lo[s0]; c0 < int64_t(p->slot_hi[s0]); c0 += kNoAllocWindow) {
+                need(nwin < kNoAllocMaxWindows, "no-allocation backward: more than 128 column windows");
+                win.w[nwin++] = NoAllocWindow{int32_t(c0),
+                                              int32_t(std::min<int64_t>(c0 + kNoAllocWindow, p->slot_hi[s0])),
+                                              s0, s1};
+            }
+            s0 = s1;
+        }
+        // row chunks: enough blocks for two per CU, at least 2,048 rows a
+        // chunk, and every chunk's slice inside the scratch
+        const int64_t max_chunks = std::min<int64_t>(kNoAllocScratchDoubles / cols, 65535);
+        int64_t nchunk = std::max<int64_t>(1, (2 * int64_t(p->ctx->cus) + nwin - 1) / nwin);
+        nchunk = std::min(nchunk, std::max<int64_t>(1, (B + 2047) / 2048));
+        nchunk = std::min(nchunk, max_chunks);
+        const int64_t rows_per_chunk = (B + nchunk - 1) / nchunk;
+        nchunk = (B + rows_per_chunk - 1) / rows_per_chunk;
+        hipLaunchKernelGGL(k_noalloc_grad_window, dim3(unsigned(nwin), unsigned(nchunk)), dim3(kNoAllocBlock),
+                           size_t(kNoAllocWindow) * sizeof(double), s, idx, B, k, g, win, rows_per_chunk, cols, part);
+        CAD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_noalloc_grad_reduce, dim3(unsigned(std::min<int64_t>(grid1(cols, 256), 4096))),
+                           dim3(256), 0, s, part, int(nchunk), cols, grad);
+        CAD_HIP(hipGetLastError());
     });
 }
 

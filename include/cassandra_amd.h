@@ -242,6 +242,17 @@ int cad_eval_backward_f32(cad_patterns* p, const int32_t* d_idx, int64_t B, cons
                           float* d_grad_w);
 int cad_eval_backward_f64(cad_patterns* p, const int32_t* d_idx, int64_t B, const double* d_g,
                           double* d_grad_w);
+/* No counterpart in the reference.  cad_eval_backward_f32 for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): it calls no hipMalloc, hipFree or
+ * allocating routine.  Its partial sums live in a static 32 MiB device array
+ * per GPU (held until the process ends, its address taken when the context is
+ * created); the pattern group may have at most 4,194,304 columns
+ * (CAD_ERR_ARGUMENT otherwise).  Same result as cad_eval_backward_f32 to
+ * rounding (FP64 sums, exact on integer g).  Calls that overlap on one GPU
+ * share that array and race: serialise them (one stream, or events). */
+int cad_eval_backward_f32_noalloc(cad_patterns* p, const int32_t* d_idx, int64_t B, const float* d_g,
+                                  float* d_grad_w);
 
 /* ---- expanded tables (Pattern/Evaluator.cpp:11-29,56-73) ----------------
  * The evaluator's form of the weights: one table of 3^n floats per variation

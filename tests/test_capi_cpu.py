@@ -57,6 +57,8 @@ def test_host_only_group_rejects_device_work():
     assert rc == -1 and b"host-only" in capi.lib().cad_last_error()
     rc = capi.lib().cad_index_extract(h, C.c_void_p(16), 1, C.c_void_p(16), 0)
     assert rc == -1
+    rc = capi.lib().cad_eval_backward_f32_noalloc(h, C.c_void_p(16), 1, C.c_void_p(16), C.c_void_p(16))
+    assert rc == -1 and b"host-only" in capi.lib().cad_last_error()
     capi.call("cad_patterns_destroy", h)
 
 
