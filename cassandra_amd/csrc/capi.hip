@@ -156,7 +156,36 @@ void upload_programs(cad_patterns* p)
         d.dense_offset = s.dense_offset;
         slots.push_back(d);
     }
+    // the ternary-word programs: per list, its entries (3^j << 5 | 2 (q mod
+    // 16)) grouped by the word q / 16, each word's run padded to whole groups
+    // of four
+    std::vector<DevTern> tslots;
+    std::vector<uint32_t> tern;
+    for (const SlotProgram& s : g.slots) {
+        DevTern t;
+        std::memset(&t, 0, sizeof(t));
+        for (int l = 0; l < 3; l++) {
+            const std::vector<uint8_t>& src = s.list[l].src;
+            t.off[l] = uint32_t(tern.size());
+            for (int w = 0; w < 4; w++) {
+                size_t n = 0;
+                for (size_t j = 0; j < src.size(); j++)
+                    if (src[j] / 16 == w) {
+                        tern.push_back((pow3(int(j)) << 5) | uint32_t(2 * (src[j] % 16)));
+                        n++;
+                    }
+                while (n % 4) {
+                    tern.push_back(0u);
+                    n++;
+                }
+                t.ngrp[l] |= uint32_t(n / 4) << (8 * w);
+            }
+        }
+        tslots.push_back(t);
+    }
     hipStream_t st = p->ctx->stream;
+    p->d_tslots = to_device(tslots, st);
+    p->d_tern = to_device(tern, st);
     p->d_slots = to_device(slots, st);
     sq.resize((sq.size() + 7) / 4 * 4, 0);  // the kernels read the squares as whole 32-bit words
     p->d_sq = to_device(sq, st);
@@ -180,12 +209,12 @@ void extract(cad_patterns* p, const uint64_t* pos, int64_t n, void* out, int byt
     hipStream_t s = p->ctx->stream;
     if (bytes == 2)
         hipLaunchKernelGGL(k_index_extract<uint16_t>, dim3(grid1(n, kRowBlock)), dim3(kRowBlock), 0, s, pos, n,
-                           p->d_slots.as<DevSlot>(), k, p->d_sq.as<uint8_t>(), p->d_dense.as<uint32_t>(),
-                           static_cast<uint16_t*>(out), slot_stride, row_stride, local);
+                           p->d_slots.as<DevSlot>(), k, p->d_tslots.as<DevTern>(), p->d_tern.as<uint32_t>(),
+                           p->d_dense.as<uint32_t>(), static_cast<uint16_t*>(out), slot_stride, row_stride, local);
     else
         hipLaunchKernelGGL(k_index_extract<uint32_t>, dim3(grid1(n, kRowBlock)), dim3(kRowBlock), 0, s, pos, n,
-                           p->d_slots.as<DevSlot>(), k, p->d_sq.as<uint8_t>(), p->d_dense.as<uint32_t>(),
-                           static_cast<uint32_t*>(out), slot_stride, row_stride, local);
+                           p->d_slots.as<DevSlot>(), k, p->d_tslots.as<DevTern>(), p->d_tern.as<uint32_t>(),
+                           p->d_dense.as<uint32_t>(), static_cast<uint32_t*>(out), slot_stride, row_stride, local);
     CAD_HIP(hipGetLastError());
 }
 
@@ -546,6 +575,8 @@ int cad_matrix_from_positions_mf(cad_patterns* p, const uint64_t* d_pos, int64_t
         copy(m->d_slots, p->d_slots);
         copy(m->d_sq, p->d_sq);
         copy(m->d_dense, p->d_dense);
+        copy(m->d_tslots, p->d_tslots);
+        copy(m->d_tern, p->d_tern);
         m->d_pos = DBuf(sizeof(uint64_t) * 2 * size_t(n));
         CAD_HIP(hipMemcpyAsync(m->d_pos.p, d_pos, m->d_pos.bytes, hipMemcpyDeviceToDevice, s));
         build_plans(m.get());
@@ -700,8 +731,9 @@ int cad_matrix_download(const cad_matrix* m, uint32_t* out)
             if (R > 0) {
                 DBuf tmp(t.size() * 4);
                 hipLaunchKernelGGL(k_index_extract<uint32_t>, dim3(grid1(R, kRowBlock)), dim3(kRowBlock), 0, s,
-                                   m->d_pos.as<uint64_t>(), R, m->d_slots.as<DevSlot>(), k, m->d_sq.as<uint8_t>(),
-                                   m->d_dense.as<uint32_t>(), tmp.as<uint32_t>(), R, int64_t(1), 0);
+                                   m->d_pos.as<uint64_t>(), R, m->d_slots.as<DevSlot>(), k,
+                                   m->d_tslots.as<DevTern>(), m->d_tern.as<uint32_t>(), m->d_dense.as<uint32_t>(),
+                                   tmp.as<uint32_t>(), R, int64_t(1), 0);
                 CAD_HIP(hipGetLastError());
                 CAD_HIP(hipMemcpyAsync(t.data(), tmp.p, t.size() * 4, hipMemcpyDeviceToHost, s));
                 CAD_HIP(hipStreamSynchronize(s));

@@ -18,6 +18,20 @@ struct DevSlot {
 };
 static_assert(sizeof(DevSlot) == 32, "DevSlot layout");
 
+// The same slot program in the form the matrix-free products evaluate: each
+// square list as entries of a ternary-word program.  A row's position is held
+// as four 32-bit "ternary words" (square q's digit P_q + 2 O_q in bits
+// 2(q mod 16) of word q / 16); entry e of a list reads word w's digit at shift
+// (e & 31) and weighs it by 3^j = e >> 5.  List l's entries start at off[l],
+// grouped by word (word w: ((ngrp[l] >> 8w) & 255) groups of four entries,
+// zero entries padding each word's run to whole groups).
+struct DevTern {
+    uint32_t off[3];
+    uint32_t ngrp[3];
+    uint32_t pad[2];
+};
+static_assert(sizeof(DevTern) == 32, "DevTern layout");
+
 // A block of rows of one system, for the row-parallel kernels (SpMV and the
 // rows-length vector kernels).  Blocks of one system are contiguous in the
 // work list.

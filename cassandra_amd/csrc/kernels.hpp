@@ -134,20 +134,100 @@ __device__ __forceinline__ uint32_t slot_index(const DevSlot& s, const uint8_t* 
     return idx;
 }
 
+
+
+// The four ternary words of a position: square q's digit P_q + 2 O_q in bits
+// 2 (q mod 16) of word q / 16 (each 16-bit half-board's bits spread to every
+// other bit).
+__device__ __forceinline__ uint32_t spread16(uint32_t x)
+{
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+__device__ __forceinline__ void ternary_words(uint64_t P, uint64_t O, uint32_t (&T)[4])
+{
+#pragma unroll
+    for (int w = 0; w < 4; w++)
+        T[w] = spread16(uint32_t(P >> (16 * w))) | (spread16(uint32_t(O >> (16 * w))) << 1);
+}
+
+// The value sum_j 3^j (P[q_j] + 2 O[q_j]) of one square list for RPT rows,
+// from the list's ternary-word program: per entry one bit-field extract and
+// one multiply-add a row.  The entries are the same for the whole wave
+// (scalar loads, four at a time).
+template <int RPT>
+__device__ __forceinline__ void tern_values(const uint32_t* __restrict__ ent, uint32_t off, uint32_t ngrp,
+                                            const uint32_t (&T)[RPT][4], uint32_t (&v)[RPT])
+{
+#pragma unroll
+    for (int i = 0; i < RPT; i++) v[i] = 0u;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const uint32_t n = (ngrp >> (8 * w)) & 0xFFu;
+        for (uint32_t g = 0; g < n; g++, off += 4) {
+            const uint4 e = *reinterpret_cast<const uint4*>(ent + off);
+            const uint32_t ee[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t sh = ee[q] & 31u, p3 = ee[q] >> 5;
+#pragma unroll
+                for (int i = 0; i < RPT; i++) v[i] += ((T[i][w] >> sh) & 3u) * p3;
+            }
+        }
+    }
+}
+
+// slot_index for RPT rows from their ternary words (the same values).
+template <int RPT>
+__device__ __forceinline__ void slot_index_rows(const DevSlot& s, const DevTern& t, const uint32_t* __restrict__ ent,
+                                                const uint32_t* __restrict__ dense, const uint32_t (&T)[RPT][4],
+                                                uint32_t (&idx)[RPT])
+{
+    tern_values<RPT>(ent, t.off[0], t.ngrp[0], T, idx);
+    if (s.kind == 0) return;
+    if (s.kind == 3) {
+#pragma unroll
+        for (int i = 0; i < RPT; i++) idx[i] = dense[s.dense_offset + idx[i]];
+        return;
+    }
+    uint32_t b[RPT];
+    tern_values<RPT>(ent, t.off[1], t.ngrp[1], T, b);
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        const uint32_t lo = idx[i] < b[i] ? idx[i] : b[i], hi = idx[i] < b[i] ? b[i] : idx[i];
+        idx[i] = lo * s.hs + hi - lo * (lo + 1u) / 2u;
+    }
+    if (s.kind == 2) {
+        tern_values<RPT>(ent, t.off[2], t.ngrp[2], T, b);
+#pragma unroll
+        for (int i = 0; i < RPT; i++) idx[i] = idx[i] * s.ds + b[i];
+    }
+}
+
+// Pattern/Indexer.cpp InsertIndices for every row, one thread per position:
+// out[s * slot_stride + r * row_stride] = (local ? 0 : col_offset) + index,
+// each slot's index from the position's ternary words.
 template <class OutT>
 __global__ __launch_bounds__(kRowBlock) void k_index_extract(
     const uint64_t* __restrict__ pos, int64_t n, const DevSlot* __restrict__ slots, int k,
-    const uint8_t* __restrict__ sq, const uint32_t* __restrict__ dense, OutT* __restrict__ out,
-    int64_t slot_stride, int64_t row_stride, int local)
+    const DevTern* __restrict__ tslots, const uint32_t* __restrict__ tern, const uint32_t* __restrict__ dense,
+    OutT* __restrict__ out, int64_t slot_stride, int64_t row_stride, int local)
 {
     const int64_t r = blockIdx.x * int64_t(kRowBlock) + threadIdx.x;
     if (r >= n) return;
-    const uint64_t P = pos[2 * r], O = pos[2 * r + 1];
+    uint32_t T[1][4];
+    ternary_words(pos[2 * r], pos[2 * r + 1], T[0]);
     for (int s = 0; s < k; s++) {
         const DevSlot sl = slots[s];
-        uint32_t v = slot_index(sl, sq, dense, P, O);
-        if (!local) v += sl.col_offset;
-        out[int64_t(s) * slot_stride + r * row_stride] = OutT(v);
+        uint32_t v[1];
+        slot_index_rows<1>(sl, tslots[s], tern, dense, T, v);
+        if (!local) v[0] += sl.col_offset;
+        out[int64_t(s) * slot_stride + r * row_stride] = OutT(v[0]);
     }
 }
 
@@ -160,6 +240,8 @@ struct PositionsView {
     const DevSlot* __restrict__ slots;
     const uint8_t* __restrict__ sq;
     const uint32_t* __restrict__ dense;
+    const DevTern* __restrict__ tslots;  // the ternary-word programs of the same slots
+    const uint32_t* __restrict__ tern;
     __device__ __forceinline__ uint32_t col(int s, int64_t r) const
     {
         const DevSlot sl = slots[s];
@@ -919,16 +1001,17 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile_mf(PositionsView m, 
     const uint32_t cb = sys_col[w.sys];
     const int64_t r0 = w.row_begin + threadIdx.x;
     const int nval = int(max<int64_t>(0, min<int64_t>(w.row_end - r0, int64_t(RPT) * kTileBlock)));
-    uint64_t P[RPT], O[RPT];
+    uint32_t T[RPT][4];
     VT acc[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
         acc[i] = VT(0);
-        P[i] = O[i] = 0;
+        uint64_t P = 0, O = 0;
         if (i * kTileBlock < nval) {
-            P[i] = m.pos[2 * (r0 + int64_t(i) * kTileBlock)];
-            O[i] = m.pos[2 * (r0 + int64_t(i) * kTileBlock) + 1];
+            P = m.pos[2 * (r0 + int64_t(i) * kTileBlock)];
+            O = m.pos[2 * (r0 + int64_t(i) * kTileBlock) + 1];
         }
+        ternary_words(P, O, T[i]);
     }
     for (int sg = 0; sg < nseg; sg++) {
         const GatherSeg seg = segs[sg];
@@ -940,9 +1023,11 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile_mf(PositionsView m, 
             for (int s = seg.slot_first; s < seg.slot_first + seg.nslots; s++) {
                 const DevSlot sl = m.slots[s];
                 const uint32_t off = sl.col_offset - seg.col_lo;
+                uint32_t ci[RPT];
+                slot_index_rows<RPT>(sl, m.tslots[s], m.tern, m.dense, T, ci);
 #pragma unroll
                 for (int i = 0; i < RPT; i++)
-                    if (i * kTileBlock < nval) acc[i] += xs[off + slot_index(sl, m.sq, m.dense, P[i], O[i])];
+                    if (i * kTileBlock < nval) acc[i] += xs[off + ci[i]];
             }
         } else {
             uint32_t c[RPT][kMaxSegSlots];
@@ -957,9 +1042,11 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile_mf(PositionsView m, 
                 if (j < seg.nslots) {
                     const DevSlot sl = m.slots[seg.slot_first + j];
                     const uint32_t off = sl.col_offset - seg.col_lo;
+                    uint32_t ci[RPT];
+                    slot_index_rows<RPT>(sl, m.tslots[seg.slot_first + j], m.tern, m.dense, T, ci);
 #pragma unroll
                     for (int i = 0; i < RPT; i++)
-                        if (i * kTileBlock < nval) c[i][j] = off + slot_index(sl, m.sq, m.dense, P[i], O[i]);
+                        if (i * kTileBlock < nval) c[i][j] = off + ci[i];
                 }
             }
             for (uint32_t wl = 0; wl < seg.width; wl += wmax) {
@@ -1386,22 +1473,24 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView
          a += int64_t(RPT) * kScatterBlock) {
         bool ok[RPT];
         double yv[RPT];
-        uint64_t P[RPT], O[RPT];
+        uint32_t T[RPT][4];
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             ok[i] = a + i >= w.row_begin && a + i < w.row_end;
             const int64_t r = min(a + i, last);  // clamped: no branch around the loads
             const double t = double(VT(y[r]) * sc);
             yv[i] = ok[i] ? t : 0.0;
-            P[i] = m.pos[2 * r];
-            O[i] = m.pos[2 * r + 1];
+            ternary_words(m.pos[2 * r], m.pos[2 * r + 1], T[i]);
         }
         for (int j = 0; j < gr.nslots; j++) {
-            const DevSlot sl = m.slots[slot_list[gr.first + j]];
+            const int sj = slot_list[gr.first + j];
+            const DevSlot sl = m.slots[sj];
             const uint32_t o = cb + sl.col_offset - w.win_lo;
+            uint32_t ci[RPT];
+            slot_index_rows<RPT>(sl, m.tslots[sj], m.tern, m.dense, T, ci);
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
-                const uint32_t uc = o + slot_index(sl, m.sq, m.dense, P[i], O[i]);
+                const uint32_t uc = o + ci[i];
                 if (ok[i] && uc < W) atomicAdd(&win[uc], yv[i]);
             }
         }
@@ -1467,22 +1556,33 @@ __global__ __launch_bounds__(kScatterBlock) void k_jacobi_part(View m, const Sca
     const SlotGroup gr = groups[w.group];
     const uint32_t cb = sys_col[w.sys];
     const int ns = gr.nslots;
-    for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kScatterBlock) {
-        uint32_t c[kMaxSegSlots];
+    // four rows a thread per step, all their entries loaded before any add
+    constexpr int kRows = 4;
+    for (int64_t r0 = w.row_begin + threadIdx.x; r0 < w.row_end; r0 += kRows * kScatterBlock) {
+        uint32_t c[kRows][kMaxSegSlots];
 #pragma unroll
-        for (int j = 0; j < kMaxSegSlots; j++) c[j] = j < ns ? cb + m.col(slot_list[gr.first + j], r) : 0xFFFFFFFFu;
+        for (int u = 0; u < kRows; u++) {
+            const int64_t r = r0 + int64_t(u) * kScatterBlock;
+            const bool valid = r < w.row_end;
 #pragma unroll
-        for (int j = 0; j < kMaxSegSlots; j++) {
-            const uint32_t uc = c[j] - w.win_lo;
-            if (j >= ns || uc >= W) continue;
-            uint32_t mult = 0;
-            bool first = true;
+            for (int j = 0; j < kMaxSegSlots; j++)
+                c[u][j] = (valid && j < ns) ? cb + m.col(slot_list[gr.first + j], r) : 0xFFFFFFFFu;
+        }
 #pragma unroll
-            for (int t = 0; t < kMaxSegSlots; t++) {
-                mult += c[t] == c[j] ? 1u : 0u;
-                if (t < j && c[t] == c[j]) first = false;
+        for (int u = 0; u < kRows; u++) {
+#pragma unroll
+            for (int j = 0; j < kMaxSegSlots; j++) {
+                const uint32_t uc = c[u][j] - w.win_lo;
+                if (c[u][j] == 0xFFFFFFFFu || uc >= W) continue;
+                uint32_t mult = 0;
+                bool first = true;
+#pragma unroll
+                for (int t = 0; t < kMaxSegSlots; t++) {
+                    mult += c[u][t] == c[u][j] ? 1u : 0u;
+                    if (t < j && c[u][t] == c[u][j]) first = false;
+                }
+                if (first) atomicAdd(&win[uc], mult * mult);
             }
-            if (first) atomicAdd(&win[uc], mult * mult);
         }
     }
     __syncthreads();

@@ -157,6 +157,7 @@ struct cad_patterns {
     cad_context* ctx = nullptr;
     std::unique_ptr<cad::PatternGroup> group;
     cad::DBuf d_slots, d_sq, d_dense;
+    cad::DBuf d_tslots, d_tern;             // the ternary-word form of the slot programs
     std::vector<uint32_t> slot_lo, slot_hi;  // global column range of every slot
     // scatter plans of the evaluator backward, one per minibatch size
     std::map<int64_t, std::unique_ptr<cad::ScatterPlan>> eval_plans;
@@ -183,7 +184,7 @@ struct cad_matrix {
     int64_t ld = 0;                 // rows rounded up to kIndexAlign (rows r >= rows are zero)
     // matrix-free: the positions and a copy of the pattern group's slot programs
     bool mf = false;
-    cad::DBuf d_pos, d_slots, d_sq, d_dense;
+    cad::DBuf d_pos, d_slots, d_sq, d_dense, d_tslots, d_tern;
     std::vector<uint32_t> slot_lo, slot_hi;
     int nsys = 1;
     std::vector<int64_t> sys_row_begin;  // nsys + 1

@@ -20,7 +20,7 @@ void with_view(cad_matrix* m, F&& f)
 {
     if (m->mf)
         f(PositionsView{m->d_pos.as<uint64_t>(), m->d_slots.as<DevSlot>(), m->d_sq.as<uint8_t>(),
-                        m->d_dense.as<uint32_t>()});
+                        m->d_dense.as<uint32_t>(), m->d_tslots.as<DevTern>(), m->d_tern.as<uint32_t>()});
     else if (m->ibytes == 2)
         f(SlotMajor<uint16_t>{m->d_idx.as<uint16_t>(), m->ld, m->d_base.as<uint32_t>()});
     else
