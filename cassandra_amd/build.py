@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcassandra_amd.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["patterns.cpp", "runtime.hip", "solver.hip", "lsmr.hip", "evaluator.hip", "capi.hip"]
+SOURCES = ["patterns.cpp", "runtime.hip", "solver.hip", "evaluator.hip", "capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CAD_OFFLOAD_ARCH", "gfx950")

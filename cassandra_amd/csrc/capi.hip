@@ -579,6 +579,10 @@ int cad_matrix_from_positions_mf(cad_patterns* p, const uint64_t* d_pos, int64_t
         copy(m->d_tern, p->d_tern);
         m->d_pos = DBuf(sizeof(uint64_t) * 2 * size_t(n));
         CAD_HIP(hipMemcpyAsync(m->d_pos.p, d_pos, m->d_pos.bytes, hipMemcpyDeviceToDevice, s));
+        m->d_tw = DBuf(sizeof(uint32_t) * 4 * size_t(n));
+        hipLaunchKernelGGL(k_ternary_rows, dim3(unsigned(std::min<int64_t>(grid1(n, 256), 65536))), dim3(256), 0, s,
+                           (const uint64_t*)m->d_pos.as<uint64_t>(), n, m->d_tw.as<uint4>());
+        CAD_HIP(hipGetLastError());
         build_plans(m.get());
         CAD_HIP(hipStreamSynchronize(s));
         *out = m.release();

@@ -159,6 +159,15 @@ struct LsqrScalars {
 };
 static_assert(sizeof(LsqrScalars) % 16 == 0, "LsqrScalars layout");
 
+// Per-system state of the device-resident LSMR rotations (Fong & Saunders
+// 2011, damp = 0; Math/Solver.h:219-287), beside the LsqrScalars of the
+// shared Golub-Kahan bidiagonalisation; k_hb, k_x, k_h are the last
+// iteration's update coefficients of hbar, x and h.
+struct LsmrState {
+    double alphabar, zetabar, rho, rhobar, cbar, sbar;
+    double k_hb, k_x, k_h, pad;
+};
+
 // Per-system state of the device-resident PCG recurrence.
 struct PcgScalars {
     double rz, rz_new, pq, alpha, beta, rr;

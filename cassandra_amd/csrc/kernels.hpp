@@ -156,6 +156,30 @@ __device__ __forceinline__ void ternary_words(uint64_t P, uint64_t O, uint32_t (
         T[w] = spread16(uint32_t(P >> (16 * w))) | (spread16(uint32_t(O >> (16 * w))) << 1);
 }
 
+// A descriptor that is the same for the whole wave, read as whole 32-bit
+// words into scalar registers (a struct with byte or half-word fields would
+// otherwise be read with vector loads and full waits).
+template <class T>
+__device__ __forceinline__ T uniform_load(const T* p)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    const uint32_t* __restrict__ w = reinterpret_cast<const uint32_t*>(p);
+    uint32_t r[sizeof(T) / 4];
+#pragma unroll
+    for (int i = 0; i < int(sizeof(T) / 4); i++) r[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    T t;
+    __builtin_memcpy(&t, r, sizeof(T));
+    return t;
+}
+
+// Element i of a uint16 array the whole wave reads (the array is padded to
+// whole 32-bit words).
+__device__ __forceinline__ int uniform_u16(const uint16_t* p, int i)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t*>(p)[i >> 1]);
+    return int((w >> (16 * (i & 1))) & 0xFFFFu);
+}
+
 // The value sum_j 3^j (P[q_j] + 2 O[q_j]) of one square list for RPT rows,
 // from the list's ternary-word program: per entry one bit-field extract and
 // one multiply-add a row.  The entries are the same for the whole wave
@@ -209,6 +233,17 @@ __device__ __forceinline__ void slot_index_rows(const DevSlot& s, const DevTern&
     }
 }
 
+// The ternary words of n positions (a matrix-free matrix keeps them, so the
+// products need not recompute them).
+__global__ void k_ternary_rows(const uint64_t* __restrict__ pos, int64_t n, uint4* __restrict__ tw)
+{
+    for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < n; r += int64_t(gridDim.x) * blockDim.x) {
+        uint32_t T[4];
+        ternary_words(pos[2 * r], pos[2 * r + 1], T);
+        tw[r] = make_uint4(T[0], T[1], T[2], T[3]);
+    }
+}
+
 // Pattern/Indexer.cpp InsertIndices for every row, one thread per position:
 // out[s * slot_stride + r * row_stride] = (local ? 0 : col_offset) + index,
 // each slot's index from the position's ternary words.
@@ -242,6 +277,7 @@ struct PositionsView {
     const uint32_t* __restrict__ dense;
     const DevTern* __restrict__ tslots;  // the ternary-word programs of the same slots
     const uint32_t* __restrict__ tern;
+    const uint4* __restrict__ tw;        // every row's ternary words (computed once, at creation)
     __device__ __forceinline__ uint32_t col(int s, int64_t r) const
     {
         const DevSlot sl = slots[s];
@@ -983,22 +1019,24 @@ __global__ __launch_bounds__(BS) void k_gather_tile(SlotMajor<IdxT> m, const Row
         gather_tile_sub<VT, RPT, IdxT, Epi, BS, S>(m, work, sys_col, steps, nsteps, phases, wmax, x, epi, g);
 }
 
-// The LDS-staged gather of a matrix-free pattern matrix: as k_gather_tile,
-// with each thread's RPT positions held in registers and every slot's column
-// computed by its program where the stored kernel loads it.  Multi-window
-// segments compute the slot columns once, before the window loop.
+// The LDS-staged gather of a matrix-free pattern matrix: the stored tile
+// kernel's plan (steps over the slots, narrow segments staged side by side in
+// one phase, wide segments window by window), with each thread's RPT rows'
+// ternary words in registers and every slot's column computed by its program
+// where the stored kernel loads it.  A wide step computes its slots' columns
+// once, before the window loop.
 template <class VT, int RPT, class Epi>
 __global__ __launch_bounds__(kTileBlock) void k_gather_tile_mf(PositionsView m, const RowWork* __restrict__ work,
                                                                const uint32_t* __restrict__ sys_col,
-                                                               const GatherSeg* __restrict__ segs, int nseg,
-                                                               uint32_t wmax, const VT* __restrict__ x, Epi epi,
-                                                               GridSys g)
+                                                               const GatherStep* __restrict__ steps, int nsteps,
+                                                               const PhaseSeg* __restrict__ phases, uint32_t wmax,
+                                                               const VT* __restrict__ x, Epi epi, GridSys g)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     VT* xs = reinterpret_cast<VT*>(smem_raw);
     const RowWork w = work[blockIdx.x];
     if (!epi.begin(w.sys)) return;
-    const uint32_t cb = sys_col[w.sys];
+    const VT* __restrict__ xb = x + sys_col[w.sys];
     const int64_t r0 = w.row_begin + threadIdx.x;
     const int nval = int(max<int64_t>(0, min<int64_t>(w.row_end - r0, int64_t(RPT) * kTileBlock)));
     uint32_t T[RPT][4];
@@ -1006,30 +1044,18 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile_mf(PositionsView m, 
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
         acc[i] = VT(0);
-        uint64_t P = 0, O = 0;
-        if (i * kTileBlock < nval) {
-            P = m.pos[2 * (r0 + int64_t(i) * kTileBlock)];
-            O = m.pos[2 * (r0 + int64_t(i) * kTileBlock) + 1];
-        }
-        ternary_words(P, O, T[i]);
+        uint4 t = make_uint4(0u, 0u, 0u, 0u);
+        if (i * kTileBlock < nval) t = m.tw[r0 + int64_t(i) * kTileBlock];
+        T[i][0] = t.x;
+        T[i][1] = t.y;
+        T[i][2] = t.z;
+        T[i][3] = t.w;
     }
-    for (int sg = 0; sg < nseg; sg++) {
-        const GatherSeg seg = segs[sg];
-        const VT* __restrict__ xg = x + cb + seg.col_lo;
-        if (seg.nwin == 1) {
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < seg.width; i += kTileBlock) xs[i] = xg[i];
-            __syncthreads();
-            for (int s = seg.slot_first; s < seg.slot_first + seg.nslots; s++) {
-                const DevSlot sl = m.slots[s];
-                const uint32_t off = sl.col_offset - seg.col_lo;
-                uint32_t ci[RPT];
-                slot_index_rows<RPT>(sl, m.tslots[s], m.tern, m.dense, T, ci);
-#pragma unroll
-                for (int i = 0; i < RPT; i++)
-                    if (i * kTileBlock < nval) acc[i] += xs[off + ci[i]];
-            }
-        } else {
+    for (int k = 0; k < nsteps; k++) {
+        const GatherStep st = step_at(steps, k);
+        if (st.nwin > 1) {
+            // a segment wider than the LDS: the step's columns (segment-local),
+            // then its windows, each slot's value kept until the last window
             uint32_t c[RPT][kMaxSegSlots];
             VT v[RPT][kMaxSegSlots];
 #pragma unroll
@@ -1039,36 +1065,65 @@ __global__ __launch_bounds__(kTileBlock) void k_gather_tile_mf(PositionsView m, 
                     v[i][j] = VT(0);
                     c[i][j] = 0xFFFFFFFFu;
                 }
-                if (j < seg.nslots) {
-                    const DevSlot sl = m.slots[seg.slot_first + j];
-                    const uint32_t off = sl.col_offset - seg.col_lo;
+                if (j < st.nslots) {
+                    const DevSlot sl = uniform_load(m.slots + st.slot_first + j);
                     uint32_t ci[RPT];
-                    slot_index_rows<RPT>(sl, m.tslots[seg.slot_first + j], m.tern, m.dense, T, ci);
+                    slot_index_rows<RPT>(sl, uniform_load(m.tslots + st.slot_first + j), m.tern, m.dense, T, ci);
 #pragma unroll
                     for (int i = 0; i < RPT; i++)
-                        if (i * kTileBlock < nval) c[i][j] = off + ci[i];
+                        if (i * kTileBlock < nval) c[i][j] = st.off[j] + sl.col_offset + ci[i];
                 }
             }
-            for (uint32_t wl = 0; wl < seg.width; wl += wmax) {
-                const uint32_t W = min(wmax, seg.width - wl);
-                __syncthreads();
-                for (uint32_t i = threadIdx.x; i < W; i += kTileBlock) xs[i] = xg[wl + i];
-                __syncthreads();
+            const bool rev = st.wflags & kWideReverse;
+            for (uint32_t n = 0; n < st.nwin; n++) {
+                const uint32_t wl = (rev ? st.nwin - 1 - n : n) * wmax;
+                const uint32_t W = min(wmax, st.width - wl);
+                uint32_t h;
+                if (n == 0 && (st.wflags & kWideStaged)) {
+                    h = line_head(xb + st.col_lo + wl);  // staged by the step before
+                } else {
+                    __syncthreads();
+                    h = stage_direct<VT, kTileBlock>(xs, xb + st.col_lo + wl, W);
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __syncthreads();
+                }
 #pragma unroll
                 for (int j = 0; j < kMaxSegSlots; j++) {
 #pragma unroll
                     for (int i = 0; i < RPT; i++) {
                         const uint32_t cl = c[i][j] - wl;
-                        if (cl < W) v[i][j] = xs[cl];
+                        if (cl < W) v[i][j] = xs[h + cl];
                     }
                 }
             }
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++)
-                if (j < seg.nslots) {
+                if (j < st.nslots) {
 #pragma unroll
                     for (int i = 0; i < RPT; i++) acc[i] += v[i][j];
                 }
+        } else {
+            if (st.phase_count > 0) {
+                __syncthreads();
+                for (int p = st.phase_first; p < st.phase_first + st.phase_count; p++) {
+                    const PhaseSeg ps = uniform_load(phases + p);
+                    stage_direct<VT, kTileBlock>(xs + ps.lds_off, xb + ps.col_lo, ps.width);
+                }
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+            }
+            const uint32_t h = line_head(xb + st.col_lo);
+            for (int j = 0; j < st.nslots; j++) {
+                const DevSlot sl = uniform_load(m.slots + st.slot_first + j);
+                // (read from the descriptor in memory: a run-time index into
+                // the copy in registers would put the step in scratch)
+                const uint32_t off = __builtin_amdgcn_readfirstlane(steps[k].off[j]) + h + sl.col_offset;
+                uint32_t ci[RPT];
+                slot_index_rows<RPT>(sl, uniform_load(m.tslots + st.slot_first + j), m.tern, m.dense, T, ci);
+#pragma unroll
+                for (int i = 0; i < RPT; i++)
+                    if (i * kTileBlock < nval) acc[i] += xs[off + ci[i]];
+            }
         }
     }
     double red = 0.0;
@@ -1460,12 +1515,12 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* win = reinterpret_cast<double*>(smem_raw);
-    const ScatterWork w = work[blockIdx.x];
+    const ScatterWork w = uniform_load(work + blockIdx.x);
     if (!scatter_selected(div, mode, w.sys)) return;
     const uint32_t W = w.win_hi - w.win_lo;
     for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = 0.0;
     __syncthreads();
-    const SlotGroup gr = groups[w.group];
+    const SlotGroup gr = uniform_load(groups + w.group);
     const uint32_t cb = sys_col[w.sys];
     const VT sc = scatter_scale<VT>(div, mode, w.sys);
     const int64_t last = w.row_end - 1;
@@ -1480,14 +1535,18 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView
             const int64_t r = min(a + i, last);  // clamped: no branch around the loads
             const double t = double(VT(y[r]) * sc);
             yv[i] = ok[i] ? t : 0.0;
-            ternary_words(m.pos[2 * r], m.pos[2 * r + 1], T[i]);
+            const uint4 tw = m.tw[r];
+            T[i][0] = tw.x;
+            T[i][1] = tw.y;
+            T[i][2] = tw.z;
+            T[i][3] = tw.w;
         }
         for (int j = 0; j < gr.nslots; j++) {
-            const int sj = slot_list[gr.first + j];
-            const DevSlot sl = m.slots[sj];
+            const int sj = uniform_u16(slot_list, gr.first + j);
+            const DevSlot sl = uniform_load(m.slots + sj);
             const uint32_t o = cb + sl.col_offset - w.win_lo;
             uint32_t ci[RPT];
-            slot_index_rows<RPT>(sl, m.tslots[sj], m.tern, m.dense, T, ci);
+            slot_index_rows<RPT>(sl, uniform_load(m.tslots + sj), m.tern, m.dense, T, ci);
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
                 const uint32_t uc = o + ci[i];
@@ -1797,6 +1856,118 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_undo(const ColWork* __restri
     const ColWork w = work[blockIdx.x];
     if (!scal[w.sys].active || !scal[w.sys].restart) return;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) x[c] = xprev[c];
+}
+
+// ------------------------------------------------------------ LSMR ------
+// The symmetric Givens rotation of Math/Solver.h:193-217 (scipy's sym_ortho).
+__device__ __forceinline__ void sym_ortho(double a, double b, double& c, double& s, double& r)
+{
+    const double sa = a < 0 ? -1.0 : (a > 0 ? 1.0 : 0.0), sb = b < 0 ? -1.0 : (b > 0 ? 1.0 : 0.0);
+    if (b == 0.0) {
+        c = sa;
+        s = 0.0;
+        r = fabs(a);
+    } else if (a == 0.0) {
+        c = 0.0;
+        s = sb;
+        r = fabs(b);
+    } else if (fabs(b) > fabs(a)) {
+        const double tau = a / b;
+        s = sb / sqrt(1.0 + tau * tau);
+        c = s * tau;
+        r = b / s;
+    } else {
+        const double tau = b / a;
+        c = sa / sqrt(1.0 + tau * tau);
+        s = c * tau;
+        r = a / c;
+    }
+}
+
+// LSMR's rotations (Math/Solver.h:255-275), one block per system: alpha =
+// |vbar| from k_lsqr_reduce_v's partials, then the two rotations, zeta and
+// the update coefficients; nres = |zetabar| (= |A^T r_k|) and the device-side
+// stop as in k_lsqr_rot.
+__global__ __launch_bounds__(kRowBlock) void k_lsmr_rot(GridSys g, LsqrScalars* scal, LsmrState* lsmr)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const int sys = blockIdx.x;
+    LsqrScalars* st = &scal[sys];
+    if (!st->active) return;
+    double sum = 0.0;
+    for (int i = threadIdx.x; i < g.blk_count[sys]; i += kRowBlock) sum += g.partials[2 * (g.blk_first[sys] + i)];
+    sum = block_sum<kRowBlock>(sum, sh);
+    if (threadIdx.x != 0) return;
+    LsmrState* ls = &lsmr[sys];
+    if (st->stop != 0) {
+        st->stop = 2;
+        return;
+    }
+    const double beta = st->beta, alpha = sqrt(sum);
+    const double rho_old = ls->rho, rhobar_old = ls->rhobar;
+    double c, s, rho;
+    sym_ortho(ls->alphabar, beta, c, s, rho);
+    const double theta_new = s * alpha;
+    ls->alphabar = c * alpha;
+    const double thetabar = ls->sbar * rho;
+    double cbar, sbar, rhobar;
+    sym_ortho(ls->cbar * rho, theta_new, cbar, sbar, rhobar);
+    const double zeta = cbar * ls->zetabar;
+    ls->zetabar = -sbar * ls->zetabar;
+    ls->cbar = cbar;
+    ls->sbar = sbar;
+    ls->rho = rho;
+    ls->rhobar = rhobar;
+    ls->k_hb = -thetabar * rho / (rho_old * rhobar_old);
+    ls->k_x = zeta / (rho * rhobar);
+    ls->k_h = rho != 0.0 ? -theta_new / rho : 0.0;
+    st->alpha = alpha;
+    st->nres = fabs(ls->zetabar);
+    st->anorm2 += sum;
+    st->iters += 1;
+    if (st->target > 0.0 && st->nres <= st->target * st->nres0) st->stop = 1;
+}
+
+// LSMR's vector updates (Math/Solver.h:277-283): v = vbar / alpha, vd = D v,
+// hbar = h + k_hb hbar, x += k_x hbar, h = v + k_h h.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_lsmr_update(const ColWork* __restrict__ work,
+                                                           const double* __restrict__ D, double* __restrict__ v,
+                                                           VT* __restrict__ vd, double* __restrict__ h,
+                                                           double* __restrict__ hb, double* __restrict__ x,
+                                                           const LsqrScalars* scal, const LsmrState* lsmr)
+{
+    const ColWork w = work[blockIdx.x];
+    const LsqrScalars* st = &scal[w.sys];
+    if (!st->active || st->stop > 1) return;
+    const LsmrState* ls = &lsmr[w.sys];
+    const double ia = st->alpha != 0.0 ? 1.0 / st->alpha : 0.0;
+    const double khb = ls->k_hb, kx = ls->k_x, kh = ls->k_h;
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const double vn = v[c] * ia;
+        v[c] = vn;
+        vd[c] = VT(D[c] * vn);
+        const double hc = h[c];
+        const double hbn = hc + khb * hb[c];
+        hb[c] = hbn;
+        x[c] += kx * hbn;
+        h[c] = vn + kh * hc;
+    }
+}
+
+// LSMR's start after the LSQR start of each system (x = 0): the rotation
+// state alphabar = alpha, zetabar = alpha beta, rho = rhobar = cbar = 1,
+// sbar = 0 (Math/Solver.h:238-252).
+__global__ void k_lsmr_init(int nsys, const LsqrScalars* scal, LsmrState* lsmr)
+{
+    const int sys = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sys >= nsys) return;
+    LsmrState* ls = &lsmr[sys];
+    ls->alphabar = scal[sys].alpha;
+    ls->zetabar = scal[sys].alpha * scal[sys].beta;
+    ls->rho = ls->rhobar = ls->cbar = 1.0;
+    ls->sbar = 0.0;
+    ls->k_hb = ls->k_x = ls->k_h = 0.0;
 }
 
 // |z|^2 per system into reduce_b (used for |A^T b| and |A^T r|).

@@ -387,7 +387,14 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
 {
     make_groups(lo, hi, sp);
     const size_t lds = lds_budget(ctx);
-    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 4), 4, ibytes, ctx.cus, tune, sp.f4);
+    // the 4-byte windows serve the Jacobi counts only (once per matrix): per-
+    // window chunks, several tiles per CU, the u32 partial slices uncapped
+    TuneParams t4 = tune;
+    t4.scatter_order = 0;
+    t4.blocks_per_cu = std::max(4.0, tune.blocks_per_cu);
+    t4.partial_cap = 1.0;
+    t4.scatter_balance = 1.0;
+    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 4), 4, ibytes, ctx.cus, t4, sp.f4);
     make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 8), 8, ibytes, ctx.cus, tune, sp.f8);
     // bound on the terms a window column receives in one work item (rows x
     // slots of the group), for the fixed-point scale of the FP32 scatter
@@ -401,7 +408,12 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
         }
     }
     sp.d_groups = to_device(sp.groups, ctx.stream);
-    sp.d_slot_list = to_device(sp.slot_list, ctx.stream);
+    // slot lists padded to whole 32-bit words (the kernels read them as words)
+    auto padded = [](std::vector<uint16_t> v) {
+        if (v.size() % 2) v.push_back(0);
+        return v;
+    };
+    sp.d_slot_list = to_device(padded(sp.slot_list), ctx.stream);
     const int nsys = int(sys_rows.size()) - 1;
     for (ScatterLists* L : {&sp.f4, &sp.f8}) {
         std::stable_sort(L->reduce.begin(), L->reduce.end(),
@@ -413,7 +425,7 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
             L->red_count[size_t(L->reduce[i].sys)]++;
         }
         L->d_groups = to_device(L->groups, ctx.stream);
-        L->d_slot_list = to_device(L->slot_list, ctx.stream);
+        L->d_slot_list = to_device(padded(L->slot_list), ctx.stream);
         L->d_work = to_device(L->work, ctx.stream);
         L->d_reduce = to_device(L->reduce, ctx.stream);
         L->d_red_first = to_device(L->red_first, ctx.stream);

@@ -184,7 +184,7 @@ struct cad_matrix {
     int64_t ld = 0;                 // rows rounded up to kIndexAlign (rows r >= rows are zero)
     // matrix-free: the positions and a copy of the pattern group's slot programs
     bool mf = false;
-    cad::DBuf d_pos, d_slots, d_sq, d_dense, d_tslots, d_tern;
+    cad::DBuf d_pos, d_slots, d_sq, d_dense, d_tslots, d_tern, d_tw;
     std::vector<uint32_t> slot_lo, slot_hi;
     int nsys = 1;
     std::vector<int64_t> sys_row_begin;  // nsys + 1

@@ -20,7 +20,8 @@ void with_view(cad_matrix* m, F&& f)
 {
     if (m->mf)
         f(PositionsView{m->d_pos.as<uint64_t>(), m->d_slots.as<DevSlot>(), m->d_sq.as<uint8_t>(),
-                        m->d_dense.as<uint32_t>(), m->d_tslots.as<DevTern>(), m->d_tern.as<uint32_t>()});
+                        m->d_dense.as<uint32_t>(), m->d_tslots.as<DevTern>(), m->d_tern.as<uint32_t>(),
+                        m->d_tw.as<uint4>()});
     else if (m->ibytes == 2)
         f(SlotMajor<uint16_t>{m->d_idx.as<uint16_t>(), m->ld, m->d_base.as<uint32_t>()});
     else
@@ -295,8 +296,8 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s, const ActiveList
                     auto kern = k_gather_tile_mf<VT, RPT, Epi>;
                     allow_lds(kern, lds);
                     hipLaunchKernelGGL(kern, dim3(unsigned(rl.n)), dim3(kTileBlock), lds, s, view, rl.p,
-                                       m->d_sys_col.as<uint32_t>(), gp.d_segs.as<GatherSeg>(), int(gp.segs.size()),
-                                       gp.wmax, x, epi, rl.sys);
+                                       m->d_sys_col.as<uint32_t>(), gp.d_steps.as<GatherStep>(),
+                                       int(gp.steps.size()), gp.d_phases.as<PhaseSeg>(), gp.wmax, x, epi, rl.sys);
                 } else {
                     with_tile_shape<sub_tiles_ok<VT, IdxT>(RPT)>(gp.bs, gp.sub, [&](auto bc, auto sc) {
                         constexpr int BS = decltype(bc)::value;
@@ -1122,6 +1123,120 @@ static void pcg_solve(cad_matrix* m, const double* b, double* x, const cad_solve
     rep.converged = o.tol > 0 && worst <= o.tol;
 }
 
+// ------------------------------------------------------------ LSMR --------
+// LSMR (Math/Solver.h:219-287, Fong & Saunders 2011, damp = 0) in FP64 on
+// the device: the same bidiagonalisation launches as LSQR (gather with the
+// u-update, window scatter, column reduction with the v-update), then the
+// LSMR rotations (one block per system) and its vector updates; scalars stay
+// on the device, check_every iterations are one captured graph, and each
+// system stops on the device once |A^T r| (|zetabar|) reaches tol / 3 of
+// |A^T b|.  Like the reference's LSMR it is not preconditioned.
+static void lsmr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
+{
+    const GridPlan& g = m->grid;
+    LsqrScalars* sc = ws.scal.as<LsqrScalars>();
+    gather<double>(m, ws.vd.as<double>(), EpiLsqrU<double>{ws.u.as<double>(), sc, 0.0}, s);
+    ScatterLists& L = lists<double>(m->splan);
+    with_view(m, [&](auto view) {
+        launch_scatter_tiles<double, double>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(),
+                                             ws.u.as<double>(), sc, kScatterLsqr, s, true);
+    });
+    if (!L.reduce.empty())
+        hipLaunchKernelGGL(k_lsqr_reduce_v<double>, dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
+                           L.d_reduce.as<ReduceWork>(), (const double*)partials<double>(L), ws.D.as<double>(),
+                           ws.v.as<double>(), L.reduce_sys(), (const LsqrScalars*)sc);
+    hipLaunchKernelGGL(k_lsmr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, L.reduce_sys(), sc,
+                       ws.lsmr.as<LsmrState>());
+    hipLaunchKernelGGL(k_lsmr_update<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
+                       g.d_cols.as<ColWork>(), ws.D.as<double>(), ws.v.as<double>(), ws.vd.as<double>(),
+                       ws.w.as<double>(), ws.hb.as<double>(), ws.y.as<double>(), (const LsqrScalars*)sc,
+                       (const LsmrState*)ws.lsmr.as<LsmrState>());
+    CAD_HIP(hipGetLastError());
+}
+
+static void lsmr_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
+                       SystemDetail& det)
+{
+    hipStream_t s = m->ctx->stream;
+    Workspace& ws = workspace(m, 400);
+    lsqr_alloc<double>(m, ws, s);
+    ensure(ws.hb, sizeof(double) * m->cols);
+    ensure(ws.lsmr, sizeof(LsmrState) * m->nsys);
+    fill<double>(ws.D.as<double>(), m->cols, 1.0, s);
+    const std::vector<double> atb = atb_norms(m, ws, b, s);
+    std::vector<int> act(m->nsys, 1);
+    set_active(m, ws, ws.scal, act, s);
+    fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
+    // x = 0: u = b / |b|, v = A^T u / |A^T u|, h = v (the LSQR start), hbar = 0
+    lsqr_start_from_residual<double>(m, ws, b, s);
+    fill<double>(ws.hb.as<double>(), m->cols, 0.0, s);
+    hipLaunchKernelGGL(k_lsmr_init, dim3(unsigned((m->nsys + 255) / 256)), dim3(256), 0, s, m->nsys,
+                       (const LsqrScalars*)ws.scal.as<LsqrScalars>(), ws.lsmr.as<LsmrState>());
+    CAD_HIP(hipGetLastError());
+    {
+        // the device stop: |A^T r| at tol / 3 of |A^T b| (none when tol = 0)
+        const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+        std::vector<LsqrScalars> hv(h, h + m->nsys);
+        for (auto& e : hv) {
+            e.target = o.tol > 0 ? o.tol / 3.0 : 0.0;
+            e.stop = 0;
+        }
+        CAD_HIP(hipMemcpyAsync(ws.scal.p, hv.data(), sizeof(LsqrScalars) * m->nsys, hipMemcpyHostToDevice, s));
+    }
+    const int batch = std::max(1, o.check_every);
+    int it = 0;
+    while (it < o.max_iterations) {
+        const int n = std::min(batch, o.max_iterations - it);
+        if (o.use_graph && batch > 1 && n == batch && s != nullptr) {
+            if (!ws.exec || ws.graph_iters != batch) {
+                if (ws.exec) (void)hipGraphExecDestroy(ws.exec);
+                if (ws.graph) (void)hipGraphDestroy(ws.graph);
+                ws.exec = nullptr;
+                ws.graph = nullptr;
+                prepare_scatter<double, double>(m);
+                prepare_gather<double, EpiLsqrU<double>>(m);
+                CAD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int i = 0; i < batch; i++) lsmr_iteration(m, ws, s);
+                CAD_HIP(hipStreamEndCapture(s, &ws.graph));
+                CAD_HIP(hipGraphInstantiate(&ws.exec, ws.graph, nullptr, nullptr, 0));
+                ws.graph_iters = batch;
+                ws.graph_prec = 8;
+            }
+            CAD_HIP(hipGraphLaunch(ws.exec, s));
+        } else {
+            for (int i = 0; i < n; i++) lsmr_iteration(m, ws, s);
+        }
+        it += n;
+        const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+        bool running = false;
+        for (int i = 0; i < m->nsys; i++) {
+            const double est = h[i].nres0 > 0 ? h[i].nres / h[i].nres0 : 0.0;
+            if (!std::isfinite(est)) throw Error(CAD_ERR_BREAKDOWN, "LSMR breakdown (non-finite recurrence)");
+            running |= h[i].active && h[i].stop == 0;
+        }
+        if (o.verbose && m->nsys == 1)
+            std::fprintf(stderr, "[lsmr] it %d est %.3e\n", it, h[0].nres0 > 0 ? h[0].nres / h[0].nres0 : 0.0);
+        if (!running) break;
+    }
+    CAD_HIP(hipMemcpyAsync(x, ws.y.p, sizeof(double) * m->cols, hipMemcpyDeviceToDevice, s));
+    std::vector<double> nres, rn;
+    explicit_residual(m, ws, x, b, nres, rn, s);
+    const LsqrScalars* h = read_scalars<LsqrScalars>(m, ws, ws.scal, s);
+    double worst = 0, rsum = 0;
+    det.rel.assign(m->nsys, 0.0);
+    det.iters.assign(m->nsys, it);
+    for (int i = 0; i < m->nsys; i++) {
+        det.rel[i] = atb[i] > 0 ? nres[i] / atb[i] : nres[i];
+        det.iters[i] = h[i].iters;
+        worst = std::max(worst, det.rel[i]);
+        rsum += rn[i] * rn[i];
+    }
+    rep.iterations = it;
+    rep.rel_normal_residual = worst;
+    rep.residual_norm = std::sqrt(rsum);
+    rep.converged = o.tol > 0 && worst <= o.tol;
+}
+
 // ------------------------------------------------------------ entry -------
 void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
            SystemDetail* detail)
@@ -1137,9 +1252,7 @@ void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o
         if (o.precision == CAD_PREC_FP32) pcg_solve<float>(m, b, x, o, rep, jac, det);
         else pcg_solve<double>(m, b, x, o, rep, jac, det);
     } else if (o.solver == CAD_SOLVER_LSMR) {
-        lsmr_solve(m, b, x, o, rep);
-        det.rel.assign(1, rep.rel_normal_residual);
-        det.iters.assign(1, rep.iterations);
+        lsmr_solve(m, b, x, o, rep, det);
     } else {
         throw Error(CAD_ERR_ARGUMENT, "unknown solver");
     }

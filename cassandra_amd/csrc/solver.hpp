@@ -27,7 +27,7 @@ struct ActiveLists {
 // the captured iteration graph.
 struct Workspace {
     int key = 0;
-    DBuf u, r64, v, w, y, x64, xprev, D, z64, vd, z, scal, scal_chk;
+    DBuf u, r64, v, w, y, x64, xprev, D, z64, vd, z, scal, scal_chk, hb, lsmr;
     DBuf P, p, q, zz, xr, rr, t, bvt, pscal, jac;
     int vd_prec = 0;
     hipGraph_t graph = nullptr;
@@ -56,7 +56,6 @@ struct SystemDetail {
 
 void solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep,
            SystemDetail* detail = nullptr);
-void lsmr_solve(cad_matrix* m, const double* b, double* x, const cad_solve_options& o, cad_solve_report& rep);
 double rel_normal_residual(cad_matrix* m, const double* b, const double* x, double* rnorm);
 void matrix_targets(cad_matrix* m, const double* xt, uint64_t seed, double sn, double* b);
 // Times the candidate schedules of the FP32 products on this matrix and

@@ -159,6 +159,18 @@ def test_cg_and_lsmr_match_oracle(ctx, oracle, prob):
     assert np.linalg.norm(oracle.spmv(idx, x.cpu().numpy()) - po) <= 1e-8 * np.linalg.norm(po)
 
 
+def test_lsmr_stops_on_device_at_tolerance(ctx, oracle, prob):
+    # LSMR's |A^T r_k| = |zetabar| estimate stops each system on the device at
+    # tol / 3; the FP64 metric at return, recomputed by the oracle, meets tol
+    idx, cols, A, b = prob
+    say("lsmr to tolerance")
+    x, rep = A.solve(b, solver=capi.SOLVER_LSMR, precision=capi.PREC_FP64, tol=1e-6, max_iterations=5000)
+    rel, _, _ = oracle.rel_normal_residual(idx, cols, b.cpu().numpy(), x.cpu().numpy())
+    say(f"  lsmr: {rep['iterations']} iterations, rel {rel:.3e}")
+    assert rep["converged"] and rel <= 1e-6 and rep["iterations"] < 5000
+    assert abs(rel - rep["rel_normal_residual"]) <= 1e-3 * rel
+
+
 def test_zero_rhs(ctx, prob):
     idx, cols, A, b = prob
     say("zero rhs")
