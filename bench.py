@@ -592,15 +592,25 @@ def main():
     # FETCH_SIZE / WRITE_SIZE passes over `bench.py --traffic` (scaled by an
     # elementwise copy of 1 GiB; see profiles/traffic.json)
     # (the scatter's traffic depends on its work order: measured for both)
+    # The figure is used only when it was measured on the kernel source this
+    # run uses (the SHA-256 of csrc/kernels.hpp recorded beside it); a
+    # kernel changed since reports null rather than a stale count.
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         try:
+            import hashlib
+
             tj = json.load(open(tpath))
+            src = os.path.join(ROOT, "cassandra_amd", "csrc", "kernels.hpp")
+            sha = hashlib.sha256(open(src, "rb").read()).hexdigest()
             key = dom_name.split()[0]
             if key == "k_scatter_tile":
                 key = f"{key}/order{A.schedule()['scatter_order']}"
-            traffic = tj.get(key, {}).get("total_bytes")
+            if tj.get("kernels_sha256") == sha:
+                traffic = tj.get(key, {}).get("total_bytes")
+            else:
+                log("traffic: profiles/traffic.json was measured on other kernel source; reporting null")
         except Exception:
             traffic = None
 

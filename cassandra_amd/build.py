@@ -21,6 +21,10 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # loops marked `#pragma unroll` unroll fully also when large (the
+    # matrix-free gather's slot loop indexes register arrays; partly unrolled,
+    # they would live in scratch memory)
+    "-mllvm", "-pragma-unroll-threshold=100000",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 

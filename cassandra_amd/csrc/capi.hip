@@ -156,9 +156,9 @@ void upload_programs(cad_patterns* p)
         d.dense_offset = s.dense_offset;
         slots.push_back(d);
     }
-    // the ternary-word programs: per list, its entries (3^j << 5 | 2 (q mod
-    // 16)) grouped by the word q / 16, each word's run padded to whole groups
-    // of four
+    // the ternary-word programs: per list, its entries {2 (q mod 16), 3^j}
+    // grouped by the word q / 16; three zero entries end the array (the
+    // kernels load four entries at a time, also at a list's last entry)
     std::vector<DevTern> tslots;
     std::vector<uint32_t> tern;
     for (const SlotProgram& s : g.slots) {
@@ -166,23 +166,23 @@ void upload_programs(cad_patterns* p)
         std::memset(&t, 0, sizeof(t));
         for (int l = 0; l < 3; l++) {
             const std::vector<uint8_t>& src = s.list[l].src;
-            t.off[l] = uint32_t(tern.size());
+            // the kernels multiply by 3^j in 24-bit arithmetic: 3^15 < 2^24
+            need(src.size() <= 16, "a pattern list of more than 16 squares", CAD_ERR_PATTERN);
+            t.off[l] = uint32_t(tern.size() / 2);
             for (int w = 0; w < 4; w++) {
                 size_t n = 0;
                 for (size_t j = 0; j < src.size(); j++)
                     if (src[j] / 16 == w) {
-                        tern.push_back((pow3(int(j)) << 5) | uint32_t(2 * (src[j] % 16)));
+                        tern.push_back(uint32_t(2 * (src[j] % 16)));
+                        tern.push_back(pow3(int(j)));
                         n++;
                     }
-                while (n % 4) {
-                    tern.push_back(0u);
-                    n++;
-                }
-                t.ngrp[l] |= uint32_t(n / 4) << (8 * w);
+                t.nent[l] |= uint32_t(n) << (8 * w);
             }
         }
         tslots.push_back(t);
     }
+    tern.insert(tern.end(), 6, 0u);
     hipStream_t st = p->ctx->stream;
     p->d_tslots = to_device(tslots, st);
     p->d_tern = to_device(tern, st);
@@ -207,12 +207,13 @@ void extract(cad_patterns* p, const uint64_t* pos, int64_t n, void* out, int byt
     if (n <= 0) return;
     const int k = p->group->k();
     hipStream_t s = p->ctx->stream;
+    const dim3 grid(grid1(n, kRowBlock * kExtractRows));
     if (bytes == 2)
-        hipLaunchKernelGGL(k_index_extract<uint16_t>, dim3(grid1(n, kRowBlock)), dim3(kRowBlock), 0, s, pos, n,
+        hipLaunchKernelGGL((k_index_extract<uint16_t, kExtractRows>), grid, dim3(kRowBlock), 0, s, pos, n,
                            p->d_slots.as<DevSlot>(), k, p->d_tslots.as<DevTern>(), p->d_tern.as<uint32_t>(),
                            p->d_dense.as<uint32_t>(), static_cast<uint16_t*>(out), slot_stride, row_stride, local);
     else
-        hipLaunchKernelGGL(k_index_extract<uint32_t>, dim3(grid1(n, kRowBlock)), dim3(kRowBlock), 0, s, pos, n,
+        hipLaunchKernelGGL((k_index_extract<uint32_t, kExtractRows>), grid, dim3(kRowBlock), 0, s, pos, n,
                            p->d_slots.as<DevSlot>(), k, p->d_tslots.as<DevTern>(), p->d_tern.as<uint32_t>(),
                            p->d_dense.as<uint32_t>(), static_cast<uint32_t*>(out), slot_stride, row_stride, local);
     CAD_HIP(hipGetLastError());
@@ -734,7 +735,8 @@ int cad_matrix_download(const cad_matrix* m, uint32_t* out)
             std::vector<uint32_t> t(size_t(R) * k);
             if (R > 0) {
                 DBuf tmp(t.size() * 4);
-                hipLaunchKernelGGL(k_index_extract<uint32_t>, dim3(grid1(R, kRowBlock)), dim3(kRowBlock), 0, s,
+                hipLaunchKernelGGL((k_index_extract<uint32_t, kExtractRows>), dim3(grid1(R, kRowBlock * kExtractRows)),
+                                   dim3(kRowBlock), 0, s,
                                    m->d_pos.as<uint64_t>(), R, m->d_slots.as<DevSlot>(), k,
                                    m->d_tslots.as<DevTern>(), m->d_tern.as<uint32_t>(), m->d_dense.as<uint32_t>(),
                                    tmp.as<uint32_t>(), R, int64_t(1), 0);

@@ -21,13 +21,14 @@ static_assert(sizeof(DevSlot) == 32, "DevSlot layout");
 // The same slot program in the form the matrix-free products evaluate: each
 // square list as entries of a ternary-word program.  A row's position is held
 // as four 32-bit "ternary words" (square q's digit P_q + 2 O_q in bits
-// 2(q mod 16) of word q / 16); entry e of a list reads word w's digit at shift
-// (e & 31) and weighs it by 3^j = e >> 5.  List l's entries start at off[l],
-// grouped by word (word w: ((ngrp[l] >> 8w) & 255) groups of four entries,
-// zero entries padding each word's run to whole groups).
+// 2(q mod 16) of word q / 16); an entry is two 32-bit words {shift, 3^j}: the
+// digit of word w at that shift, weighed by 3^j.  List l's entries start at
+// entry off[l], grouped by word (word w: (nent[l] >> 8w) & 255 entries), with
+// no padding between lists; the array ends with three zero entries, so that a
+// load of four entries at any list's last entry stays inside it.
 struct DevTern {
     uint32_t off[3];
-    uint32_t ngrp[3];
+    uint32_t nent[3];
     uint32_t pad[2];
 };
 static_assert(sizeof(DevTern) == 32, "DevTern layout");

@@ -180,30 +180,68 @@ __device__ __forceinline__ int uniform_u16(const uint16_t* p, int i)
     return int((w >> (16 * (i & 1))) & 0xFFFFu);
 }
 
+// Four entries of a ternary-word program (two words each: the digit's shift,
+// then 3^j) at a wave-uniform address, through the constant address space so
+// that they load into scalar registers with one 32-byte load (the kernels
+// never write the arrays read this way).
+struct TernEntries {
+    uint32_t w[8];
+};
+__device__ __forceinline__ TernEntries scalar_load_entries(const uint32_t* p)
+{
+    typedef const __attribute__((address_space(4))) uint32_t* ConstU32;
+    const ConstU32 c = (ConstU32)(p);
+    TernEntries e;
+#pragma unroll
+    for (int i = 0; i < 8; i++) e.w[i] = c[i];
+    return e;
+}
+
+// Adds entry q's term, digit x 3^j, to RPT rows (word W of each row's
+// ternary words): one bit-field extract (shift straight from the entry) and
+// one 24-bit multiply-add a row.
+template <int RPT, int W>
+__device__ __forceinline__ void tern_term(const TernEntries& e, int q, const uint32_t (&T)[RPT][4],
+                                          uint32_t (&v)[RPT])
+{
+#pragma unroll
+    for (int i = 0; i < RPT; i++) v[i] += __umul24((T[i][W] >> e.w[2 * q]) & 3u, e.w[2 * q + 1]);
+}
+
+// The n entries of word W from entry `off` on: whole groups of four, then the
+// last one to three behind uniform branches (no padding entries to evaluate).
+template <int RPT, int W>
+__device__ __forceinline__ uint32_t tern_word(const uint32_t* __restrict__ ent, uint32_t off, uint32_t n,
+                                              const uint32_t (&T)[RPT][4], uint32_t (&v)[RPT])
+{
+    for (uint32_t g = n >> 2; g > 0; g--, off += 4) {
+        const TernEntries e = scalar_load_entries(ent + 2 * off);
+#pragma unroll
+        for (int q = 0; q < 4; q++) tern_term<RPT, W>(e, q, T, v);
+    }
+    const uint32_t r = n & 3u;
+    if (r) {
+        const TernEntries e = scalar_load_entries(ent + 2 * off);
+        tern_term<RPT, W>(e, 0, T, v);
+        if (r > 1) tern_term<RPT, W>(e, 1, T, v);
+        if (r > 2) tern_term<RPT, W>(e, 2, T, v);
+    }
+    return off + r;
+}
+
 // The value sum_j 3^j (P[q_j] + 2 O[q_j]) of one square list for RPT rows,
-// from the list's ternary-word program: per entry one bit-field extract and
-// one multiply-add a row.  The entries are the same for the whole wave
-// (scalar loads, four at a time).
+// from the list's ternary-word program (3^j < 2^24: the host builds no list of
+// more than 16 squares).  The entries are the same for the whole wave.
 template <int RPT>
-__device__ __forceinline__ void tern_values(const uint32_t* __restrict__ ent, uint32_t off, uint32_t ngrp,
+__device__ __forceinline__ void tern_values(const uint32_t* __restrict__ ent, uint32_t off, uint32_t nent,
                                             const uint32_t (&T)[RPT][4], uint32_t (&v)[RPT])
 {
 #pragma unroll
     for (int i = 0; i < RPT; i++) v[i] = 0u;
-#pragma unroll
-    for (int w = 0; w < 4; w++) {
-        const uint32_t n = (ngrp >> (8 * w)) & 0xFFu;
-        for (uint32_t g = 0; g < n; g++, off += 4) {
-            const uint4 e = *reinterpret_cast<const uint4*>(ent + off);
-            const uint32_t ee[4] = {e.x, e.y, e.z, e.w};
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t sh = ee[q] & 31u, p3 = ee[q] >> 5;
-#pragma unroll
-                for (int i = 0; i < RPT; i++) v[i] += ((T[i][w] >> sh) & 3u) * p3;
-            }
-        }
-    }
+    off = tern_word<RPT, 0>(ent, off, nent & 0xFFu, T, v);
+    off = tern_word<RPT, 1>(ent, off, (nent >> 8) & 0xFFu, T, v);
+    off = tern_word<RPT, 2>(ent, off, (nent >> 16) & 0xFFu, T, v);
+    tern_word<RPT, 3>(ent, off, nent >> 24, T, v);
 }
 
 // slot_index for RPT rows from their ternary words (the same values).
@@ -212,7 +250,7 @@ __device__ __forceinline__ void slot_index_rows(const DevSlot& s, const DevTern&
                                                 const uint32_t* __restrict__ dense, const uint32_t (&T)[RPT][4],
                                                 uint32_t (&idx)[RPT])
 {
-    tern_values<RPT>(ent, t.off[0], t.ngrp[0], T, idx);
+    tern_values<RPT>(ent, t.off[0], t.nent[0], T, idx);
     if (s.kind == 0) return;
     if (s.kind == 3) {
 #pragma unroll
@@ -220,14 +258,14 @@ __device__ __forceinline__ void slot_index_rows(const DevSlot& s, const DevTern&
         return;
     }
     uint32_t b[RPT];
-    tern_values<RPT>(ent, t.off[1], t.ngrp[1], T, b);
+    tern_values<RPT>(ent, t.off[1], t.nent[1], T, b);
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
         const uint32_t lo = idx[i] < b[i] ? idx[i] : b[i], hi = idx[i] < b[i] ? b[i] : idx[i];
         idx[i] = lo * s.hs + hi - lo * (lo + 1u) / 2u;
     }
     if (s.kind == 2) {
-        tern_values<RPT>(ent, t.off[2], t.ngrp[2], T, b);
+        tern_values<RPT>(ent, t.off[2], t.nent[2], T, b);
 #pragma unroll
         for (int i = 0; i < RPT; i++) idx[i] = idx[i] * s.ds + b[i];
     }
@@ -244,27 +282,40 @@ __global__ void k_ternary_rows(const uint64_t* __restrict__ pos, int64_t n, uint
     }
 }
 
-// Pattern/Indexer.cpp InsertIndices for every row, one thread per position:
-// out[s * slot_stride + r * row_stride] = (local ? 0 : col_offset) + index,
-// each slot's index from the position's ternary words.
-template <class OutT>
+// Pattern/Indexer.cpp InsertIndices for every row, RPT positions a thread
+// (rows r0 + i kRowBlock, so that each slot's stores of a wave stay
+// contiguous): out[s * slot_stride + r * row_stride] = (local ? 0 :
+// col_offset) + index, each slot's index from the position's ternary words.
+template <class OutT, int RPT>
 __global__ __launch_bounds__(kRowBlock) void k_index_extract(
     const uint64_t* __restrict__ pos, int64_t n, const DevSlot* __restrict__ slots, int k,
     const DevTern* __restrict__ tslots, const uint32_t* __restrict__ tern, const uint32_t* __restrict__ dense,
     OutT* __restrict__ out, int64_t slot_stride, int64_t row_stride, int local)
 {
-    const int64_t r = blockIdx.x * int64_t(kRowBlock) + threadIdx.x;
-    if (r >= n) return;
-    uint32_t T[1][4];
-    ternary_words(pos[2 * r], pos[2 * r + 1], T[0]);
+    const int64_t r0 = blockIdx.x * int64_t(kRowBlock * RPT) + threadIdx.x;
+    if (r0 >= n) return;
+    uint32_t T[RPT][4];
+    bool ok[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        const int64_t r = r0 + int64_t(i) * kRowBlock;
+        ok[i] = r < n;
+        const int64_t rr = ok[i] ? r : r0;  // a valid row's words for the lanes past the end
+        ternary_words(pos[2 * rr], pos[2 * rr + 1], T[i]);
+    }
     for (int s = 0; s < k; s++) {
-        const DevSlot sl = slots[s];
-        uint32_t v[1];
-        slot_index_rows<1>(sl, tslots[s], tern, dense, T, v);
-        if (!local) v[0] += sl.col_offset;
-        out[int64_t(s) * slot_stride + r * row_stride] = OutT(v[0]);
+        const DevSlot sl = uniform_load(slots + s);
+        uint32_t v[RPT];
+        slot_index_rows<RPT>(sl, uniform_load(tslots + s), tern, dense, T, v);
+        const uint32_t base = local ? 0u : sl.col_offset;
+#pragma unroll
+        for (int i = 0; i < RPT; i++)
+            if (ok[i]) out[int64_t(s) * slot_stride + (r0 + int64_t(i) * kRowBlock) * row_stride] = OutT(base + v[i]);
     }
 }
+
+// Positions a thread of k_index_extract.
+constexpr int kExtractRows = 4;
 
 // Matrix-free view of a pattern matrix: the column of slot s in row r is
 // recomputed from the row's position (16 bytes) with the slot's program
