@@ -45,6 +45,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-fit at rel. residual 1e-10 (mixed+refine vs FP64 LSQR); SpMV HBM GB/s"
 HBM_PEAK_GBPS = 8000.0
+# 32-bit integer VALU issue of one MI355X: 256 CUs x 4 SIMDs x 16 lanes at
+# 2.4 GHz (a wave64 instruction takes a SIMD 4 cycles), in lane operations/s
+VALU_PEAK_OPS = 256 * 64 * 2.4e9
 
 
 def log(*a):
@@ -282,10 +285,44 @@ def leg_buckets(ctx):
             "max_rel_one_by_one": float(np.max(seq_rel)), "above_tol": off_tol}
 
 
-def leg_matrix_free(ctx):
+def mf_valu_ops(P):
+    """Algorithmic 32-bit VALU lane operations per row of the matrix-free
+    products: a bit-field extract and a multiply-add per square of every
+    slot's pattern variation (its index, sum_j 3^j digit_j), plus, per slot,
+    the column's LDS address and the accumulation (A x) or the atomic's
+    address (A^T y, whose LDS atomic is not a VALU operation)."""
+    squares = sum(bin(int(v)).count("1") for v in P.variations())
+    return {"squares_per_row": squares, "spmv": 2 * squares + 2 * P.k, "spmtv": 2 * squares + P.k}
+
+
+def mf_launches(ctx, F):
+    """Three launches each of the matrix-free FP32 A x and A^T y (for counter
+    runs: SQ_INSTS_VALU and the like per dispatch)."""
+    import ctypes as C_
+
+    import torch
+
+    import cassandra_amd.capi as capi
+
+    L = capi.lib()
+    hp = lambda t_: C_.c_void_p(t_.data_ptr())  # noqa: E731
+    x32 = torch.randn(F.cols, dtype=torch.float32, device="cuda")
+    y32 = torch.randn(F.rows, dtype=torch.float32, device="cuda")
+    z32 = torch.empty(F.cols, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        capi.check(L.cad_spmv_f32(F.h, hp(x32), hp(y32)))
+    for _ in range(3):
+        capi.check(L.cad_spmtv_f32(F.h, hp(y32), hp(z32)))
+    ctx.synchronize()
+    log(f"matrix-free launches done: A x and A^T y on {F.rows}x{F.cols} (k={F.k})")
+
+
+def leg_matrix_free(ctx, traffic=False):
     """20M positions over 15..45 empties, 12 patterns: the matrix-free mixed
     LSQR (indices recomputed in every product) against the stored-index FP64
-    LSQR on the same GPU."""
+    LSQR on the same GPU, and the products against the VALU issue rate (with
+    traffic set: only launch the products, for counter runs)."""
     import ctypes as C_
 
     import torch
@@ -298,6 +335,9 @@ def leg_matrix_free(ctx):
     P = Patterns(ctx, N.FIT12)
     pos = ctx.random_play(1005, n, 15, 45)
     F = P.matrix_free(pos)
+    if traffic:
+        mf_launches(ctx, F)
+        return None
     _, b = F.make_targets(2005, 3005)
     log(f"matrix-free: {F.rows} rows, k={F.k}, cols={F.cols}")
     F.solve(b, precision=capi.PREC_MIXED, tol=1e-10)  # warm
@@ -326,13 +366,21 @@ def leg_matrix_free(ctx):
     log(f"stored fp64: {t_s:.3f} s {rep_s}")
     st_spmv = event_time_ms(ctx, lambda: L.cad_spmv_f32(S.h, hp(x32), hp(y32)), 10)
     csr_u32 = S.rows * (4 * S.k + 4) + 4 * S.cols
+    ops = mf_valu_ops(P)
+    valu = {}
+    for name, ms in (("spmv", mf_spmv), ("spmtv", mf_spmtv)):
+        achieved = ops[name] * S.rows / (ms * 1e-3)
+        valu[name] = {"bound": "valu", "ops_per_row": ops[name], "achieved": achieved / 1e12,
+                      "peak": VALU_PEAK_OPS / 1e12, "unit": "T lane-ops/s", "frac": achieved / VALU_PEAK_OPS}
+    log(f"matrix-free VALU roofline ({ops['squares_per_row']} squares a row): {valu}")
     return {"rows": S.rows, "k": S.k, "cols": S.cols, "seconds_matrix_free_mixed": t_mf,
             "iterations_matrix_free": rep_f["iterations"], "rel_matrix_free": rep_f["rel_normal_residual"],
             "seconds_stored_fp64": t_s, "iterations_stored_fp64": rep_s["iterations"],
             "rel_stored_fp64": rep_s["rel_normal_residual"], "speedup": t_s / t_mf,
             "mf_spmv_us": mf_spmv * 1e3, "mf_spmtv_us": mf_spmtv * 1e3, "stored_spmv_us": st_spmv * 1e3,
             "mf_spmv_equiv_u32_csr_GBps": csr_u32 / (mf_spmv * 1e-3) / 1e9,
-            "index_bytes_stored": S.index_bytes}
+            "index_bytes_stored": S.index_bytes, "squares_per_row": ops["squares_per_row"],
+            "valu_roofline": valu}
 
 
 def leg_evaluator(ctx):
@@ -471,7 +519,8 @@ def main():
     ap.add_argument("--legs", default="reference_fit,buckets,evaluator,matrix_free",
                     help="extra workloads: reference_fit,buckets,evaluator,matrix_free (or empty)")
     ap.add_argument("--traffic", action="store_true",
-                    help="only launch the measured kernels a few times and a 1 GiB device copy (for counter runs)")
+                    help="only launch the measured kernels a few times and a 1 GiB device copy (for counter runs; "
+                         "with matrix_free among the legs: the matrix-free products instead)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default product schedules instead of timing the candidates")
     ap.add_argument("--schedule", default="",
@@ -491,6 +540,11 @@ def main():
     from cassandra_amd.fit import Context, Patterns
 
     ctx = Context(0)
+    legs = [x for x in args.legs.split(",") if x]
+    if args.traffic and "matrix_free" in legs:
+        leg_matrix_free(ctx, traffic=True)
+        ctx.close()
+        return
     t = time.perf_counter()
     pos = ctx.random_play(1002, args.rows, args.empties)
     P = Patterns(ctx, N.FIT8)

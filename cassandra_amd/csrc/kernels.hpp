@@ -156,9 +156,17 @@ __device__ __forceinline__ void ternary_words(uint64_t P, uint64_t O, uint32_t (
         T[w] = spread16(uint32_t(P >> (16 * w))) | (spread16(uint32_t(O >> (16 * w))) << 1);
 }
 
+// Read-only memory at a wave-uniform address through the constant address
+// space: scalar loads into scalar registers.  Only for arrays no kernel
+// writes (programs uploaded before the launch).  Scalar loads and LDS
+// operations share one wait counter, so a wait for one waits for the other.
+typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+
 // A descriptor that is the same for the whole wave, read as whole 32-bit
 // words into scalar registers (a struct with byte or half-word fields would
-// otherwise be read with vector loads and full waits).
+// otherwise be read with vector loads and full waits).  Vector loads and
+// lane-to-scalar moves: their wait does not also wait for the LDS operations
+// in flight, as a scalar load's would.
 template <class T>
 __device__ __forceinline__ T uniform_load(const T* p)
 {
@@ -189,8 +197,7 @@ struct TernEntries {
 };
 __device__ __forceinline__ TernEntries scalar_load_entries(const uint32_t* p)
 {
-    typedef const __attribute__((address_space(4))) uint32_t* ConstU32;
-    const ConstU32 c = (ConstU32)(p);
+    const ConstWords c = (ConstWords)(p);
     TernEntries e;
 #pragma unroll
     for (int i = 0; i < 8; i++) e.w[i] = c[i];
