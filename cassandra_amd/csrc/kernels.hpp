@@ -34,6 +34,31 @@ struct SlotMajor {
     {
         return base[s] + uint32_t(idx[int64_t(s) * ld + r]);
     }
+    // Slot s's entries, for a loop over rows (s the same for the whole wave:
+    // the pointer and base stay in scalar registers, and the rows' loads of
+    // several slots can all be in flight together).
+    struct Slot {
+        const IdxT* __restrict__ p;
+        uint32_t base;
+        __device__ __forceinline__ uint32_t col(int64_t r) const { return base + uint32_t(p[r]); }
+    };
+    __device__ __forceinline__ Slot slot(int s) const
+    {
+        return Slot{idx + int64_t(s) * ld, uint32_t(__builtin_amdgcn_readfirstlane(base[s]))};
+    }
+};
+
+template <class View>
+struct is_slot_major : std::false_type {};
+template <class IdxT>
+struct is_slot_major<SlotMajor<IdxT>> : std::true_type {};
+template <class View>
+struct SlotRefOf {
+    using type = int;
+};
+template <class IdxT>
+struct SlotRefOf<SlotMajor<IdxT>> {
+    using type = typename SlotMajor<IdxT>::Slot;
 };
 
 // Row-major int32 indices as torch hands them over ([B, k], contiguous).
@@ -1670,11 +1695,20 @@ __global__ __launch_bounds__(kScatterBlock) void k_jacobi_part(View m, const Sca
     const uint32_t W = w.win_hi - w.win_lo;
     for (uint32_t i = threadIdx.x; i < W; i += kScatterBlock) win[i] = 0u;
     __syncthreads();
-    const SlotGroup gr = groups[w.group];
+    const SlotGroup gr = uniform_load(groups + w.group);
     const uint32_t cb = sys_col[w.sys];
     const int ns = gr.nslots;
     // four rows a thread per step, all their entries loaded before any add
     constexpr int kRows = 4;
+    // stored indices: the group's slots are looked up once, outside the row
+    // loop (slots past the group's end repeat its first, unused)
+    using SlotRef = std::conditional_t<is_slot_major<View>::value, typename SlotRefOf<View>::type, int>;
+    SlotRef sref[kMaxSegSlots];
+    if constexpr (is_slot_major<View>::value) {
+#pragma unroll
+        for (int j = 0; j < kMaxSegSlots; j++) sref[j] = m.slot(uniform_u16(slot_list, gr.first + (j < ns ? j : 0)));
+    }
+    const int64_t last = w.row_end - 1;
     for (int64_t r0 = w.row_begin + threadIdx.x; r0 < w.row_end; r0 += kRows * kScatterBlock) {
         uint32_t c[kRows][kMaxSegSlots];
 #pragma unroll
@@ -1682,8 +1716,17 @@ __global__ __launch_bounds__(kScatterBlock) void k_jacobi_part(View m, const Sca
             const int64_t r = r0 + int64_t(u) * kScatterBlock;
             const bool valid = r < w.row_end;
 #pragma unroll
-            for (int j = 0; j < kMaxSegSlots; j++)
-                c[u][j] = (valid && j < ns) ? cb + m.col(slot_list[gr.first + j], r) : 0xFFFFFFFFu;
+            for (int j = 0; j < kMaxSegSlots; j++) {
+                if constexpr (is_slot_major<View>::value) {
+                    // every load issued (a clamped row, the group's first slot
+                    // past its end), then the unused ones dropped: no branch
+                    // around a load, so all of them are in flight together
+                    const uint32_t v = cb + sref[j].col(min(r, last));
+                    c[u][j] = (valid && j < ns) ? v : 0xFFFFFFFFu;
+                } else {
+                    c[u][j] = (valid && j < ns) ? cb + m.col(slot_list[gr.first + j], r) : 0xFFFFFFFFu;
+                }
+            }
         }
 #pragma unroll
         for (int u = 0; u < kRows; u++) {
