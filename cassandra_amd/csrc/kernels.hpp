@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kRowBlock) void k_index_extract(
 }
 
 // Positions a thread of k_index_extract.
-constexpr int kExtractRows = 4;
+constexpr int kExtractRows = 8;
 
 // Matrix-free view of a pattern matrix: the column of slot s in row r is
 // recomputed from the row's position (16 bytes) with the slot's program
@@ -1680,8 +1680,8 @@ __global__ __launch_bounds__(kRowBlock) void k_scatter_reduce(const ReduceWork* 
 // Jacobi of A^T A (Math/MatrixCSR.cpp:21-47): diag_c = sum over rows of the
 // squared multiplicity of c in the row.  A row's columns of one slot group
 // (repeats can only occur within a group, of at most kMaxSegSlots slots) are
-// loaded once into registers; the first slot holding a column adds mult^2.
-// Counts are exact 32-bit integers.
+// loaded once into registers, and the occurrences of a column add up to its
+// mult^2.  Counts are exact 32-bit integers.
 template <class View>
 __global__ __launch_bounds__(kScatterBlock) void k_jacobi_part(View m, const ScatterWork* __restrict__ work,
                                                                const SlotGroup* __restrict__ groups,
@@ -1728,20 +1728,19 @@ __global__ __launch_bounds__(kScatterBlock) void k_jacobi_part(View m, const Sca
                 }
             }
         }
+        // a column that m slots of the row hold must gain m^2 = 1 + 3 + ...
+        // + (2m - 1): its k-th occurrence adds 2k - 1, i.e. one plus twice
+        // the number of earlier slots holding it
 #pragma unroll
         for (int u = 0; u < kRows; u++) {
 #pragma unroll
             for (int j = 0; j < kMaxSegSlots; j++) {
                 const uint32_t uc = c[u][j] - w.win_lo;
                 if (c[u][j] == 0xFFFFFFFFu || uc >= W) continue;
-                uint32_t mult = 0;
-                bool first = true;
+                uint32_t before = 0;
 #pragma unroll
-                for (int t = 0; t < kMaxSegSlots; t++) {
-                    mult += c[u][t] == c[u][j] ? 1u : 0u;
-                    if (t < j && c[u][t] == c[u][j]) first = false;
-                }
-                if (first) atomicAdd(&win[uc], mult * mult);
+                for (int t = 0; t < j; t++) before += c[u][t] == c[u][j] ? 1u : 0u;
+                atomicAdd(&win[uc], 2u * before + 1u);
             }
         }
     }
