@@ -108,12 +108,14 @@ def event_time_ms(ctx, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def cpu_baseline(A, b, fp64_iterations, budget_s, x_gpu):
+def cpu_baseline(A, b, fp64_iterations, budget_s, x_gpu, pos_host=None, masks=None):
     """The oracle's FP64 LSQR (column-scaled, OpenMP over all host threads) on
     the same matrix.  When the iterations the GPU FP64 fit needed fit in the
     budget they all run, the time is measured whole, and the CPU solution is
     the FP64 reference for the GPU weights; otherwise a bounded sample of
-    iterations is timed and scaled."""
+    iterations is timed and scaled.  Beside it, timed on the same cores: the
+    oracle's index build of every position (GroupIndexer, compared with the
+    device matrix entry for entry) and its Jacobi count pass."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -121,14 +123,24 @@ def cpu_baseline(A, b, fp64_iterations, budget_s, x_gpu):
 
     idx = A.download()
     bh = b.cpu().numpy()
+    steps = {}
+    if pos_host is not None:
+        G = O.Group(masks)
+        t0 = time.perf_counter()
+        ih = G.indices(pos_host)
+        steps["index_build_s"] = time.perf_counter() - t0
+        steps["index_build_matches_gpu"] = bool(np.array_equal(ih, idx))
+        del ih
+    t0 = time.perf_counter()
     d = O.jacobi(idx, A.cols).astype(np.float64)
+    steps["jacobi_s"] = time.perf_counter() - t0
     ok = np.isfinite(d) & (d > 0)
     D = np.where(ok, 1.0 / np.sqrt(np.rint(1.0 / np.where(ok, d, 1.0))), 1.0)
     t0 = time.perf_counter()
     O.lsqr(idx, A.cols, bh, 2, colscale=D)
     per = (time.perf_counter() - t0) / 2
     cores = omp_threads()
-    out = {"unit": "s", "cores": cores, "kind": "port", **cpu_info()}
+    out = {"unit": "s", "cores": cores, "kind": "port", **cpu_info(), **steps}
     if per * fp64_iterations <= budget_s:
         t0 = time.perf_counter()
         xc, _, _ = O.lsqr(idx, A.cols, bh, fp64_iterations, colscale=D)
@@ -567,6 +579,7 @@ def main():
                    "frac": ex_bytes / (ex_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS}
         log(f"index extraction: {extract}")
         del out_idx
+    pos_host = None if (args.traffic or args.no_cpu) else pos.cpu().numpy()
     del pos
     t = time.perf_counter()
     tune_log = A.autotune() if args.autotune else ""
@@ -639,6 +652,15 @@ def main():
     jacobi = {"kernel": "k_jacobi_part", "launch_us": kt["jacobi_part"], "algorithmic_bytes": jac_bytes,
               "achieved_GBps": jac_bytes / (kt["jacobi_part"] * 1e-6) / 1e9,
               "frac": jac_bytes / (kt["jacobi_part"] * 1e-6) / 1e9 / HBM_PEAK_GBPS}
+    # the measured ceiling: a device-to-device copy of 1 GiB (torch's copy
+    # kernel on the library's stream), read + write bytes over its time
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    with torch.cuda.stream(ctx.torch_stream()):
+        copy_ms = event_time_ms(ctx, lambda: dst.copy_(src), 10)
+    copy_gbps = 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9
+    del src, dst
+    log(f"device copy of 1 GiB: {copy_ms * 1e3:.0f} us, {copy_gbps:.0f} GB/s")
     cands = [("k_gather_tile (A x)", kt["gather"]), ("k_scatter_tile (A^T y windows)", kt["scatter_tiles"])]
     dom_name, dom_us = max(cands, key=lambda c: c[1])
     dom_gbps = bytes_pass / (dom_us * 1e-6) / 1e9
@@ -670,7 +692,7 @@ def main():
 
     cpu = None
     if not args.no_cpu and fp64 is not None:
-        cpu = cpu_baseline(A, b, fp64["iterations"], args.cpu_budget, x)
+        cpu = cpu_baseline(A, b, fp64["iterations"], args.cpu_budget, x, pos_host, N.FIT8)
         log(f"cpu: {cpu}")
     shape = dict(rows=A.rows, k=A.k, cols=A.cols)
     del A, b, x
@@ -696,12 +718,13 @@ def main():
         "jacobi": jacobi,
         "kernels": {"spmv_us": spmv_ms * 1e3, "spmv_GBps": spmv_gbps, "spmtv_us": spmtv_ms * 1e3,
                     "spmtv_GBps": spmtv_gbps, "bytes_per_pass": bytes_pass, "index_bytes": ib,
-                    "kernel_us": kt,
+                    "kernel_us": kt, "device_copy_GBps": copy_gbps,
                     # the same pass in the reference's u32 CSR layout, over the same time
                     "spmv_equiv_u32_GBps": bytes_u32 / (spmv_ms * 1e-3) / 1e9,
                     "spmtv_equiv_u32_GBps": bytes_u32 / (spmtv_ms * 1e-3) / 1e9},
         "roofline": {"kernel": dom_name, "launch_us": dom_us, "bound": "hbm", "achieved": dom_gbps,
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": dom_gbps / HBM_PEAK_GBPS, "traffic": traffic},
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": dom_gbps / HBM_PEAK_GBPS, "traffic": traffic,
+                     "measured_copy_GBps": copy_gbps},
         "schedule": tune_log.strip().splitlines()[-1] if tune_log else "default",
         "cpu_baseline": cpu,
         "reference_fit": reference_fit,
