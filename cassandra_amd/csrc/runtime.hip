@@ -73,6 +73,24 @@ namespace {
 
 constexpr uint32_t kReduceCols = kRowBlock;  // columns of one scatter-reduce work item
 
+// Row chunks of one window of a system: the share of the target tile count,
+// capped so that the partial slices stay near partial_cap of the window's
+// index traffic -- and, for a small system that cap would leave on a few
+// CUs, raised towards one tile per CU while its slices stay under
+// kPartialFloorBytes a window (their write and read then cost less than the
+// parallelism gains).  At least kScatterBlock rows a chunk.
+constexpr double kPartialFloorBytes = 2.0 * 1024 * 1024;
+
+int64_t window_chunks(int64_t R, double share_chunks, double cap_chunks, int cus, int windows, uint32_t W,
+                      int vbytes)
+{
+    int64_t C = std::max<int64_t>(1, std::min<int64_t>(std::llround(share_chunks), int64_t(cap_chunks)));
+    const int64_t par = std::max<int64_t>(1, cus / std::max(1, windows));
+    const int64_t floor_c = std::max<int64_t>(1, int64_t(kPartialFloorBytes / (2.0 * double(W) * vbytes)));
+    C = std::max(C, std::min(par, floor_c));
+    return std::max<int64_t>(1, std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock));
+}
+
 // Group slots whose column ranges overlap (transitively).
 void make_groups(const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi, ScatterPlan& sp)
 {
@@ -161,7 +179,7 @@ constexpr int kXcds = 8;
 
 void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
                             const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, uint32_t wmax, int vbytes,
-                            int ibytes, double target, const TuneParams& tune, ScatterLists& L)
+                            int ibytes, double target, int cus, const TuneParams& tune, ScatterLists& L)
 {
     (void)sp;
     const int nsys = int(sys_col.size());
@@ -204,11 +222,10 @@ void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& s
         std::vector<int64_t> woff(size_t(std::max(1, nw)));
         if (R > 0 && nw > 0) {
             const double share = total_rows > 0 ? double(R) / double(total_rows) : 1.0;
-            C = std::max<int64_t>(1, std::llround(double(chunks) * share));
             const double cap = tune.partial_cap * double(R) * work_per_row * ibytes /
                                (2.0 * width * vbytes * double(tiles_per_chunk) / std::max(1, nw));
-            C = std::max<int64_t>(1, std::min<int64_t>(C, int64_t(cap)));
-            C = std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock);
+            C = window_chunks(R, double(chunks) * share, cap, cus, tiles_per_chunk, uint32_t(width / std::max(1, nw)),
+                              vbytes);
             const int64_t chunk = (R + C - 1) / C;
             C = (R + chunk - 1) / chunk;
             for (int j = 0; j < nw; j++) {
@@ -293,9 +310,11 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
                 total += double(R) * win_cost(g, std::min(g.col_hi, wl + wmax) - wl);
         }
     const double target = tune.blocks_per_cu * std::max(cus, 1);
+    int windows = 0;
+    for (auto& g : L.groups) windows += int((g.col_hi - g.col_lo + wmax - 1) / wmax);
     int64_t off = 0;
     if (tune.scatter_order == 1) {
-        make_scatter_lists_xcd(sp, sys_rows, sys_col, cols_per_sys, wmax, vbytes, ibytes, target, tune, L);
+        make_scatter_lists_xcd(sp, sys_rows, sys_col, cols_per_sys, wmax, vbytes, ibytes, target, cus, tune, L);
         return;
     }
     for (int s = 0; s < nsys; s++) {
@@ -311,10 +330,8 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
                 int64_t C = 0;
                 if (R > 0) {
                     const double share = total > 0 ? double(R) * win_cost(g, W) / total : 1.0;
-                    C = std::max<int64_t>(1, int64_t(std::llround(target * share)));
                     const double cap = tune.partial_cap * double(R) * g.nslots * ibytes / (2.0 * W * vbytes);
-                    C = std::max<int64_t>(1, std::min<int64_t>(C, int64_t(cap)));
-                    C = std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock);
+                    C = window_chunks(R, target * share, cap, cus, windows, W, vbytes);
                     const int64_t chunk = (R + C - 1) / C;
                     C = (R + chunk - 1) / chunk;
                     for (int64_t c = 0; c < C; c++) {
@@ -545,7 +562,11 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int b
     // first row rounded down to rpt, so a tile spans at most sub * bs * rpt
     // rows from there; tiles of a system are a multiple of rpt rows long (their
     // misalignment is the system's).  A single system is cut into a multiple of
-    // the CU count so that every CU gets the same number of tiles.
+    // the CU count so that every CU gets the same number of tiles.  In a batch,
+    // a small system is cut into tiles of about kSmallTileRows rows (up to
+    // kSmallTiles of them): once the large systems have converged, its
+    // iterations run alone, and full tiles would leave all but a few CUs idle.
+    constexpr int64_t kSmallTileRows = 2048, kSmallTiles = 32;
     const int64_t cap = int64_t(sub) * bs * rpt - (rpt - 1);
     gp.tiles.clear();
     gp.first.assign(m->nsys, 0);
@@ -555,6 +576,7 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int b
         gp.first[s] = int32_t(gp.tiles.size());
         int64_t n = (R + cap - 1) / cap;
         if (m->nsys == 1 && n > 0) n = (n + ctx.cus * per_cu - 1) / (ctx.cus * per_cu) * (ctx.cus * per_cu);
+        if (m->nsys > 1) n = std::max(n, std::min((R + kSmallTileRows - 1) / kSmallTileRows, kSmallTiles));
         int64_t per = n > 0 ? (R + n - 1) / n : 0;
         per = std::min(cap / rpt * rpt, (per + rpt - 1) / rpt * rpt);
         for (int64_t t = 0; per > 0 && t * per < R; t++)

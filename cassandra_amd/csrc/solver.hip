@@ -268,9 +268,10 @@ RowList row_list(cad_matrix* m, const ActiveLists* al)
 {
     GatherPlan& gp = gather_plan<VT>(m);
     if (al) {
-        const GridSys base = gp.ok ? gp.sys() : m->grid.row_sys();
+        const GridSys base = gp.ok && !al->row_kernel ? gp.sys() : m->grid.row_sys();
         return RowList{al->tiles.as<RowWork>(), al->ntiles,
-                       GridSys{al->tfirst.as<int32_t>(), al->tcount.as<int32_t>(), base.counters, base.partials}};
+                       GridSys{al->tfirst.as<int32_t>(), al->tcount.as<int32_t>(), base.counters,
+                               al->row_kernel ? al->gpart.as<double>() : base.partials}};
     }
     if (gp.ok) return RowList{gp.d_tiles.as<RowWork>(), int(gp.tiles.size()), gp.sys()};
     return RowList{m->grid.d_rows.as<RowWork>(), int(m->grid.rows.size()), m->grid.row_sys()};
@@ -278,7 +279,7 @@ RowList row_list(cad_matrix* m, const ActiveLists* al)
 
 // Row gather with an epilogue: the LDS-staged tile kernel for pattern
 // matrices, the generic row kernel otherwise.  al: launch only the blocks of
-// the systems it holds.
+// the systems it holds (with the row kernel when it says so).
 template <class VT, class Epi>
 void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s, const ActiveLists* al = nullptr)
 {
@@ -288,7 +289,7 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s, const ActiveList
     with_view(m, [&](auto view) {
         using View = decltype(view);
         using IdxT = typename View::index_type;
-        if (gp.ok) {
+        if (gp.ok && !(al && al->row_kernel)) {
             const size_t lds = size_t(gp.lds_elems) * sizeof(VT);
             with_rpt<VT, View>(gp.rpt, [&](auto rc) {
                 constexpr int RPT = decltype(rc)::value;
@@ -523,15 +524,39 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s, const Ac
     CAD_HIP(hipGetLastError());
 }
 
+// Rows of the running systems up to which their LSQR iterations gather
+// with the row kernel rather than the staged tiles.
+constexpr int64_t kRowKernelRows = 262144;
+
 // The iteration lists of the systems in `on` (see ActiveLists).
 template <class VT>
 static void build_active(cad_matrix* m, ActiveLists& al, const std::vector<int>& on, hipStream_t s)
 {
     const int n = m->nsys;
     GatherPlan& gp = gather_plan<VT>(m);
-    const std::vector<RowWork>& tiles = gp.ok ? gp.tiles : m->grid.rows;
-    const std::vector<int32_t>& tf = gp.ok ? gp.first : m->grid.row_first;
-    const std::vector<int32_t>& tc = gp.ok ? gp.count : m->grid.row_count;
+    // A tile of the staged gather costs about the same whatever its rows (it
+    // stages every x window of the pattern set into LDS): once the running
+    // systems are small, the row kernel (x read through L2, the same row sums
+    // in the same order) finishes sooner.
+    int64_t running_rows = 0;
+    for (int i = 0; i < n; i++)
+        if (on[i]) running_rows += m->sys_row_begin[i + 1] - m->sys_row_begin[i];
+    al.row_kernel = !gp.ok || running_rows <= kRowKernelRows;
+    // the row kernel's blocks: one row a thread (its rows' loads are then all
+    // in flight at once), with partials of their own
+    std::vector<RowWork> fine;
+    std::vector<int32_t> fine_first(n, 0), fine_count(n, 0);
+    if (al.row_kernel)
+        for (int i = 0; i < n; i++) {
+            fine_first[i] = int32_t(fine.size());
+            if (on[i])
+                for (int64_t r = m->sys_row_begin[i]; r < m->sys_row_begin[i + 1]; r += kRowBlock)
+                    fine.push_back(RowWork{r, std::min(m->sys_row_begin[i + 1], r + int64_t(kRowBlock)), i, 0});
+            fine_count[i] = int32_t(fine.size()) - fine_first[i];
+        }
+    const std::vector<RowWork>& tiles = al.row_kernel ? fine : gp.tiles;
+    const std::vector<int32_t>& tf = al.row_kernel ? fine_first : gp.first;
+    const std::vector<int32_t>& tc = al.row_kernel ? fine_count : gp.count;
     ScatterLists& L = lists<VT>(m->splan);
     std::vector<RowWork> t;
     std::vector<int32_t> t_first(n, 0), t_count(n, 0), r_first(n, 0), r_count(n, 0);
@@ -560,6 +585,7 @@ static void build_active(cad_matrix* m, ActiveLists& al, const std::vector<int>&
     al.nreduce = int(rw.size());
     al.ncols = int(cw.size());
     al.tiles = to_device(t, s);
+    if (al.row_kernel) al.gpart = DBuf(2 * sizeof(double) * std::max<size_t>(1, t.size()));
     al.tfirst = to_device(t_first, s);
     al.tcount = to_device(t_count, s);
     al.scatter = to_device(sw, s);

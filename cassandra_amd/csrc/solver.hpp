@@ -16,7 +16,9 @@ struct ActiveLists {
     std::vector<uint8_t> on;      // per system: in the lists
     int64_t rows = 0;             // rows of those systems
     int ntiles = 0, nscatter = 0, nreduce = 0, ncols = 0;
+    bool row_kernel = false;      // the gather blocks are row blocks of the generic row kernel
     DBuf tiles, tfirst, tcount;   // gather blocks and each system's range of them
+    DBuf gpart;                   // the row kernel's block partials
     DBuf scatter;                 // A^T y window tiles
     DBuf reduce, rfirst, rcount;  // column reduction items and each system's range
     DBuf cols;                    // column blocks
