@@ -406,7 +406,7 @@ def leg_evaluator(ctx):
 
     import cassandra_amd.capi as capi
     from cassandra_amd import names as N
-    from cassandra_amd.evaluator import huber_fit
+    from cassandra_amd.evaluator import huber_fit, huber_fit_fused
     from cassandra_amd.fit import Patterns
 
     P = Patterns(ctx, N.FIT4)
@@ -424,6 +424,15 @@ def leg_evaluator(ctx):
     torch.cuda.synchronize()
     t_epoch = time.perf_counter() - t0
     log(f"evaluator: epoch {t_epoch:.3f} s, loss {losses}")
+    # the same epoch with every step one library call (forward with the loss
+    # and its gradient, backward through the minibatch rows in place, Adam)
+    huber_fit_fused(P, idx[: 4 * B], target[: 4 * B], epochs=1, batch=B)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, losses_f = huber_fit_fused(P, idx, target, epochs=1, batch=B)
+    torch.cuda.synchronize()
+    t_epoch_f = time.perf_counter() - t0
+    log(f"evaluator: fused epoch {t_epoch_f:.4f} s, loss {losses_f}")
 
     # one minibatch through the C-ABI on the library's stream, kernels alone
     L = capi.lib()
@@ -477,6 +486,8 @@ def leg_evaluator(ctx):
                                      rel_normal_residual=rep_ls["rel_normal_residual"])}
     log(f"evaluator fits: {fit_cmp}")
     return {"positions": n, "minibatch": B, "epoch_seconds": t_epoch, "positions_per_s": n / t_epoch,
+            "epoch_seconds_fused_step": t_epoch_f, "positions_per_s_fused_step": n / t_epoch_f,
+            "loss_epoch_fused_step": losses_f[0],
             "loss_epoch": losses[0], "bytes_per_minibatch": nbytes,
             "kernel_us": {"forward": fwd_ms * 1e3, "backward": bwd_ms * 1e3, "backward_noalloc": bwd_na_ms * 1e3,
                           "resident_eval_positions": res_ms * 1e3},

@@ -1,5 +1,6 @@
 // extern "C" boundary of libcassandra_amd.so (declared in include/cassandra_amd.h).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -251,8 +252,8 @@ const ScatterPlan& eval_plan(cad_patterns* p, int64_t B)
     return ref;
 }
 
-template <class VT>
-void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, VT* grad)
+template <class VT, class View>
+void eval_backward_view(cad_patterns* p, const View& view, int64_t B, const VT* g, VT* grad)
 {
     hipStream_t s = p->ctx->stream;
     const int64_t C = int64_t(p->group->cols());
@@ -268,18 +269,56 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
     VT* part = L.part.as<VT>();
     if (!L.work.empty()) {
         const size_t lds = size_t(L.widest) * sizeof(double);
-        auto kern = k_scatter_part<VT, VT, RowMajor32>;
+        auto kern = k_scatter_part<VT, VT, View>;
         if (lds > 64 * 1024)
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s,
-                           RowMajor32{idx, p->group->k()}, L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(),
+                           view, L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(),
                            L.d_slot_list.as<uint16_t>(), p->d_syscol0.as<uint32_t>(), g,
                            (const LsqrScalars*)nullptr, kScatterAll, FixedScale{nullptr, 0}, 0, part);
         CAD_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL((k_scatter_reduce<VT, VT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
                        L.d_reduce.as<ReduceWork>(), (const VT*)part, grad, (const LsqrScalars*)nullptr, kScatterAll);
+    CAD_HIP(hipGetLastError());
+}
+
+template <class VT>
+void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, VT* grad)
+{
+    eval_backward_view<VT>(p, RowMajor32{idx, p->group->k()}, B, g, grad);
+}
+
+// One Adam step of the Huber-loss fit on the minibatch rows sel[0..B): the
+// forward with the loss and its gradient, the backward (the evaluator's
+// scatter plan, rows read through sel), the Adam update.
+void train_step(cad_patterns* p, const int32_t* idx, const float* target, const int64_t* sel, int64_t B,
+                float delta, float lr, float b1, float b2, float eps, int64_t step, float* w, float* m, float* v,
+                double* loss)
+{
+    hipStream_t s = p->ctx->stream;
+    const int64_t C = int64_t(p->group->cols());
+    const int k = p->group->k();
+    const int nblk = grid1(B, kRowBlock);
+    auto grow = [](DBuf& b, size_t bytes) {
+        if (b.bytes < bytes) b = DBuf(bytes);
+    };
+    grow(p->d_train_g, sizeof(float) * size_t(std::max<int64_t>(1, B)));
+    grow(p->d_train_grad, sizeof(float) * size_t(C));
+    grow(p->d_train_part, sizeof(double) * size_t(nblk));
+    float* g = p->d_train_g.as<float>();
+    float* grad = p->d_train_grad.as<float>();
+    double* part = p->d_train_part.as<double>();
+    if (B > 0)
+        hipLaunchKernelGGL(k_train_forward, dim3(nblk), dim3(kRowBlock), 0, s, idx, sel, B, k, w, uint32_t(C), target,
+                           delta, g, part);
+    eval_backward_view<float>(p, RowMajor32Sel{idx, sel, k}, B, g, grad);
+    const float bias1 = 1.0f - std::pow(b1, float(step));
+    const float bias2_sqrt = std::sqrt(1.0f - std::pow(b2, float(step)));
+    hipLaunchKernelGGL(k_adam, dim3(grid1(C, kRowBlock)), dim3(kRowBlock), 0, s, w, (const float*)grad, m, v, C, lr,
+                       b1, b2, eps, bias1, bias2_sqrt, (const double*)part, B > 0 ? nblk : 0, std::max<int64_t>(1, B),
+                       loss);
     CAD_HIP(hipGetLastError());
 }
 
@@ -905,6 +944,21 @@ int cad_eval_backward_f64(cad_patterns* p, const int32_t* idx, int64_t B, const 
         need(p && grad && B >= 0 && (B == 0 || (idx && g)), "invalid argument");
         need(p->ctx != nullptr, "host-only pattern group (created without a context)");
         eval_backward<double>(p, idx, B, g, grad);
+    });
+}
+
+int cad_eval_train_step_f32(cad_patterns* p, const int32_t* idx, const float* target, const int64_t* sel, int64_t B,
+                            float delta, float lr, float beta1, float beta2, float eps, int64_t step, float* w,
+                            float* m, float* v, double* loss)
+{
+    return guarded([&] {
+        need(p && w && m && v && loss && B >= 0 && (B == 0 || (idx && target && sel)), "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        need(step >= 1, "Adam steps count from 1");
+        need(delta > 0.0f && lr > 0.0f && beta1 >= 0.0f && beta1 < 1.0f && beta2 >= 0.0f && beta2 < 1.0f &&
+                 eps >= 0.0f,
+             "invalid Huber or Adam parameter");
+        train_step(p, idx, target, sel, B, delta, lr, beta1, beta2, eps, step, w, m, v, loss);
     });
 }
 

@@ -71,6 +71,18 @@ struct RowMajor32 {
     }
 };
 
+// The rows sel[0..B) of a row-major int32 index matrix [N, k] (a shuffled
+// minibatch, read in place).
+struct RowMajor32Sel {
+    const int32_t* __restrict__ idx;
+    const int64_t* __restrict__ sel;
+    int k;
+    __device__ __forceinline__ uint32_t col(int s, int64_t r) const
+    {
+        return uint32_t(idx[sel[r] * k + s]);
+    }
+};
+
 // ------------------------------------------------------------ reductions ---
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -2281,6 +2293,63 @@ __global__ __launch_bounds__(kRowBlock) void k_eval_forward(const int32_t* __res
         if (c < cols) acc += w[c];
     }
     out[r] = acc;
+}
+
+// One Huber-loss training step, forward half: for the minibatch rows sel[b]
+// the prediction (the evaluator's gather-sum, slot order from zero), the
+// Huber loss of pred - target (0.5 d^2 inside [-delta, delta], delta (|d| -
+// delta/2) outside) and its gradient for the mean over the batch,
+// g[b] = clamp(d, -delta, delta) / B; the block sums of the losses (FP64) go
+// to part[blockIdx.x].
+__global__ __launch_bounds__(kRowBlock) void k_train_forward(const int32_t* __restrict__ idx,
+                                                             const int64_t* __restrict__ sel, int64_t B, int k,
+                                                             const float* __restrict__ w, uint32_t cols,
+                                                             const float* __restrict__ target, float delta,
+                                                             float* __restrict__ g, double* __restrict__ part)
+{
+    __shared__ double sh[kRowBlock / 64];
+    const int64_t b = blockIdx.x * int64_t(kRowBlock) + threadIdx.x;
+    double loss = 0.0;
+    if (b < B) {
+        const int64_t r = sel[b];
+        const int32_t* __restrict__ row = idx + r * k;
+        float acc = 0.0f;
+        for (int s = 0; s < k; s++) {
+            const uint32_t c = uint32_t(row[s]);
+            if (c < cols) acc += w[c];
+        }
+        const float d = acc - target[r];
+        const float ad = fabsf(d);
+        loss = ad < delta ? 0.5 * double(d) * double(d) : double(delta) * (double(ad) - 0.5 * double(delta));
+        g[b] = fminf(fmaxf(d, -delta), delta) / float(B);
+    }
+    loss = block_sum<kRowBlock>(loss, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = loss;
+}
+
+// The step's Adam update (torch.optim.Adam without weight decay or amsgrad:
+// m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, w -= lr / (1 - b1^t) * m /
+// (sqrt(v) / sqrt(1 - b2^t) + eps)); block 0 also adds the step's mean loss
+// (the forward's block sums in block order, / B) to *loss.
+__global__ __launch_bounds__(kRowBlock) void k_adam(float* __restrict__ w, const float* __restrict__ grad,
+                                                    float* __restrict__ m, float* __restrict__ v, int64_t n,
+                                                    float lr, float b1, float b2, float eps, float bias1,
+                                                    float bias2_sqrt, const double* __restrict__ part, int nparts,
+                                                    int64_t B, double* __restrict__ loss)
+{
+    for (int64_t c = blockIdx.x * int64_t(kRowBlock) + threadIdx.x; c < n; c += int64_t(gridDim.x) * kRowBlock) {
+        const float gc = grad[c];
+        const float mc = b1 * m[c] + (1.0f - b1) * gc;
+        const float vc = b2 * v[c] + (1.0f - b2) * gc * gc;
+        m[c] = mc;
+        v[c] = vc;
+        w[c] -= (lr / bias1) * mc / (sqrtf(vc) / bias2_sqrt + eps);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < nparts; i++) s += part[i];
+        *loss += s / double(B);
+    }
 }
 
 // ------------------------------------------------------------ data ---------

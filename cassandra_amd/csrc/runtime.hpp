@@ -163,6 +163,9 @@ struct cad_patterns {
     std::map<int64_t, std::unique_ptr<cad::ScatterPlan>> eval_plans;
     cad::DBuf d_base0, d_syscol0;            // zeros
     std::unique_ptr<cad::ExpandedLayout> layout;  // expanded tables (built on first use)
+    // the training step's buffers: per-row gradient of the loss, the weight
+    // gradient, the forward's block sums of the loss (grown on demand)
+    cad::DBuf d_train_g, d_train_grad, d_train_part;
 };
 
 // A resident evaluator: the expanded tables of one weight vector in HBM and

@@ -88,6 +88,46 @@ class PatternEvaluator(torch.nn.Module):
         return pattern_eval(self.weight, idx, self.patterns)
 
 
+def huber_fit_fused(patterns: Patterns, idx: torch.Tensor, target: torch.Tensor, epochs: int = 1,
+                    batch: int = 65536, lr: float = 0.05, delta: float = 4.0, seed: int = 0, shuffle: bool = True,
+                    betas=(0.9, 0.999), eps: float = 1e-8):
+    """huber_fit with every step one call of the library
+    (cad_eval_train_step_f32: the forward with the loss and its gradient, the
+    backward through the minibatch's rows in place, the Adam update); the same
+    minibatches, loss and update as huber_fit.  Returns the trained evaluator
+    and the per-epoch mean loss."""
+    if idx.dtype != torch.int32 or not idx.is_contiguous() or idx.dim() != 2 or idx.shape[1] != patterns.k:
+        raise ValueError("idx must be a contiguous int32 [N, k] tensor of the pattern group's indices")
+    if target.shape[0] != idx.shape[0]:
+        raise ValueError("Size mismatch.")
+    n = idx.shape[0]
+    gen = torch.Generator(device=idx.device).manual_seed(seed)
+    L = capi.lib()
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    losses = []
+    step = 0
+    torch.cuda.current_stream(idx.device).synchronize()  # idx and target are complete
+    # everything on the library's stream, in order with its kernels
+    with torch.cuda.stream(patterns.ctx.torch_stream()):
+        model = PatternEvaluator(patterns)
+        w = model.weight.data
+        mom = torch.zeros_like(w)
+        var = torch.zeros_like(w)
+        tgt = target.to(torch.float32).contiguous()
+        for _ in range(epochs):
+            order = torch.randperm(n, device=idx.device, generator=gen) if shuffle else torch.arange(n, device=idx.device)
+            total = torch.zeros((), dtype=torch.float64, device=idx.device)
+            nb = math.ceil(n / batch)
+            for i in range(nb):
+                sel = order[i * batch:(i + 1) * batch]
+                step += 1
+                capi.check(L.cad_eval_train_step_f32(patterns.h, ptr(idx), ptr(tgt), ptr(sel), sel.shape[0], delta,
+                                                     lr, betas[0], betas[1], eps, step, ptr(w), ptr(mom), ptr(var),
+                                                     ptr(total)))
+            losses.append(float(total) / nb)
+    return model, losses
+
+
 def huber_fit(patterns: Patterns, idx: torch.Tensor, target: torch.Tensor, epochs: int = 1, batch: int = 65536,
               lr: float = 0.05, delta: float = 4.0, seed: int = 0, shuffle: bool = True):
     """Adam on the Huber loss (delta) over minibatches of rows; returns the

@@ -253,6 +253,21 @@ int cad_eval_backward_f64(cad_patterns* p, const int32_t* d_idx, int64_t B, cons
  * share that array and race: serialise them (one stream, or events). */
 int cad_eval_backward_f32_noalloc(cad_patterns* p, const int32_t* d_idx, int64_t B, const float* d_g,
                                   float* d_grad_w);
+/* No counterpart in the reference (its fit is the least-squares PCG of
+ * FitWeights, PatternFit/PatternFit.cpp:7-39); for training the evaluator
+ * itself.  One Adam step of the Huber-loss fit on the minibatch rows
+ * d_sel[0..B) of the row-major index matrix d_idx [N, k] with targets
+ * d_target [N]: pred = the forward above, loss = mean over the batch of
+ * huber(pred - target, delta), then the gradient (the backward above, rows
+ * read through d_sel) and torch.optim.Adam's update (no weight decay) of
+ * d_w with its moments d_m, d_v ([cols] floats, zero before step 1); step
+ * counts from 1.  Adds the batch's mean loss to *d_loss (one device double).
+ * Device buffers for the per-row gradient and the weight gradient are kept on
+ * the pattern group (grown on first use). */
+int cad_eval_train_step_f32(cad_patterns* p, const int32_t* d_idx, const float* d_target,
+                            const int64_t* d_sel, int64_t B, float delta, float lr, float beta1,
+                            float beta2, float eps, int64_t step, float* d_w, float* d_m, float* d_v,
+                            double* d_loss);
 
 /* ---- expanded tables (Pattern/Evaluator.cpp:11-29,56-73) ----------------
  * The evaluator's form of the weights: one table of 3^n floats per variation

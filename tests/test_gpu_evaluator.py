@@ -136,3 +136,66 @@ def test_huber_fit_against_least_squares(ctx, oracle):
     r0 = float(torch.mean(b ** 2))
     assert float(torch.mean((b - p_ls) ** 2)) < 0.5 * r0
     assert float(torch.mean((b - p_h) ** 2)) < 0.7 * r0
+
+
+def test_fused_train_step_closed_form(ctx, oracle, c1):
+    """One fused step from zero weights over all rows (no shuffle): pred = 0,
+    so the loss is mean huber(-t) and the gradient g = A^T clamp(-t, -4, 4) / n
+    (the oracle's FP64 A^T); Adam's first step is w = -lr g / (|g| + eps)
+    (bias corrections cancel)."""
+    from cassandra_amd.evaluator import huber_fit_fused
+
+    say("fused step, closed form")
+    P, pos, d, idx, ih = c1
+    n = idx.shape[0]
+    rng = np.random.default_rng(11)
+    t = (2.0 * rng.integers(-32, 33, size=n)).astype(np.float32)  # even integers in [-64, 64]
+    model, losses = huber_fit_fused(P, idx, torch.from_numpy(t).cuda(), epochs=1, batch=n, lr=0.05,
+                                    shuffle=False)
+    w = model.weight.detach().cpu().numpy().astype(np.float64)
+    dd = -t.astype(np.float64)
+    loss = np.where(np.abs(dd) < 4.0, 0.5 * dd * dd, 4.0 * (np.abs(dd) - 2.0)).mean()
+    assert abs(losses[0] - loss) <= 1e-12 * loss
+    g = oracle.spmtv(ih, np.clip(dd, -4.0, 4.0) / n, P.cols)
+    want = -0.05 * g / (np.abs(g) + 1e-8)
+    # the exact gradients are integer multiples of 2/n (>= 6.7e-5) or 0 (the
+    # oracle's FP64 sum leaves ~1e-19 where the terms cancel)
+    nz = np.abs(g) > 1e-9
+    err = np.max(np.abs(w - want)[nz])
+    say(f"max |w - closed form| where g != 0: {err:.2e}")
+    assert err <= 1e-6
+    # where the exact gradient is 0, the device's float partial sums may leave
+    # a residual up to 2^-22 of the column's sum of |terms|, and Adam's first
+    # step moves the weight by lr |g| / (|g| + eps)
+    bound = oracle.spmtv(ih, np.abs(np.clip(dd, -4.0, 4.0)) / n, P.cols) * 2.0 ** -22
+    lim = 0.05 * bound / (bound + 1e-8)
+    assert np.all(np.abs(w[~nz]) <= lim[~nz] + 1e-12)
+
+
+def test_fused_fit_matches_torch_path(ctx, oracle):
+    """The fused steps against huber_fit's torch path (autograd Function,
+    torch's Huber loss, torch's fused Adam) on the same shuffled minibatches.
+    The two compute the same gradient up to one rounding of 1/B per row and
+    Adam's update in another float order; Adam normalises each column's
+    gradient, so where a column's gradient nearly cancels those roundings move
+    its step by a fraction of lr.  Bounds: the weights' relative L2 distance
+    1e-5, any weight within 1 % of lr (5e-4), the epoch losses within 1e-5."""
+    from cassandra_amd.evaluator import huber_fit_fused
+
+    say("fused fit vs torch path")
+    P = Patterns(ctx, N.FIT4)
+    pos = ctx.random_play(1014, 20000, 30)
+    A = P.matrix(pos)
+    _, b = A.make_targets(2014, 3014)
+    idx = P.extract(pos, capi.LAYOUT_ROW_MAJOR)
+    mt, lt = huber_fit(P, idx, b.float(), epochs=2, batch=4096, lr=0.05, seed=5)
+    mf, lf = huber_fit_fused(P, idx, b.float(), epochs=2, batch=4096, lr=0.05, seed=5)
+    torch.cuda.synchronize()
+    wt, wf = mt.weight.detach().double(), mf.weight.detach().double()
+    err = float(torch.max(torch.abs(wt - wf)))
+    rel = float(torch.linalg.norm(wt - wf) / torch.linalg.norm(wt))
+    say(f"max |w_torch - w_fused| = {err:.2e}, relative L2 {rel:.2e}, losses {lt} {lf}")
+    assert err <= 5e-4
+    assert rel <= 1e-5
+    for a_, b_ in zip(lt, lf):
+        assert abs(a_ - b_) <= 1e-5 * abs(a_)
