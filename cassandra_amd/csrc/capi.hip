@@ -165,25 +165,11 @@ void upload_programs(cad_patterns* p)
     for (const SlotProgram& s : g.slots) {
         DevTern t;
         std::memset(&t, 0, sizeof(t));
-        for (int l = 0; l < 3; l++) {
-            const std::vector<uint8_t>& src = s.list[l].src;
-            // the kernels multiply by 3^j in 24-bit arithmetic: 3^15 < 2^24
-            need(src.size() <= 16, "a pattern list of more than 16 squares", CAD_ERR_PATTERN);
-            t.off[l] = uint32_t(tern.size() / 2);
-            for (int w = 0; w < 4; w++) {
-                size_t n = 0;
-                for (size_t j = 0; j < src.size(); j++)
-                    if (src[j] / 16 == w) {
-                        tern.push_back(uint32_t(2 * (src[j] % 16)));
-                        tern.push_back(pow3(int(j)));
-                        n++;
-                    }
-                t.nent[l] |= uint32_t(n) << (8 * w);
-            }
-        }
+        for (int l = 0; l < 3; l++)
+            append_ternary_list(s.list[l].src.data(), int(s.list[l].src.size()), tern, t, l);
         tslots.push_back(t);
     }
-    tern.insert(tern.end(), 6, 0u);
+    tern.insert(tern.end(), kTernPadding, 0u);
     hipStream_t st = p->ctx->stream;
     p->d_tslots = to_device(tslots, st);
     p->d_tern = to_device(tern, st);

@@ -15,18 +15,6 @@ namespace cad {
 
 namespace {
 
-// The variation's squares in ascending order: FastIndex(pos, mask) =
-// sum_j 3^j (P[sq_j] + 2 O[sq_j]) (Pattern/Helpers.cpp:44-48).
-__device__ __forceinline__ uint32_t fast_index_sq(const EvalSlot& s, uint64_t P, uint64_t O)
-{
-    uint32_t v = 0;
-    for (int j = int(s.n) - 1; j >= 0; --j) {
-        const int q = s.sq[j];
-        v = v * 3u + uint32_t((P >> q) & 1u) + 2u * uint32_t((O >> q) & 1u);
-    }
-    return v;
-}
-
 // w_i[t] = x[column of variation i for the configuration whose FastIndex over
 // the variation's mask is t]: t's base-3 digits (least significant first) are
 // the squares of the mask in ascending order, digit 1 = P, 2 = O.  One thread
@@ -119,21 +107,43 @@ __global__ __launch_bounds__(256) void k_compact_write(const EvalSlot* __restric
 
 // Eval for every position: the sum over the variations of table_i[FastIndex(
 // pos, mask_i)], in variation order from 0.0f (bit-identical to
-// ScoreEstimator::Eval and to (A x)[row]).  The slot descriptors are the same
-// for the whole wave (scalar loads).
+// ScoreEstimator::Eval and to (A x)[row]).  A thread scores kEvalRows
+// positions (r, r + 256, ...) from their ternary words; each variation's
+// FastIndex is its ternary-word program (the variation's squares in
+// ascending order), read once for all of them.  The descriptors are the same
+// for the whole wave.
+constexpr int kEvalRows = 4;
+
 __global__ __launch_bounds__(256) void k_eval_tables(const uint64_t* __restrict__ pos, int64_t n,
-                                                     const EvalSlot* __restrict__ es, int k,
+                                                     const EvalSlot* __restrict__ es,
+                                                     const DevTern* __restrict__ ts,
+                                                     const uint32_t* __restrict__ tern, int k,
                                                      const float* __restrict__ tables, float* __restrict__ out)
 {
-    const int64_t r = blockIdx.x * int64_t(256) + threadIdx.x;
-    if (r >= n) return;
-    const uint64_t P = pos[2 * r], O = pos[2 * r + 1];
-    float sum = 0.0f;
-    for (int s = 0; s < k; s++) {
-        const EvalSlot e = es[s];
-        sum += tables[e.table_offset + fast_index_sq(e, P, O)];
+    const int64_t r0 = blockIdx.x * int64_t(256 * kEvalRows) + threadIdx.x;
+    if (r0 >= n) return;
+    uint32_t T[kEvalRows][4];
+    bool ok[kEvalRows];
+    float sum[kEvalRows];
+#pragma unroll
+    for (int i = 0; i < kEvalRows; i++) {
+        const int64_t r = r0 + int64_t(i) * 256;
+        ok[i] = r < n;
+        const int64_t rr = ok[i] ? r : r0;  // a valid row's words for the lanes past the end
+        ternary_words(pos[2 * rr], pos[2 * rr + 1], T[i]);
+        sum[i] = 0.0f;
     }
-    out[r] = sum;
+    for (int s = 0; s < k; s++) {
+        const EvalSlot e = uniform_load(es + s);
+        const DevTern t = uniform_load(ts + s);
+        uint32_t v[kEvalRows];
+        tern_values<kEvalRows>(tern, t.off[0], t.nent[0], T, v);
+#pragma unroll
+        for (int i = 0; i < kEvalRows; i++) sum[i] += tables[e.table_offset + v[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < kEvalRows; i++)
+        if (ok[i]) out[r0 + int64_t(i) * 256] = sum[i];
 }
 
 int blocks_for(uint64_t n, int cap = 4096)
@@ -171,9 +181,21 @@ const ExpandedLayout& expanded_layout(cad_patterns* p)
         }
     }
     if (slot != g.k()) throw Error(CAD_ERR_PATTERN, "variation count differs from the slot count");
+    // each variation's FastIndex as a ternary-word program (list 0)
+    std::vector<DevTern> ts;
+    std::vector<uint32_t> tern;
+    for (const EvalSlot& e : es) {
+        DevTern t;
+        std::memset(&t, 0, sizeof(t));
+        append_ternary_list(e.sq, int(e.n), tern, t, 0);
+        ts.push_back(t);
+    }
+    tern.insert(tern.end(), kTernPadding, 0u);
     L->slots = es;
     L->size = int64_t(off);
     hipStream_t s = p->ctx->stream;
+    L->d_tslots = to_device(ts, s);
+    L->d_tern = to_device(tern, s);
     L->d_slots = to_device(es, s);
     L->d_first_slot = to_device(L->first_slot, s);
     CAD_HIP(hipStreamSynchronize(s));  // the host vectors are uploaded
@@ -218,8 +240,9 @@ void eval_tables(cad_patterns* p, const float* tables, const uint64_t* pos, int6
 {
     if (n <= 0) return;
     const ExpandedLayout& L = expanded_layout(p);
-    hipLaunchKernelGGL(k_eval_tables, dim3(unsigned((n + 255) / 256)), dim3(256), 0, p->ctx->stream, pos, n,
-                       L.d_slots.as<EvalSlot>(), int(L.slots.size()), tables, out);
+    hipLaunchKernelGGL(k_eval_tables, dim3(unsigned((n + 256 * kEvalRows - 1) / (256 * kEvalRows))), dim3(256), 0,
+                       p->ctx->stream, pos, n, L.d_slots.as<EvalSlot>(), L.d_tslots.as<DevTern>(),
+                       L.d_tern.as<uint32_t>(), int(L.slots.size()), tables, out);
     CAD_HIP(hipGetLastError());
 }
 

@@ -1,6 +1,8 @@
 // Device buffers, contexts, pattern groups, matrices and their launch plans.
 #include "runtime.hpp"
 
+#include "../../include/cassandra_amd.h"
+
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -385,6 +387,24 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
 }
 
 }  // namespace
+
+void append_ternary_list(const uint8_t* src, int n, std::vector<uint32_t>& tern, DevTern& t, int l)
+{
+    // the kernels multiply by 3^j in 24-bit arithmetic: 3^15 < 2^24
+    if (n > 16) throw Error(CAD_ERR_PATTERN, "a pattern list of more than 16 squares");
+    t.off[l] = uint32_t(tern.size() / 2);
+    t.nent[l] = 0;
+    for (int w = 0; w < 4; w++) {
+        uint32_t cnt = 0;
+        for (int j = 0; j < n; j++)
+            if (src[j] / 16 == w) {
+                tern.push_back(uint32_t(2 * (src[j] % 16)));
+                tern.push_back(pow3(j));
+                cnt++;
+            }
+        t.nent[l] |= cnt << (8 * w);
+    }
+}
 
 // Dynamic LDS a window may use: the device limit minus 1 KB kept for the
 // kernels' static LDS (block-reduction scratch).

@@ -47,6 +47,14 @@ DBuf to_device(const std::vector<T>& v, hipStream_t s)
     return b;
 }
 
+// Appends list l of a ternary-word program (DevTern): the squares src[j] of a
+// base-3 number sum_j 3^j (P[src_j] + 2 O[src_j]) as entries {2 (q mod 16),
+// 3^j} grouped by the word q / 16; sets t.off[l] and t.nent[l].  Throws for
+// more than 16 squares (the kernels multiply by 3^j in 24 bits).  A program
+// array must end with kTernPadding zero words.
+void append_ternary_list(const uint8_t* src, int n, std::vector<uint32_t>& tern, DevTern& t, int l);
+constexpr int kTernPadding = 6;
+
 // The two-stage scatter of one value size: tiles (window x row chunk) that
 // write partial slices, and the column reduction over the slices.
 struct ScatterLists {
@@ -138,6 +146,7 @@ struct ExpandedLayout {
     std::vector<int32_t> first_slot;  // first slot (variation 0) of each pattern
     int64_t size = 0;                 // floats of all tables
     DBuf d_slots, d_first_slot;
+    DBuf d_tslots, d_tern;            // FastIndex of every variation as a ternary-word program
 };
 
 struct Context {
