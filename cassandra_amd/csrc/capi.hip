@@ -541,6 +541,7 @@ cad_matrix* matrix_from_positions(cad_patterns* p, const uint64_t* d_pos, const 
     for (size_t s = 0; s < m->slot_lo.size(); s++) widest = std::max(widest, m->slot_hi[s] - m->slot_lo[s]);
     m->ibytes = widest <= 65536 ? 2 : 4;
     m->tune.rpt4 = m->ibytes == 2 ? 8 : 4;
+    if (nsys == 1) default_schedule(m->tune, m->ibytes, n);
     m->ld = padded_rows(n);
     m->d_idx = DBuf(size_t(m->ibytes) * size_t(m->ld) * size_t(g.k()));
     CAD_HIP(hipMemsetAsync(m->d_idx.p, 0, m->d_idx.bytes, p->ctx->stream));
@@ -699,7 +700,7 @@ int cad_matrix_create(cad_context* ctx, const uint32_t* h_idx, int64_t rows, int
         m->slot_lo = lo;
         m->slot_hi = hi;
         m->ibytes = widest <= 65536 ? 2 : 4;
-        m->tune.rpt4 = m->ibytes == 2 ? 8 : 4;
+        default_schedule(m->tune, m->ibytes, rows);
         std::vector<uint32_t> base(k);
         for (int s = 0; s < k; s++) base[s] = m->ibytes == 2 ? lo[s] : 0u;
         m->d_base = to_device(base, ctx->stream);

@@ -638,7 +638,9 @@ void build_plans(cad_matrix* m)
     g.row_count.assign(m->nsys, 0);
     g.col_first.assign(m->nsys, 0);
     g.col_count.assign(m->nsys, 0);
-    const int64_t rchunk = int64_t(kRowBlock) * kRowsPerThread;
+    // row blocks of the row kernels: one row a thread for a small matrix (its
+    // gathers use the row kernel, whose rows' loads are then all in flight)
+    const int64_t rchunk = int64_t(kRowBlock) * (m->rows <= kRowKernelRows ? 1 : kRowsPerThread);
     const int64_t cchunk = int64_t(kRowBlock) * 2;  // enough column blocks to cover the CUs
     for (int s = 0; s < m->nsys; s++) {
         g.row_first[s] = int32_t(g.rows.size());

@@ -131,6 +131,20 @@ struct TuneParams {
     double scatter_balance = 1.0; // order 0: weight of slot loads against atomics in a window's work (1: loads)
 };
 
+// A matrix (or, in a batch, the systems still iterating) of at most this many
+// rows gathers with the row kernel: a staged tile costs about the same (≈22
+// µs) whatever its rows, because it stages every x window of the pattern set,
+// while the row kernel (one row a thread, x read through L2) takes ≈10 µs at
+// 100k rows and ≈15 µs at 250k (FIT4).
+constexpr int64_t kRowKernelRows = 262144;
+
+// Default product schedule of a new stored-index matrix of one system.
+inline void default_schedule(TuneParams& t, int ibytes, int64_t rows)
+{
+    t.rpt4 = ibytes == 2 ? 8 : 4;
+    if (rows <= kRowKernelRows) t.rpt4 = t.rpt8 = 0;
+}
+
 // Slot columns of a stored index matrix start at multiples of this many rows,
 // so a thread can load the entries of up to 8 consecutive rows of one slot in
 // one aligned vector load.

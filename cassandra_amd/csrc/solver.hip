@@ -524,10 +524,6 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s, const Ac
     CAD_HIP(hipGetLastError());
 }
 
-// Rows of the running systems up to which their LSQR iterations gather
-// with the row kernel rather than the staged tiles.
-constexpr int64_t kRowKernelRows = 262144;
-
 // The iteration lists of the systems in `on` (see ActiveLists).
 template <class VT>
 static void build_active(cad_matrix* m, ActiveLists& al, const std::vector<int>& on, hipStream_t s)
