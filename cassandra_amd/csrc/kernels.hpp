@@ -2330,7 +2330,7 @@ __global__ __launch_bounds__(kRowBlock) void k_train_forward(const int32_t* __re
 // The step's Adam update (torch.optim.Adam without weight decay or amsgrad:
 // m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, w -= lr / (1 - b1^t) * m /
 // (sqrt(v) / sqrt(1 - b2^t) + eps)); block 0 also adds the step's mean loss
-// (the forward's block sums in block order, / B) to *loss.
+// (the forward's block sums, reduced in a fixed order, / B) to *loss.
 __global__ __launch_bounds__(kRowBlock) void k_adam(float* __restrict__ w, const float* __restrict__ grad,
                                                     float* __restrict__ m, float* __restrict__ v, int64_t n,
                                                     float lr, float b1, float b2, float eps, float bias1,
@@ -2345,10 +2345,12 @@ __global__ __launch_bounds__(kRowBlock) void k_adam(float* __restrict__ w, const
         v[c] = vc;
         w[c] -= (lr / bias1) * mc / (sqrtf(vc) / bias2_sqrt + eps);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0) {
+        __shared__ double sh[kRowBlock / 64];
         double s = 0.0;
-        for (int i = 0; i < nparts; i++) s += part[i];
-        *loss += s / double(B);
+        for (int i = threadIdx.x; i < nparts; i += kRowBlock) s += part[i];
+        s = block_sum<kRowBlock>(s, sh);
+        if (threadIdx.x == 0) *loss += s / double(B);
     }
 }
 
