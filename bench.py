@@ -424,8 +424,9 @@ def leg_evaluator(ctx):
     torch.cuda.synchronize()
     t_epoch = time.perf_counter() - t0
     log(f"evaluator: epoch {t_epoch:.3f} s, loss {losses}")
-    # the same epoch with every step one library call (forward with the loss
-    # and its gradient, backward through the minibatch rows in place, Adam)
+    # the same epoch as one library call (per step: the forward with the loss
+    # and its gradient reading the minibatch's rows through the order, the
+    # backward, Adam)
     huber_fit_fused(P, idx[: 4 * B], target[: 4 * B], epochs=1, batch=B)  # warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()

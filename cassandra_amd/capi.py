@@ -112,6 +112,8 @@ SIGNATURES = {
     "cad_eval_backward_f32_noalloc": (C.c_int, [vp, vp, i64, vp, vp]),
     "cad_eval_train_step_f32": (C.c_int, [vp, vp, vp, vp, i64, C.c_float, C.c_float, C.c_float, C.c_float,
                                           C.c_float, i64, vp, vp, vp, vp]),
+    "cad_eval_train_epoch_f32": (C.c_int, [vp, vp, vp, vp, i64, i64, C.c_float, C.c_float, C.c_float, C.c_float,
+                                           C.c_float, i64, vp, vp, vp, vp]),
     "cad_patterns_expanded_size": (C.c_int, [vp, C.POINTER(i64)]),
     "cad_weights_expand": (C.c_int, [vp, vp, vp]),
     "cad_weights_compact": (C.c_int, [vp, vp, vp]),

@@ -277,8 +277,9 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
 }
 
 // One Adam step of the Huber-loss fit on the minibatch rows sel[0..B): the
-// forward with the loss and its gradient, the backward (the evaluator's
-// scatter plan, rows read through sel), the Adam update.
+// forward with the loss and its gradient (which also copies the minibatch's
+// rows contiguously), the backward over that copy (the evaluator's scatter
+// plan), the Adam update.
 void train_step(cad_patterns* p, const int32_t* idx, const float* target, const int64_t* sel, int64_t B,
                 float delta, float lr, float b1, float b2, float eps, int64_t step, float* w, float* m, float* v,
                 double* loss)
@@ -293,13 +294,16 @@ void train_step(cad_patterns* p, const int32_t* idx, const float* target, const 
     grow(p->d_train_g, sizeof(float) * size_t(std::max<int64_t>(1, B)));
     grow(p->d_train_grad, sizeof(float) * size_t(C));
     grow(p->d_train_part, sizeof(double) * size_t(nblk));
+    grow(p->d_train_rows, sizeof(int32_t) * size_t(std::max<int64_t>(1, B)) * size_t(k));
     float* g = p->d_train_g.as<float>();
     float* grad = p->d_train_grad.as<float>();
     double* part = p->d_train_part.as<double>();
+    int32_t* rows = p->d_train_rows.as<int32_t>();
     if (B > 0)
         hipLaunchKernelGGL(k_train_forward, dim3(nblk), dim3(kRowBlock), 0, s, idx, sel, B, k, w, uint32_t(C), target,
-                           delta, g, part);
-    eval_backward_view<float>(p, RowMajor32Sel{idx, sel, k}, B, g, grad);
+                           delta, g, part, rows);
+    // the backward over the contiguous copy of the minibatch's rows
+    eval_backward_view<float>(p, RowMajor32{rows, k}, B, g, grad);
     const float bias1 = 1.0f - std::pow(b1, float(step));
     const float bias2_sqrt = std::sqrt(1.0f - std::pow(b2, float(step)));
     hipLaunchKernelGGL(k_adam, dim3(grid1(C, kRowBlock)), dim3(kRowBlock), 0, s, w, (const float*)grad, m, v, C, lr,
@@ -946,6 +950,25 @@ int cad_eval_train_step_f32(cad_patterns* p, const int32_t* idx, const float* ta
                  eps >= 0.0f,
              "invalid Huber or Adam parameter");
         train_step(p, idx, target, sel, B, delta, lr, beta1, beta2, eps, step, w, m, v, loss);
+    });
+}
+
+int cad_eval_train_epoch_f32(cad_patterns* p, const int32_t* idx, const float* target, const int64_t* order,
+                             int64_t n, int64_t batch, float delta, float lr, float beta1, float beta2, float eps,
+                             int64_t step0, float* w, float* m, float* v, double* loss)
+{
+    return guarded([&] {
+        need(p && w && m && v && loss && n >= 0 && batch > 0 && (n == 0 || (idx && target && order)),
+             "invalid argument");
+        need(p->ctx != nullptr, "host-only pattern group (created without a context)");
+        need(step0 >= 1, "Adam steps count from 1");
+        need(delta > 0.0f && lr > 0.0f && beta1 >= 0.0f && beta1 < 1.0f && beta2 >= 0.0f && beta2 < 1.0f &&
+                 eps >= 0.0f,
+             "invalid Huber or Adam parameter");
+        int64_t step = step0;
+        for (int64_t b0 = 0; b0 < n; b0 += batch, step++)
+            train_step(p, idx, target, order + b0, std::min(batch, n - b0), delta, lr, beta1, beta2, eps, step, w, m,
+                       v, loss);
     });
 }
 

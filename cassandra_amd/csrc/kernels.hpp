@@ -71,18 +71,6 @@ struct RowMajor32 {
     }
 };
 
-// The rows sel[0..B) of a row-major int32 index matrix [N, k] (a shuffled
-// minibatch, read in place).
-struct RowMajor32Sel {
-    const int32_t* __restrict__ idx;
-    const int64_t* __restrict__ sel;
-    int k;
-    __device__ __forceinline__ uint32_t col(int s, int64_t r) const
-    {
-        return uint32_t(idx[sel[r] * k + s]);
-    }
-};
-
 // ------------------------------------------------------------ reductions ---
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -2295,33 +2283,57 @@ __global__ __launch_bounds__(kRowBlock) void k_eval_forward(const int32_t* __res
     out[r] = acc;
 }
 
+// The Huber loss of d = pred - target (0.5 d^2 inside [-delta, delta],
+// delta (|d| - delta/2) outside) and the gradient of the batch mean,
+// clamp(d, -delta, delta) / B.
+__device__ __forceinline__ double huber_term(float d, float delta, int64_t B, float& grad)
+{
+    const float ad = fabsf(d);
+    grad = fminf(fmaxf(d, -delta), delta) / float(B);
+    return ad < delta ? 0.5 * double(d) * double(d) : double(delta) * (double(ad) - 0.5 * double(delta));
+}
+
 // One Huber-loss training step, forward half: for the minibatch rows sel[b]
-// the prediction (the evaluator's gather-sum, slot order from zero), the
-// Huber loss of pred - target (0.5 d^2 inside [-delta, delta], delta (|d| -
-// delta/2) outside) and its gradient for the mean over the batch,
-// g[b] = clamp(d, -delta, delta) / B; the block sums of the losses (FP64) go
-// to part[blockIdx.x].
+// the prediction (the evaluator's gather-sum, slot order from zero), the loss
+// and g[b] (huber_term); the block sums of the losses (FP64) go to
+// part[blockIdx.x], and each row is copied to rows[b * k ..] (contiguous, for
+// the backward).  A thread takes one row, two entries per 8-byte load where k
+// is even.
 __global__ __launch_bounds__(kRowBlock) void k_train_forward(const int32_t* __restrict__ idx,
                                                              const int64_t* __restrict__ sel, int64_t B, int k,
                                                              const float* __restrict__ w, uint32_t cols,
                                                              const float* __restrict__ target, float delta,
-                                                             float* __restrict__ g, double* __restrict__ part)
+                                                             float* __restrict__ g, double* __restrict__ part,
+                                                             int32_t* __restrict__ rows)
 {
     __shared__ double sh[kRowBlock / 64];
     const int64_t b = blockIdx.x * int64_t(kRowBlock) + threadIdx.x;
     double loss = 0.0;
     if (b < B) {
         const int64_t r = sel[b];
-        const int32_t* __restrict__ row = idx + r * k;
         float acc = 0.0f;
-        for (int s = 0; s < k; s++) {
-            const uint32_t c = uint32_t(row[s]);
-            if (c < cols) acc += w[c];
+        if ((k & 1) == 0) {
+            const int2* __restrict__ src = reinterpret_cast<const int2*>(idx + r * k);
+            int2* __restrict__ dst = reinterpret_cast<int2*>(rows + b * k);
+#pragma unroll 3
+            for (int j = 0; j < k / 2; j++) {
+                const int2 c = src[j];
+                dst[j] = c;
+                if (uint32_t(c.x) < cols) acc += w[c.x];
+                if (uint32_t(c.y) < cols) acc += w[c.y];
+            }
+        } else {
+            const int32_t* __restrict__ src = idx + r * k;
+            int32_t* __restrict__ dst = rows + b * k;
+            for (int j = 0; j < k; j++) {
+                const int32_t c = src[j];
+                dst[j] = c;
+                if (uint32_t(c) < cols) acc += w[c];
+            }
         }
-        const float d = acc - target[r];
-        const float ad = fabsf(d);
-        loss = ad < delta ? 0.5 * double(d) * double(d) : double(delta) * (double(ad) - 0.5 * double(delta));
-        g[b] = fminf(fmaxf(d, -delta), delta) / float(B);
+        float gr;
+        loss = huber_term(acc - target[r], delta, B, gr);
+        g[b] = gr;
     }
     loss = block_sum<kRowBlock>(loss, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = loss;

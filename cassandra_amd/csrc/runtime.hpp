@@ -187,8 +187,9 @@ struct cad_patterns {
     cad::DBuf d_base0, d_syscol0;            // zeros
     std::unique_ptr<cad::ExpandedLayout> layout;  // expanded tables (built on first use)
     // the training step's buffers: per-row gradient of the loss, the weight
-    // gradient, the forward's block sums of the loss (grown on demand)
-    cad::DBuf d_train_g, d_train_grad, d_train_part;
+    // gradient, the forward's block sums of the loss, the minibatch's index
+    // rows made contiguous (grown on demand)
+    cad::DBuf d_train_g, d_train_grad, d_train_part, d_train_rows;
 };
 
 // A resident evaluator: the expanded tables of one weight vector in HBM and

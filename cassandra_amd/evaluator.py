@@ -91,11 +91,11 @@ class PatternEvaluator(torch.nn.Module):
 def huber_fit_fused(patterns: Patterns, idx: torch.Tensor, target: torch.Tensor, epochs: int = 1,
                     batch: int = 65536, lr: float = 0.05, delta: float = 4.0, seed: int = 0, shuffle: bool = True,
                     betas=(0.9, 0.999), eps: float = 1e-8):
-    """huber_fit with every step one call of the library
-    (cad_eval_train_step_f32: the forward with the loss and its gradient, the
-    backward through the minibatch's rows in place, the Adam update); the same
-    minibatches, loss and update as huber_fit.  Returns the trained evaluator
-    and the per-epoch mean loss."""
+    """huber_fit with every epoch one call of the library
+    (cad_eval_train_epoch_f32; each step: the forward with the loss and its
+    gradient, reading the minibatch's rows through the shuffled order, the
+    backward, the Adam update); the same minibatches, loss and update as
+    huber_fit.  Returns the trained evaluator and the per-epoch mean loss."""
     if idx.dtype != torch.int32 or not idx.is_contiguous() or idx.dim() != 2 or idx.shape[1] != patterns.k:
         raise ValueError("idx must be a contiguous int32 [N, k] tensor of the pattern group's indices")
     if target.shape[0] != idx.shape[0]:
@@ -118,12 +118,11 @@ def huber_fit_fused(patterns: Patterns, idx: torch.Tensor, target: torch.Tensor,
             order = torch.randperm(n, device=idx.device, generator=gen) if shuffle else torch.arange(n, device=idx.device)
             total = torch.zeros((), dtype=torch.float64, device=idx.device)
             nb = math.ceil(n / batch)
-            for i in range(nb):
-                sel = order[i * batch:(i + 1) * batch]
-                step += 1
-                capi.check(L.cad_eval_train_step_f32(patterns.h, ptr(idx), ptr(tgt), ptr(sel), sel.shape[0], delta,
-                                                     lr, betas[0], betas[1], eps, step, ptr(w), ptr(mom), ptr(var),
-                                                     ptr(total)))
+            # the epoch's steps, enqueued by one call (cad_eval_train_epoch_f32)
+            capi.check(L.cad_eval_train_epoch_f32(patterns.h, ptr(idx), ptr(tgt), ptr(order), n, batch, delta, lr,
+                                                  betas[0], betas[1], eps, step + 1, ptr(w), ptr(mom), ptr(var),
+                                                  ptr(total)))
+            step += nb
             losses.append(float(total) / nb)
     return model, losses
 

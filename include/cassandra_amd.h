@@ -262,12 +262,21 @@ int cad_eval_backward_f32_noalloc(cad_patterns* p, const int32_t* d_idx, int64_t
  * read through d_sel) and torch.optim.Adam's update (no weight decay) of
  * d_w with its moments d_m, d_v ([cols] floats, zero before step 1); step
  * counts from 1.  Adds the batch's mean loss to *d_loss (one device double).
- * Device buffers for the per-row gradient and the weight gradient are kept on
- * the pattern group (grown on first use). */
+ * Device buffers for the per-row gradient, the weight gradient and a
+ * contiguous copy of the minibatch's index rows are kept on the pattern group
+ * (grown on first use). */
 int cad_eval_train_step_f32(cad_patterns* p, const int32_t* d_idx, const float* d_target,
                             const int64_t* d_sel, int64_t B, float delta, float lr, float beta1,
                             float beta2, float eps, int64_t step, float* d_w, float* d_m, float* d_v,
                             double* d_loss);
+/* No counterpart in the reference.  An epoch of cad_eval_train_step_f32: the
+ * minibatches d_order[b0 .. b0 + batch) for b0 = 0, batch, ... < n (the last
+ * one shorter), Adam steps step0, step0 + 1, ...; every step's mean loss is
+ * added to *d_loss.  All steps are enqueued by this one call. */
+int cad_eval_train_epoch_f32(cad_patterns* p, const int32_t* d_idx, const float* d_target,
+                             const int64_t* d_order, int64_t n, int64_t batch, float delta, float lr,
+                             float beta1, float beta2, float eps, int64_t step0, float* d_w, float* d_m,
+                             float* d_v, double* d_loss);
 
 /* ---- expanded tables (Pattern/Evaluator.cpp:11-29,56-73) ----------------
  * The evaluator's form of the weights: one table of 3^n floats per variation

@@ -199,3 +199,68 @@ def test_fused_fit_matches_torch_path(ctx, oracle):
     assert rel <= 1e-5
     for a_, b_ in zip(lt, lf):
         assert abs(a_ - b_) <= 1e-5 * abs(a_)
+
+
+def test_train_step_calls_equal_epoch_call(ctx, oracle, c1):
+    """cad_eval_train_step_f32 called once per minibatch gives bit for bit the
+    weights, moments and loss of one cad_eval_train_epoch_f32 call over the
+    same order (the same kernels in the same order)."""
+    import ctypes as C
+
+    say("train step calls vs epoch call")
+    P, pos, d, idx, ih = c1
+    n, B = idx.shape[0], 4096
+    rng = np.random.default_rng(12)
+    tgt = torch.from_numpy((2.0 * rng.integers(-32, 33, size=n)).astype(np.float32)).cuda()
+    order = torch.from_numpy(rng.permutation(n).astype(np.int64)).cuda()
+    L = capi.lib()
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    outs = []
+    for per_step in (True, False):
+        w = torch.zeros(P.cols, dtype=torch.float32, device="cuda")
+        m = torch.zeros_like(w)
+        v = torch.zeros_like(w)
+        loss = torch.zeros((), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        if per_step:
+            for i, b0 in enumerate(range(0, n, B)):
+                sel = order[b0:b0 + B]
+                capi.check(L.cad_eval_train_step_f32(P.h, ptr(idx), ptr(tgt), ptr(sel), sel.shape[0], 4.0, 0.05, 0.9,
+                                                     0.999, 1e-8, i + 1, ptr(w), ptr(m), ptr(v), ptr(loss)))
+        else:
+            capi.check(L.cad_eval_train_epoch_f32(P.h, ptr(idx), ptr(tgt), ptr(order), n, B, 4.0, 0.05, 0.9, 0.999,
+                                                  1e-8, 1, ptr(w), ptr(m), ptr(v), ptr(loss)))
+        ctx.synchronize()
+        outs.append((w.clone(), m.clone(), v.clone(), float(loss)))
+    (w0, m0, v0, l0), (w1, m1, v1, l1) = outs
+    assert torch.equal(w0, w1) and torch.equal(m0, m1) and torch.equal(v0, v1)
+    assert l0 == l1 and l0 > 0
+
+
+def test_fused_train_step_wide_rows_closed_form(ctx, oracle):
+    """The one-step closed form of the fused step on a second pattern group
+    with wider rows (FIT12 with L02X and B5 once more: k = 66)."""
+    from cassandra_amd.evaluator import huber_fit_fused
+
+    say("fused step, 66 slots a row")
+    masks = list(N.FIT12) + [N.L02X, N.B5]
+    P = Patterns(ctx, masks)
+    assert P.k == 66
+    pos = oracle.random_play(1024, 9001, 30)
+    d = torch.from_numpy(pos.view(np.int64)).cuda()
+    idx = P.extract(d, capi.LAYOUT_ROW_MAJOR)
+    ih = oracle.Group(masks).indices(pos)
+    n = idx.shape[0]
+    rng = np.random.default_rng(13)
+    t = (2.0 * rng.integers(-32, 33, size=n)).astype(np.float32)
+    model, losses = huber_fit_fused(P, idx, torch.from_numpy(t).cuda(), epochs=1, batch=n, lr=0.05,
+                                    shuffle=False)
+    w = model.weight.detach().cpu().numpy().astype(np.float64)
+    dd = -t.astype(np.float64)
+    loss = np.where(np.abs(dd) < 4.0, 0.5 * dd * dd, 4.0 * (np.abs(dd) - 2.0)).mean()
+    assert abs(losses[0] - loss) <= 1e-12 * loss
+    g = oracle.spmtv(ih, np.clip(dd, -4.0, 4.0) / n, P.cols)
+    nz = np.abs(g) > 1e-9
+    err = np.max(np.abs(w - (-0.05 * g / (np.abs(g) + 1e-8)))[nz])
+    say(f"max |w - closed form| where g != 0: {err:.2e}")
+    assert err <= 1e-6

@@ -6,10 +6,12 @@ PCG on the same indices and targets.
 
 The normal equations are singular, and PCG's iterates after 100 steps are
 sensitive to rounding: the test measures that sensitivity on the oracle itself
-(the same PCG on targets moved by one ulp each, with random signs) and
-requires the GPU's predictions to lie within 10 times that distance of the
-oracle's.  At 10 iterations, before the rounding has grown, the predictions
-agree to 1e-9."""
+(the same PCG on targets moved by one ulp each, with random signs; the
+largest distance over four such perturbations, since one sample varies with
+the order of the OpenMP reductions by a factor of about 5) and requires the
+GPU's predictions to lie within 10 times that distance of the oracle's.  At
+10 iterations, before the rounding has grown, the predictions agree to
+1e-9."""
 import sys
 
 import numpy as np
@@ -65,12 +67,14 @@ def test_pcg_100_iterations_within_oracle_sensitivity(ctx, oracle, fit1):
     say("reference fit: 100 PCG iterations")
     bh = b.cpu().numpy()
     xo, _ = oracle.pcg(idx, A.cols, bh, 100)
-    sign = np.random.default_rng(7).integers(0, 2, size=bh.shape[0])
-    bp = np.where(sign == 1, np.nextafter(bh, np.inf), np.nextafter(bh, -np.inf))
-    bp = np.where(bh == 0, bh, bp)  # zero targets stay zero (one ulp of 0 is subnormal)
-    xp, _ = oracle.pcg(idx, A.cols, bp, 100)
-    sens = rel_dist(oracle, idx, xp, xo)
+    sens = 0.0
+    for seed in (7, 8, 9, 10):
+        sign = np.random.default_rng(seed).integers(0, 2, size=bh.shape[0])
+        bp = np.where(sign == 1, np.nextafter(bh, np.inf), np.nextafter(bh, -np.inf))
+        bp = np.where(bh == 0, bh, bp)  # zero targets stay zero (one ulp of 0 is subnormal)
+        xp, _ = oracle.pcg(idx, A.cols, bp, 100)
+        sens = max(sens, rel_dist(oracle, idx, xp, xo))
     dist = rel_dist(oracle, idx, pcg_gpu(A, b, 100), xo)
-    say(f"reference fit: oracle one-ulp sensitivity {sens:.3e}, GPU distance {dist:.3e}")
+    say(f"reference fit: oracle one-ulp sensitivity (largest of 4) {sens:.3e}, GPU distance {dist:.3e}")
     assert 0 < sens < 1e-2
     assert dist <= 10 * sens
