@@ -262,7 +262,7 @@ void eval_backward_view(cad_patterns* p, const View& view, int64_t B, const VT* 
         hipLaunchKernelGGL(kern, dim3(unsigned(L.work.size())), dim3(kScatterBlock), lds, s,
                            view, L.d_work.as<ScatterWork>(), L.d_groups.as<SlotGroup>(),
                            L.d_slot_list.as<uint16_t>(), p->d_syscol0.as<uint32_t>(), g,
-                           (const LsqrScalars*)nullptr, kScatterAll, FixedScale{nullptr, 0}, 0, part);
+                           (LsqrScalars*)nullptr, kScatterAll, FixedScale{nullptr, 0}, 0, part, GridSys{});
         CAD_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL((k_scatter_reduce<VT, VT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,

@@ -46,7 +46,7 @@ struct RowWork {
 struct ColWork {
     int64_t col_begin, col_end;  // global columns
     int32_t sys;
-    int32_t pad;
+    int32_t first;               // 1 on the first column block of its system
 };
 
 // A scatter tile of A^T y: rows [row_begin,row_end) x the slots of `group`,
@@ -156,6 +156,10 @@ struct LsqrScalars {
     int32_t iters;
     int32_t stop;             // 0 running; 1 target met this iteration (its step still applies); 2 stopped
     int32_t restart;          // selects the systems that the start / apply kernels (re)initialise
+    double beta2;             // |u_raw|^2 of the iteration's gather (beta = sqrt(beta2))
+    double rhobar0, phibar0;  // rhobar, phibar and stop as the iteration's rotation found them
+    int32_t stop0;
+    int32_t pad;
     double pad2;
 };
 static_assert(sizeof(LsqrScalars) % 16 == 0, "LsqrScalars layout");

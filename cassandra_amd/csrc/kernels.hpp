@@ -112,6 +112,21 @@ __device__ __forceinline__ bool arrive_last(unsigned* counter, unsigned nblocks)
     return last;
 }
 
+// Sum of one system's block partials (slot 0), returned to every thread:
+// each thread adds its strided share in block order, then the block's fixed
+// reduction tree, so every block that forms the sum gets the same value.
+template <int BS>
+__device__ __forceinline__ double system_sum(const GridSys& g, int sys, double* sh)
+{
+    double s = 0.0;
+    const int first = g.blk_first[sys], n = g.blk_count[sys];
+    for (int i = threadIdx.x; i < n; i += BS) s += g.partials[2 * (first + i)];
+    s = block_sum<BS>(s, sh);
+    if (threadIdx.x == 0) sh[0] = s;
+    __syncthreads();
+    return sh[0];
+}
+
 // ------------------------------------------------------------ fill --------
 template <class T>
 __global__ void k_fill(T* __restrict__ p, int64_t n, T v)
@@ -1268,6 +1283,25 @@ __device__ __forceinline__ VT scatter_scale(const LsqrScalars* div, int mode, in
     return VT(d != 0.0 ? 1.0 / d : 0.0);
 }
 
+// The same, where an LSQR iteration hands over its gather's |u_raw|^2 block
+// partials (bsum.partials non-null): beta = sqrt of their sum is formed here,
+// by every block from the same partials in the same order (so all blocks
+// agree), and stored with its square for the kernels after; the scatter then
+// reads no beta from the state.
+template <class VT, int BS>
+__device__ __forceinline__ VT scatter_scale(LsqrScalars* div, int mode, int sys, const GridSys& bsum)
+{
+    if (mode != kScatterLsqr || !bsum.partials) return scatter_scale<VT>(div, mode, sys);
+    __shared__ double sh[BS / 64];
+    const double s = system_sum<BS>(bsum, sys, sh);
+    const double d = sqrt(s);
+    if (threadIdx.x == 0) {
+        div[sys].beta = d;
+        div[sys].beta2 = s;
+    }
+    return VT(d != 0.0 ? 1.0 / d : 0.0);
+}
+
 // Integer accumulation of an FP32 scatter window.  Each term y * scale is
 // rounded to a multiple of 2^-s and the window adds the int64 multiples with
 // LDS integer atomics (ds_add_u64), which run about 1.8 times as fast as the
@@ -1362,9 +1396,9 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
                                                                 const SlotGroup* __restrict__ groups,
                                                                 const uint16_t* __restrict__ slot_list,
                                                                 const uint32_t* __restrict__ sys_col,
-                                                                const YT* __restrict__ y, const LsqrScalars* div,
+                                                                const YT* __restrict__ y, LsqrScalars* div,
                                                                 int mode, FixedScale fx, int y_padded,
-                                                                VT* __restrict__ part)
+                                                                VT* __restrict__ part, GridSys bsum)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* win = reinterpret_cast<double*>(smem_raw);
@@ -1375,7 +1409,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
     __syncthreads();
     const SlotGroup gr = groups[w.group];
     const uint32_t cb = sys_col[w.sys];
-    const VT sc = scatter_scale<VT>(div, mode, w.sys);
+    const VT sc = scatter_scale<VT, kScatterBlock>(div, mode, w.sys, bsum);
     for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kScatterBlock) {
         const double yv = double(VT(y[r]) * sc);
         for (int j = 0; j < gr.nslots; j++) {
@@ -1483,9 +1517,9 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
                                                                 const SlotGroup* __restrict__ groups,
                                                                 const uint16_t* __restrict__ slot_list,
                                                                 const uint32_t* __restrict__ sys_col,
-                                                                const YT* __restrict__ y, const LsqrScalars* div,
+                                                                const YT* __restrict__ y, LsqrScalars* div,
                                                                 int mode, FixedScale fx, int y_padded,
-                                                                VT* __restrict__ part)
+                                                                VT* __restrict__ part, GridSys bsum)
 {
     using V = IdxVec<IdxT, RPT>;
     using VecT = typename V::T;
@@ -1499,7 +1533,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     const SlotGroup gr = groups[w.group];
     const int ns = gr.nslots;
     const uint32_t cb = sys_col[w.sys];
-    const VT sc = scatter_scale<VT>(div, mode, w.sys);
+    const VT sc = scatter_scale<VT, kScatterBlock>(div, mode, w.sys, bsum);
     const int64_t vld = m.ld / RPT;
     // FP32 windows accumulate in fixed point where the scale allows it
     const double fscale = std::is_same_v<VT, float> && ns <= kMaxSegSlots ? fixed_scale(fx, w.sys, double(sc)) : 0.0;
@@ -1592,9 +1626,9 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView
                                                                    const SlotGroup* __restrict__ groups,
                                                                    const uint16_t* __restrict__ slot_list,
                                                                    const uint32_t* __restrict__ sys_col,
-                                                                   const YT* __restrict__ y, const LsqrScalars* div,
+                                                                   const YT* __restrict__ y, LsqrScalars* div,
                                                                    int mode, FixedScale fx, int y_padded,
-                                                                VT* __restrict__ part)
+                                                                   VT* __restrict__ part, GridSys bsum)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* win = reinterpret_cast<double*>(smem_raw);
@@ -1605,7 +1639,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView
     __syncthreads();
     const SlotGroup gr = uniform_load(groups + w.group);
     const uint32_t cb = sys_col[w.sys];
-    const VT sc = scatter_scale<VT>(div, mode, w.sys);
+    const VT sc = scatter_scale<VT, kScatterBlock>(div, mode, w.sys, bsum);
     const int64_t last = w.row_end - 1;
     for (int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x; a < w.row_end;
          a += int64_t(RPT) * kScatterBlock) {
@@ -1768,19 +1802,26 @@ __global__ void k_colscale(const uint32_t* __restrict__ cnt, int64_t n, double* 
 // A^T (u / beta): per column z = the slices' sum rounded to VT (as
 // k_scatter_reduce writes it), then vbar = D z - beta v and the block
 // partials of |vbar|^2, one block per reduce item (g: the reduce items of
-// each system); k_lsqr_rot finishes alpha and the rotation.  (A grid-wide
-// last-block ticket here would cost every block an L2 write-back and
-// invalidate on gfx950.)
+// each system); k_lsqr_step (LSQR) or k_lsmr_rot (LSMR) finishes alpha.  The
+// system's first block also keeps rhobar, phibar and stop aside for
+// k_lsqr_step, whose blocks read them while its first block updates them.
+// (A grid-wide last-block ticket here would cost every block an L2
+// write-back and invalidate on gfx950.)
 template <class VT>
 __global__ __launch_bounds__(kRowBlock) void k_lsqr_reduce_v(const ReduceWork* __restrict__ work,
                                                              const VT* __restrict__ part,
                                                              const double* __restrict__ D, double* __restrict__ v,
-                                                             GridSys g, const LsqrScalars* scal)
+                                                             GridSys g, LsqrScalars* scal)
 {
     __shared__ double sh[kRowBlock / 64];
     const ReduceWork w = work[blockIdx.x];
-    const LsqrScalars* st = &scal[w.sys];
+    LsqrScalars* st = &scal[w.sys];
     if (!st->active) return;
+    if (threadIdx.x == 0 && int(blockIdx.x) == g.blk_first[w.sys]) {
+        st->rhobar0 = st->rhobar;
+        st->phibar0 = st->phibar;
+        st->stop0 = st->stop;
+    }
     const double beta = st->beta;
     double acc = 0.0;
     if (st->stop == 0)
@@ -1794,67 +1835,68 @@ __global__ __launch_bounds__(kRowBlock) void k_lsqr_reduce_v(const ReduceWork* _
     if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
 }
 
-// The rotation (Math/Solver.h:165-171), one block per system: alpha = |vbar|
-// from k_lsqr_reduce_v's partials, then rho, c, s, theta, phi, phibar and the
-// device-side stop.
-__global__ __launch_bounds__(kRowBlock) void k_lsqr_rot(GridSys g, LsqrScalars* scal)
+// The LSQR rotation and update in one launch (Math/Solver.h:165-175).  Every
+// block of a system forms alpha = |vbar| from k_lsqr_reduce_v's partials and
+// the rotation (rho, c, s, theta, phi, phibar) from the rhobar, phibar and
+// stop that k_lsqr_reduce_v kept aside -- the same values in every block --
+// and updates its columns: v = vbar / alpha, vd = D v, y += (phi/rho) w,
+// w = v - (theta/rho) w.  The system's first column block stores the new
+// state, with the device-side stop; the fields it writes are read by no
+// block of this launch.
+template <class VT>
+__global__ __launch_bounds__(kRowBlock) void k_lsqr_step(const ColWork* __restrict__ work, GridSys g,
+                                                         const double* __restrict__ D, double* __restrict__ v,
+                                                         VT* __restrict__ vd, double* __restrict__ w_,
+                                                         double* __restrict__ y, LsqrScalars* scal)
 {
     __shared__ double sh[kRowBlock / 64];
-    const int sys = blockIdx.x;
-    LsqrScalars* st = &scal[sys];
+    const ColWork w = work[blockIdx.x];
+    LsqrScalars* st = &scal[w.sys];
     if (!st->active) return;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < g.blk_count[sys]; i += kRowBlock) s += g.partials[2 * (g.blk_first[sys] + i)];
-    s = block_sum<kRowBlock>(s, sh);
-    if (threadIdx.x != 0) return;
-    if (st->stop != 0) {
+    const double s = system_sum<kRowBlock>(g, w.sys, sh);
+    const bool writer = w.first != 0 && threadIdx.x == 0;
+    if (st->stop0 != 0) {
         // the iteration after the stop: no further step
-        st->stop = 2;
-        st->step_x = 0.0;
-        st->step_w = 0.0;
+        if (writer) {
+            st->stop = 2;
+            st->step_x = 0.0;
+            st->step_w = 0.0;
+        }
         return;
     }
-    const double beta = st->beta;
+    const double beta = st->beta, rhobar = st->rhobar0, phibar = st->phibar0;
     const double alpha = sqrt(s);
-    const double rho = sqrt(st->rhobar * st->rhobar + beta * beta);
-    const double c = rho != 0.0 ? st->rhobar / rho : 0.0;
+    const double rho = sqrt(rhobar * rhobar + beta * beta);
+    const double c = rho != 0.0 ? rhobar / rho : 0.0;
     const double sn = rho != 0.0 ? beta / rho : 0.0;
     const double theta = sn * alpha;
-    const double phi = c * st->phibar;
-    st->rhobar = -c * alpha;
-    st->phibar = sn * st->phibar;
-    st->step_x = rho != 0.0 ? phi / rho : 0.0;
-    st->step_w = rho != 0.0 ? theta / rho : 0.0;
-    st->alpha = alpha;
-    st->nres = st->phibar * alpha * fabs(c);
-    st->rnorm = st->phibar;
-    st->anorm2 += s;
-    st->iters += 1;
-    // met the target: this iteration's step is the last (a rank-deficient
-    // system that continued would iterate on rounding noise)
-    if (st->target > 0.0 && st->nres <= st->target * st->nres0) st->stop = 1;
-}
-
-// LSQR update (Math/Solver.h:173-175): v = vbar/alpha, vd = D v,
-// y += (phi/rho) w, w = v - (theta/rho) w.
-template <class VT>
-__global__ __launch_bounds__(kRowBlock) void k_lsqr_xw(const ColWork* __restrict__ work,
-                                                       const double* __restrict__ D, double* __restrict__ v,
-                                                       VT* __restrict__ vd, double* __restrict__ w_,
-                                                       double* __restrict__ y, const LsqrScalars* scal)
-{
-    const ColWork w = work[blockIdx.x];
-    const LsqrScalars* st = &scal[w.sys];
-    if (!st->active || st->stop > 1) return;
-    const double ia = st->alpha != 0.0 ? 1.0 / st->alpha : 0.0;
-    const double sx = st->step_x, sw = st->step_w;
-    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
-        const double vn = v[c] * ia;
-        v[c] = vn;
-        vd[c] = VT(D[c] * vn);
-        const double wc = w_[c];
-        y[c] += sx * wc;
-        w_[c] = vn - sw * wc;
+    const double phi = c * phibar;
+    const double sx = rho != 0.0 ? phi / rho : 0.0;
+    const double sw = rho != 0.0 ? theta / rho : 0.0;
+    if (writer) {
+        const double phibar_new = sn * phibar;
+        st->rhobar = -c * alpha;
+        st->phibar = phibar_new;
+        st->step_x = sx;
+        st->step_w = sw;
+        st->alpha = alpha;
+        st->nres = phibar_new * alpha * fabs(c);
+        st->rnorm = phibar_new;
+        st->anorm2 += st->beta2;
+        st->anorm2 += s;
+        st->iters += 1;
+        // met the target: this iteration's step is the last (a rank-deficient
+        // system that continued would iterate on rounding noise)
+        if (st->target > 0.0 && st->nres <= st->target * st->nres0) st->stop = 1;
+    }
+    const double ia = alpha != 0.0 ? 1.0 / alpha : 0.0;
+    for (int64_t c0 = w.col_begin + threadIdx.x; c0 < w.col_end; c0 += kRowBlock) {
+        const double vn = v[c0] * ia;
+        v[c0] = vn;
+        vd[c0] = VT(D[c0] * vn);
+        const double wc = w_[c0];
+        y[c0] += sx * wc;
+        w_[c0] = vn - sw * wc;
     }
 }
 
@@ -1987,7 +2029,7 @@ __device__ __forceinline__ void sym_ortho(double a, double b, double& c, double&
 // LSMR's rotations (Math/Solver.h:255-275), one block per system: alpha =
 // |vbar| from k_lsqr_reduce_v's partials, then the two rotations, zeta and
 // the update coefficients; nres = |zetabar| (= |A^T r_k|) and the device-side
-// stop as in k_lsqr_rot.
+// stop as in k_lsqr_step.
 __global__ __launch_bounds__(kRowBlock) void k_lsmr_rot(GridSys g, LsqrScalars* scal, LsmrState* lsmr)
 {
     __shared__ double sh[kRowBlock / 64];

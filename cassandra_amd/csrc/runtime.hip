@@ -649,7 +649,7 @@ void build_plans(cad_matrix* m)
         g.row_count[s] = int32_t(g.rows.size()) - g.row_first[s];
         g.col_first[s] = int32_t(g.cols.size());
         const int64_t c0 = m->sys_col[s], c1 = c0 + m->cols_per_sys;
-        for (int64_t c = c0; c < c1; c += cchunk) g.cols.push_back(ColWork{c, std::min(c1, c + cchunk), s, 0});
+        for (int64_t c = c0; c < c1; c += cchunk) g.cols.push_back(ColWork{c, std::min(c1, c + cchunk), s, c == c0});
         g.col_count[s] = int32_t(g.cols.size()) - g.col_first[s];
     }
     g.d_rows = to_device(g.rows, ctx.stream);

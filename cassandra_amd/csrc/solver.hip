@@ -126,7 +126,7 @@ void launch_absmax(ScatterPlan& sp, const YT* y, const GridPlan& g, int nsys, hi
 template <class VT, class YT, class View>
 void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
                           const LsqrScalars* div, int mode, hipStream_t s, bool y_padded = false,
-                          const ActiveLists* al = nullptr)
+                          const ActiveLists* al = nullptr, const GridSys& bsum = GridSys{})
 {
     ScatterLists& L = lists<VT>(sp);
     const ScatterWork* work = al ? al->scatter.as<ScatterWork>() : L.d_work.as<ScatterWork>();
@@ -140,8 +140,8 @@ void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint
     with_scatter_kernel<VT, YT, View>(rpt, [&](auto kern) {
         allow_lds(kern, lds);
         hipLaunchKernelGGL(kern, dim3(unsigned(nwork)), dim3(kScatterBlock), lds, s, view, work,
-                           L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(), sys_col, y, div, mode, fx,
-                           int(y_padded), partials<VT>(L));
+                           L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(), sys_col, y,
+                           const_cast<LsqrScalars*>(div), mode, fx, int(y_padded), partials<VT>(L), bsum);
     });
     CAD_HIP(hipGetLastError());
 }
@@ -279,9 +279,12 @@ RowList row_list(cad_matrix* m, const ActiveLists* al)
 
 // Row gather with an epilogue: the LDS-staged tile kernel for pattern
 // matrices, the generic row kernel otherwise.  al: launch only the blocks of
-// the systems it holds (with the row kernel when it says so).
+// the systems it holds (with the row kernel when it says so).  finish: false
+// leaves the epilogue's block partials for the next launch to sum (the LSQR
+// iteration's scatter forms beta from them).
 template <class VT, class Epi>
-void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s, const ActiveLists* al = nullptr)
+void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s, const ActiveLists* al = nullptr,
+            bool finish = true)
 {
     GatherPlan& gp = gather_plan<VT>(m);
     const RowList rl = row_list<VT>(m, al);
@@ -319,8 +322,10 @@ void gather(cad_matrix* m, const VT* x, Epi epi, hipStream_t s, const ActiveList
     CAD_HIP(hipGetLastError());
     // the epilogue's per-system finish, from the block partials the gather left
     if constexpr (finish_kernel<Epi>()) {
-        hipLaunchKernelGGL(k_epi_finish<Epi>, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, epi, rl.sys);
-        CAD_HIP(hipGetLastError());
+        if (finish) {
+            hipLaunchKernelGGL(k_epi_finish<Epi>, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, epi, rl.sys);
+            CAD_HIP(hipGetLastError());
+        }
     }
 }
 
@@ -497,13 +502,16 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s, const Ac
 {
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
-    gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s, al);
-    // A^T (u / beta): the tiles, then the column reduction fused with vbar =
-    // D z - beta v (u is padded to whole 16-byte groups)
+    // u_raw = A v - alpha u, leaving the block partials of |u_raw|^2 ...
+    gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s, al, false);
+    // ... from which the tiles of A^T (u / beta) form beta; then the column
+    // reduction fused with vbar = D z - beta v (u is padded to whole 16-byte
+    // groups), and the rotation with the column update
     ScatterLists& L = lists<VT>(m->splan);
+    const GridSys usys = row_list<VT>(m, al).sys;
     with_view(m, [&](auto view) {
         launch_scatter_tiles<VT, VT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), ws.u.as<VT>(), sc,
-                                     kScatterLsqr, s, true, al);
+                                     kScatterLsqr, s, true, al, usys);
     });
     const GridSys rsys = al ? GridSys{al->rfirst.as<int32_t>(), al->rcount.as<int32_t>(),
                                       L.reduce_sys().counters, L.reduce_sys().partials}
@@ -512,15 +520,12 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s, const Ac
     const int nred = al ? al->nreduce : int(L.reduce.size());
     if (nred > 0)
         hipLaunchKernelGGL(k_lsqr_reduce_v<VT>, dim3(unsigned(nred)), dim3(kRowBlock), 0, s, red,
-                           (const VT*)partials<VT>(L), ws.D.as<double>(), ws.v.as<double>(), rsys,
-                           (const LsqrScalars*)sc);
-    hipLaunchKernelGGL(k_lsqr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, rsys, sc);
+                           (const VT*)partials<VT>(L), ws.D.as<double>(), ws.v.as<double>(), rsys, sc);
     const ColWork* cw = al ? al->cols.as<ColWork>() : g.d_cols.as<ColWork>();
     const int ncw = al ? al->ncols : int(g.cols.size());
     if (ncw > 0)
-        hipLaunchKernelGGL(k_lsqr_xw<VT>, dim3(unsigned(ncw)), dim3(kRowBlock), 0, s, cw, ws.D.as<double>(),
-                           ws.v.as<double>(), ws.vd.as<VT>(), ws.w.as<double>(), ws.y.as<double>(),
-                           (const LsqrScalars*)sc);
+        hipLaunchKernelGGL(k_lsqr_step<VT>, dim3(unsigned(ncw)), dim3(kRowBlock), 0, s, cw, rsys, ws.D.as<double>(),
+                           ws.v.as<double>(), ws.vd.as<VT>(), ws.w.as<double>(), ws.y.as<double>(), sc);
     CAD_HIP(hipGetLastError());
 }
 
@@ -1166,7 +1171,7 @@ static void lsmr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     if (!L.reduce.empty())
         hipLaunchKernelGGL(k_lsqr_reduce_v<double>, dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
                            L.d_reduce.as<ReduceWork>(), (const double*)partials<double>(L), ws.D.as<double>(),
-                           ws.v.as<double>(), L.reduce_sys(), (const LsqrScalars*)sc);
+                           ws.v.as<double>(), L.reduce_sys(), sc);
     hipLaunchKernelGGL(k_lsmr_rot, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, L.reduce_sys(), sc,
                        ws.lsmr.as<LsmrState>());
     hipLaunchKernelGGL(k_lsmr_update<double>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
