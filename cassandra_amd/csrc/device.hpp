@@ -177,7 +177,7 @@ struct LsmrState {
 struct PcgScalars {
     double rz, rz_new, pq, alpha, beta, rr;
     int32_t active, iters;
-    double pad;
+    double rz0;  // rz as the iteration's update found it
 };
 
 // Per-system grid bookkeeping shared by the reducing kernels.

@@ -127,6 +127,27 @@ __device__ __forceinline__ double system_sum(const GridSys& g, int sys, double* 
     return sh[0];
 }
 
+// The same for both slots of the partials.
+template <int BS>
+__device__ __forceinline__ void system_sum2(const GridSys& g, int sys, double* sh, double& s0, double& s1)
+{
+    double a0 = 0.0, a1 = 0.0;
+    const int first = g.blk_first[sys], n = g.blk_count[sys];
+    for (int i = threadIdx.x; i < n; i += BS) {
+        a0 += g.partials[2 * (first + i)];
+        a1 += g.partials[2 * (first + i) + 1];
+    }
+    a0 = block_sum<BS>(a0, sh);
+    a1 = block_sum<BS>(a1, sh);
+    if (threadIdx.x == 0) {
+        sh[0] = a0;
+        sh[1] = a1;
+    }
+    __syncthreads();
+    s0 = sh[0];
+    s1 = sh[1];
+}
+
 // ------------------------------------------------------------ fill --------
 template <class T>
 __global__ void k_fill(T* __restrict__ p, int64_t n, T v)
@@ -2146,64 +2167,55 @@ __global__ void k_scale_mul(const double* __restrict__ D, const double* __restri
 
 // ------------------------------------------------------------ PCG ---------
 // PCG on A^T A with a diagonal preconditioner (Math/Solver.h:78-117); the
-// FP64 scalars live on the device.  pq = p . q, alpha = rz / pq.
+// FP64 scalars live on the device.  An iteration is five launches: the gather
+// t = A p, the window tiles of A^T t, then
+//   k_pcg_reduce_q  q = A^T t (the slices' sum per column) and the block
+//                   partials of p.q, one block per reduce item;
+//   k_pcg_update    alpha = rz / pq, formed by every block from those partials
+//                   (the same value in each), then x, r, zz = P r and the
+//                   partials of r.zz and r.r;
+//   k_pcg_p         beta = rz' / rz formed the same way, then p = zz + beta p.
+// A system's first column block stores the scalars.  k_pcg_update keeps rz
+// aside as rz0, so that k_pcg_p's blocks read the old rz while its first
+// block stores the new one.  faithful: the scalars are rounded to VT before
+// use, as in the float solver.
 template <class VT>
-__global__ __launch_bounds__(kRowBlock) void k_pcg_alpha(const ColWork* __restrict__ work, const VT* __restrict__ p,
-                                                         const VT* __restrict__ q, GridSys g, PcgScalars* scal)
+__global__ __launch_bounds__(kRowBlock) void k_pcg_reduce_q(const ReduceWork* __restrict__ work,
+                                                            const VT* __restrict__ part, const VT* __restrict__ p,
+                                                            VT* __restrict__ q, GridSys g, const PcgScalars* scal)
 {
     __shared__ double sh[kRowBlock / 64];
-    const ColWork w = work[blockIdx.x];
-    PcgScalars* st = &scal[w.sys];
-    if (!st->active) return;
+    const ReduceWork w = work[blockIdx.x];
+    if (!scal[w.sys].active) return;
     double acc = 0.0;
-    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) acc += double(p[c]) * double(q[c]);
+    for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
+        const VT qc = VT(reduce_column(part, w, c));
+        q[c] = qc;
+        acc += double(p[c]) * double(qc);
+    }
     acc = block_sum<kRowBlock>(acc, sh);
     if (threadIdx.x == 0) g.partials[2 * blockIdx.x] = acc;
 }
 
-// Per system (one block each): the sums of the column kernels' block partials
-// (slot 0 and slot 1 of each block), in block order.  The column kernels
-// leave partials instead of ending with a last-block ticket, whose
-// device-scope fences cost every block an L2 write-back and invalidate.
-__device__ __forceinline__ void sum_partials(const GridSys& g, int sys, double& s0, double& s1, double* sh)
-{
-    s0 = 0.0;
-    s1 = 0.0;
-    for (int i = threadIdx.x; i < g.blk_count[sys]; i += kRowBlock) {
-        s0 += g.partials[2 * (g.blk_first[sys] + i)];
-        s1 += g.partials[2 * (g.blk_first[sys] + i) + 1];
-    }
-    s0 = block_sum<kRowBlock>(s0, sh);
-    s1 = block_sum<kRowBlock>(s1, sh);
-}
-
-// pq = p . q (slot 0), alpha = rz / pq.
-__global__ __launch_bounds__(kRowBlock) void k_pcg_alpha_fin(GridSys g, PcgScalars* scal)
-{
-    __shared__ double sh[kRowBlock / 64];
-    PcgScalars* st = &scal[blockIdx.x];
-    if (!st->active) return;
-    double s0, s1;
-    sum_partials(g, blockIdx.x, s0, s1, sh);
-    if (threadIdx.x == 0) {
-        st->pq = s0;
-        st->alpha = s0 != 0.0 ? st->rz / s0 : 0.0;
-    }
-}
-
-// x += alpha p, r -= alpha q, zz = P r; rz' = r.zz, rr = r.r; beta = rz'/rz.
-// faithful: the scalars are rounded to VT before use, as in the float solver.
+// gq: k_pcg_reduce_q's partials; g: this launch's
 template <class VT>
 __global__ __launch_bounds__(kRowBlock) void k_pcg_update(const ColWork* __restrict__ work, const VT* __restrict__ P,
                                                           const VT* __restrict__ p, const VT* __restrict__ q,
                                                           VT* __restrict__ x, VT* __restrict__ r, VT* __restrict__ zz,
-                                                          GridSys g, PcgScalars* scal)
+                                                          GridSys gq, GridSys g, PcgScalars* scal)
 {
     __shared__ double sh[kRowBlock / 64];
     const ColWork w = work[blockIdx.x];
     PcgScalars* st = &scal[w.sys];
     if (!st->active) return;
-    const VT a = VT(st->alpha);
+    const double pq = system_sum<kRowBlock>(gq, w.sys, sh);
+    const double alpha = pq != 0.0 ? st->rz / pq : 0.0;
+    if (w.first && threadIdx.x == 0) {
+        st->pq = pq;
+        st->alpha = alpha;
+        st->rz0 = st->rz;
+    }
+    const VT a = VT(alpha);
     double a0 = 0.0, a1 = 0.0;
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         x[c] += a * p[c];
@@ -2222,31 +2234,27 @@ __global__ __launch_bounds__(kRowBlock) void k_pcg_update(const ColWork* __restr
     }
 }
 
-// rz' (slot 0), rr (slot 1), beta = rz' / rz.
-__global__ __launch_bounds__(kRowBlock) void k_pcg_update_fin(GridSys g, PcgScalars* scal)
-{
-    __shared__ double sh[kRowBlock / 64];
-    PcgScalars* st = &scal[blockIdx.x];
-    if (!st->active) return;
-    double s0, s1;
-    sum_partials(g, blockIdx.x, s0, s1, sh);
-    if (threadIdx.x == 0) {
-        st->beta = st->rz != 0.0 ? s0 / st->rz : 0.0;
-        st->rz = s0;
-        st->rr = s1;
-        st->iters += 1;
-    }
-}
-
+// rz' (slot 0 of k_pcg_update's partials), rr (slot 1), beta = rz' / rz;
 // p = zz + beta p; q = 0 for the next scatter.
 template <class VT>
 __global__ __launch_bounds__(kRowBlock) void k_pcg_p(const ColWork* __restrict__ work, const VT* __restrict__ zz,
-                                                     VT* __restrict__ p, VT* __restrict__ q, const PcgScalars* scal)
+                                                     VT* __restrict__ p, VT* __restrict__ q, GridSys g,
+                                                     PcgScalars* scal)
 {
+    __shared__ double sh[kRowBlock / 64];
     const ColWork w = work[blockIdx.x];
-    const PcgScalars* st = &scal[w.sys];
+    PcgScalars* st = &scal[w.sys];
     if (!st->active) return;
-    const VT b = VT(st->beta);
+    double rz, rr;
+    system_sum2<kRowBlock>(g, w.sys, sh, rz, rr);
+    const double beta = st->rz0 != 0.0 ? rz / st->rz0 : 0.0;
+    if (w.first && threadIdx.x == 0) {
+        st->beta = beta;
+        st->rz = rz;
+        st->rr = rr;
+        st->iters += 1;
+    }
+    const VT b = VT(beta);
     for (int64_t c = w.col_begin + threadIdx.x; c < w.col_end; c += kRowBlock) {
         p[c] = zz[c] + b * p[c];
         q[c] = VT(0);

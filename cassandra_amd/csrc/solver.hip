@@ -1050,16 +1050,24 @@ static void pcg_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     const GridPlan& g = m->grid;
     PcgScalars* sc = ws.pscal.as<PcgScalars>();
     spmv<VT>(m, ws.p.as<VT>(), ws.t.as<VT>(), s);
-    scatter<VT, VT, VT>(m, ws.t.as<VT>(), nullptr, ws.q.as<VT>(), s);
-    hipLaunchKernelGGL(k_pcg_alpha<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
-                       ws.p.as<VT>(), ws.q.as<VT>(), g.col_sys(), sc);
-    hipLaunchKernelGGL(k_pcg_alpha_fin, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, g.col_sys(), sc);
+    // A^T t: the window tiles (after the max |t| pre-pass of the fixed-point
+    // windows), then the column sums with the partials of p.q
+    ScatterLists& L = lists<VT>(m->splan);
+    with_view(m, [&](auto view) {
+        if (fixed_point_scatter<VT, decltype(view)>() && !L.work.empty())
+            launch_absmax(m->splan, ws.t.as<VT>(), g, m->nsys, s);
+        launch_scatter_tiles<VT, VT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), ws.t.as<VT>(),
+                                     nullptr, kScatterAll, s);
+    });
+    if (!L.reduce.empty())
+        hipLaunchKernelGGL(k_pcg_reduce_q<VT>, dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
+                           L.d_reduce.as<ReduceWork>(), (const VT*)partials<VT>(L), ws.p.as<VT>(), ws.q.as<VT>(),
+                           L.reduce_sys(), (const PcgScalars*)sc);
     hipLaunchKernelGGL(k_pcg_update<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s,
                        g.d_cols.as<ColWork>(), ws.P.as<VT>(), ws.p.as<VT>(), ws.q.as<VT>(), ws.xr.as<VT>(),
-                       ws.rr.as<VT>(), ws.zz.as<VT>(), g.col_sys(), sc);
-    hipLaunchKernelGGL(k_pcg_update_fin, dim3(unsigned(m->nsys)), dim3(kRowBlock), 0, s, g.col_sys(), sc);
+                       ws.rr.as<VT>(), ws.zz.as<VT>(), L.reduce_sys(), g.col_sys(), sc);
     hipLaunchKernelGGL(k_pcg_p<VT>, dim3(unsigned(g.cols.size())), dim3(kRowBlock), 0, s, g.d_cols.as<ColWork>(),
-                       ws.zz.as<VT>(), ws.p.as<VT>(), ws.q.as<VT>(), (const PcgScalars*)sc);
+                       ws.zz.as<VT>(), ws.p.as<VT>(), ws.q.as<VT>(), g.col_sys(), sc);
     CAD_HIP(hipGetLastError());
 }
 
