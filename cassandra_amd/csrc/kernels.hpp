@@ -510,10 +510,12 @@ struct EpiLsqrU {
         for (int i = 0; i < RPT; i++) u[a + i] = o[i];
         return red;
     }
+    // beta = |u_raw|; its square is kept for the rotation, which adds it to
+    // the Frobenius estimate (as the scatter that forms beta itself does)
     __device__ void finish(int sys, double s)
     {
         scal[sys].beta = sqrt(s);
-        scal[sys].anorm2 += s;
+        scal[sys].beta2 = s;
     }
 };
 
@@ -1308,10 +1310,12 @@ __device__ __forceinline__ VT scatter_scale(const LsqrScalars* div, int mode, in
 // partials (bsum.partials non-null): beta = sqrt of their sum is formed here,
 // by every block from the same partials in the same order (so all blocks
 // agree), and stored with its square for the kernels after; the scatter then
-// reads no beta from the state.
-template <class VT, int BS>
+// reads no beta from the state.  Only the kBeta instantiations of the scatter
+// kernels carry this code (the others compile as if it did not exist).
+template <class VT, int BS, bool kBeta>
 __device__ __forceinline__ VT scatter_scale(LsqrScalars* div, int mode, int sys, const GridSys& bsum)
 {
+    if constexpr (!kBeta) return scatter_scale<VT>(div, mode, sys);
     if (mode != kScatterLsqr || !bsum.partials) return scatter_scale<VT>(div, mode, sys);
     __shared__ double sh[BS / 64];
     const double s = system_sum<BS>(bsum, sys, sh);
@@ -1412,7 +1416,7 @@ __global__ __launch_bounds__(kRowBlock) void k_absmax_systems(const int32_t* __r
 // atomic adds on doubles run several times faster than on floats on this GPU,
 // and the sums are exact for integer data either way.
 // scale = 1 / div[sys].beta when div is given (0 when beta == 0), else 1.
-template <class VT, class YT, class View>
+template <class VT, class YT, class View, bool kBeta = false>
 __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const ScatterWork* __restrict__ work,
                                                                 const SlotGroup* __restrict__ groups,
                                                                 const uint16_t* __restrict__ slot_list,
@@ -1430,7 +1434,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_part(View m, const Sc
     __syncthreads();
     const SlotGroup gr = groups[w.group];
     const uint32_t cb = sys_col[w.sys];
-    const VT sc = scatter_scale<VT, kScatterBlock>(div, mode, w.sys, bsum);
+    const VT sc = scatter_scale<VT, kScatterBlock, kBeta>(div, mode, w.sys, bsum);
     for (int64_t r = w.row_begin + threadIdx.x; r < w.row_end; r += kScatterBlock) {
         const double yv = double(VT(y[r]) * sc);
         for (int j = 0; j < gr.nslots; j++) {
@@ -1532,7 +1536,7 @@ __device__ __forceinline__ void scatter_rows(const SlotMajor<IdxT>& m, const Sca
 // For groups of up to kMaxSegSlots slots the slot numbers and window offsets
 // are hoisted out of the row loop and the loads are software-pipelined
 // (scatter_rows).
-template <class VT, class YT, class IdxT, int RPT>
+template <class VT, class YT, class IdxT, int RPT, bool kBeta = false>
 __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> m,
                                                                 const ScatterWork* __restrict__ work,
                                                                 const SlotGroup* __restrict__ groups,
@@ -1554,7 +1558,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     const SlotGroup gr = groups[w.group];
     const int ns = gr.nslots;
     const uint32_t cb = sys_col[w.sys];
-    const VT sc = scatter_scale<VT, kScatterBlock>(div, mode, w.sys, bsum);
+    const VT sc = scatter_scale<VT, kScatterBlock, kBeta>(div, mode, w.sys, bsum);
     const int64_t vld = m.ld / RPT;
     // FP32 windows accumulate in fixed point where the scale allows it
     const double fscale = std::is_same_v<VT, float> && ns <= kMaxSegSlots ? fixed_scale(fx, w.sys, double(sc)) : 0.0;
@@ -1641,7 +1645,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
 // Stage 1 for a matrix-free matrix: as k_scatter_tile, with each thread's RPT
 // positions loaded once per step and every slot's column computed by its
 // program (the DevSlot is the same for the whole block).
-template <class VT, class YT, int RPT>
+template <class VT, class YT, int RPT, bool kBeta = false>
 __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView m,
                                                                    const ScatterWork* __restrict__ work,
                                                                    const SlotGroup* __restrict__ groups,
@@ -1660,7 +1664,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile_mf(PositionsView
     __syncthreads();
     const SlotGroup gr = uniform_load(groups + w.group);
     const uint32_t cb = sys_col[w.sys];
-    const VT sc = scatter_scale<VT, kScatterBlock>(div, mode, w.sys, bsum);
+    const VT sc = scatter_scale<VT, kScatterBlock, kBeta>(div, mode, w.sys, bsum);
     const int64_t last = w.row_end - 1;
     for (int64_t a = (w.row_begin & ~int64_t(RPT - 1)) + int64_t(RPT) * threadIdx.x; a < w.row_end;
          a += int64_t(RPT) * kScatterBlock) {
@@ -2086,6 +2090,7 @@ __global__ __launch_bounds__(kRowBlock) void k_lsmr_rot(GridSys g, LsqrScalars* 
     ls->k_h = rho != 0.0 ? -theta_new / rho : 0.0;
     st->alpha = alpha;
     st->nres = fabs(ls->zetabar);
+    st->anorm2 += st->beta2;
     st->anorm2 += sum;
     st->iters += 1;
     if (st->target > 0.0 && st->nres <= st->target * st->nres0) st->stop = 1;

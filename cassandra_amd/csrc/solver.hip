@@ -86,19 +86,25 @@ PT* partials(ScatterLists& L)
 // The first-stage scatter kernel of a view: vector loads of rpt (4, or 8 for
 // u16 indices) consecutive rows for stored indices, 4 positions per step for
 // a matrix-free matrix, one row per step otherwise; f receives the kernel.
+// beta: the instantiation that forms an LSQR iteration's beta itself.
 template <class VT, class YT, class View, class F>
-void with_scatter_kernel(int rpt, F&& f)
+void with_scatter_kernel(int rpt, bool beta, F&& f)
 {
-    if constexpr (std::is_same_v<View, SlotMajor<uint16_t>>) {
-        if (rpt == 8) return f(k_scatter_tile<VT, YT, uint16_t, 8>);
-        return f(k_scatter_tile<VT, YT, uint16_t, 4>);
-    } else if constexpr (std::is_same_v<View, SlotMajor<uint32_t>>) {
-        return f(k_scatter_tile<VT, YT, uint32_t, 4>);
-    } else if constexpr (std::is_same_v<View, PositionsView>) {
-        return f(k_scatter_tile_mf<VT, YT, 4>);
-    } else {
-        return f(k_scatter_part<VT, YT, View>);
-    }
+    auto pick = [&](auto kb) {
+        constexpr bool B = decltype(kb)::value;
+        if constexpr (std::is_same_v<View, SlotMajor<uint16_t>>) {
+            if (rpt == 8) return f(k_scatter_tile<VT, YT, uint16_t, 8, B>);
+            return f(k_scatter_tile<VT, YT, uint16_t, 4, B>);
+        } else if constexpr (std::is_same_v<View, SlotMajor<uint32_t>>) {
+            return f(k_scatter_tile<VT, YT, uint32_t, 4, B>);
+        } else if constexpr (std::is_same_v<View, PositionsView>) {
+            return f(k_scatter_tile_mf<VT, YT, 4, B>);
+        } else {
+            return f(k_scatter_part<VT, YT, View, B>);
+        }
+    };
+    if (beta) return pick(std::true_type{});
+    return pick(std::false_type{});
 }
 
 // A^T y in stages: the max |y| pre-pass of the fixed-point FP32 windows
@@ -137,7 +143,7 @@ void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint
     const FixedScale fx{fixed_point_scatter<VT, View>() && mode != kScatterLsqr ? sp.d_ymax.as<unsigned>() : nullptr,
                         L.cbits};
     const size_t lds = size_t(L.widest) * sizeof(double);
-    with_scatter_kernel<VT, YT, View>(rpt, [&](auto kern) {
+    with_scatter_kernel<VT, YT, View>(rpt, bsum.partials != nullptr, [&](auto kern) {
         allow_lds(kern, lds);
         hipLaunchKernelGGL(kern, dim3(unsigned(nwork)), dim3(kScatterBlock), lds, s, view, work,
                            L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(), sys_col, y,
@@ -174,7 +180,8 @@ void prepare_scatter(cad_matrix* m)
     partials<VT>(L);
     const size_t lds = size_t(L.widest) * sizeof(double);
     with_view(m, [&](auto view) {
-        with_scatter_kernel<VT, YT, decltype(view)>(m->tune.srpt, [&](auto kern) { allow_lds(kern, lds); });
+        for (bool beta : {false, true})
+            with_scatter_kernel<VT, YT, decltype(view)>(m->tune.srpt, beta, [&](auto kern) { allow_lds(kern, lds); });
     });
 }
 
@@ -502,13 +509,17 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s, const Ac
 {
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
-    // u_raw = A v - alpha u, leaving the block partials of |u_raw|^2 ...
-    gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s, al, false);
-    // ... from which the tiles of A^T (u / beta) form beta; then the column
-    // reduction fused with vbar = D z - beta v (u is padded to whole 16-byte
-    // groups), and the rotation with the column update
+    // u_raw = A v - alpha u, leaving the block partials of |u_raw|^2, from
+    // which the tiles of A^T (u / beta) form beta -- except on a matrix-free
+    // matrix, whose scatter variant with that prologue runs about 10% slower (its
+    // index programs need the registers): there the gather's finish launch
+    // forms beta.  Then the column reduction fused with vbar = D z - beta v
+    // (u is padded to whole 16-byte groups), and the rotation with the
+    // column update.
+    const bool fuse = !m->mf;
+    gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s, al, !fuse);
     ScatterLists& L = lists<VT>(m->splan);
-    const GridSys usys = row_list<VT>(m, al).sys;
+    const GridSys usys = fuse ? row_list<VT>(m, al).sys : GridSys{};
     with_view(m, [&](auto view) {
         launch_scatter_tiles<VT, VT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), ws.u.as<VT>(), sc,
                                      kScatterLsqr, s, true, al, usys);
