@@ -77,11 +77,14 @@ constexpr uint32_t kReduceCols = kRowBlock;  // columns of one scatter-reduce wo
 
 // Row chunks of one window of a system: the share of the target tile count,
 // capped so that the partial slices stay near partial_cap of the window's
-// index traffic -- and, for a small system that cap would leave on a few
-// CUs, raised towards one tile per CU while its slices stay under
-// kPartialFloorBytes a window (their write and read then cost less than the
-// parallelism gains).  At least kScatterBlock rows a chunk.
+// index traffic -- and, where that cap would leave CUs idle, raised towards
+// one tile per CU while its slices stay under kPartialFloorBytes a window
+// (their write and read then cost less than the parallelism gains).
+// `windows` counts the windows of every system planned together: those of
+// many systems already fill the CUs side by side.  At least kScatterBlock
+// rows a chunk.
 constexpr double kPartialFloorBytes = 2.0 * 1024 * 1024;
+constexpr double kMultiSystemTiles = 8.0;  // target tiles a CU, over blocks_per_cu, for several systems
 
 int64_t window_chunks(int64_t R, double share_chunks, double cap_chunks, int cus, int windows, uint32_t W,
                       int vbytes)
@@ -181,10 +184,12 @@ constexpr int kXcds = 8;
 
 void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
                             const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, uint32_t wmax, int vbytes,
-                            int ibytes, double target, int cus, const TuneParams& tune, ScatterLists& L)
+                            int ibytes, double target, int cus, const TuneParams& tune, ScatterLists& L,
+                            const std::vector<uint8_t>* on)
 {
     (void)sp;
     const int nsys = int(sys_col.size());
+    auto in = [&](int s) { return !on || (*on)[size_t(s)] != 0; };
     // Windows of the slot groups; a window with more slots than the lightest
     // is split into parts (row sub-ranges of the chunk), so that every tile
     // of a chunk does about the same work and the chunk's tiles, running
@@ -211,7 +216,12 @@ void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& s
         tiles_per_chunk += w.parts;
     }
     int64_t total_rows = 0;
-    for (int s = 0; s < nsys; s++) total_rows += sys_rows[s + 1] - sys_rows[s];
+    int nrun = 0;  // systems planned here that have rows: their windows run side by side
+    for (int s = 0; s < nsys; s++)
+        if (in(s)) {
+            total_rows += sys_rows[s + 1] - sys_rows[s];
+            nrun += sys_rows[s + 1] > sys_rows[s];
+        }
     // chunks: about target tiles in all, a multiple of the XCD count
     int64_t chunks = std::max<int64_t>(1, std::llround(target / std::max(1, tiles_per_chunk)));
     if (chunks > kXcds) chunks = chunks / kXcds * kXcds;
@@ -219,6 +229,7 @@ void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& s
     int64_t off = 0;
     int chunk_id = 0;
     for (int s = 0; s < nsys; s++) {
+        if (!in(s)) continue;
         const int64_t R = sys_rows[s + 1] - sys_rows[s];
         int64_t C = 0;
         std::vector<int64_t> woff(size_t(std::max(1, nw)));
@@ -226,8 +237,8 @@ void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& s
             const double share = total_rows > 0 ? double(R) / double(total_rows) : 1.0;
             const double cap = tune.partial_cap * double(R) * work_per_row * ibytes /
                                (2.0 * width * vbytes * double(tiles_per_chunk) / std::max(1, nw));
-            C = window_chunks(R, double(chunks) * share, cap, cus, tiles_per_chunk, uint32_t(width / std::max(1, nw)),
-                              vbytes);
+            C = window_chunks(R, double(chunks) * share, cap, cus, tiles_per_chunk * std::max(1, nrun),
+                              uint32_t(width / std::max(1, nw)), vbytes);
             const int64_t chunk = (R + C - 1) / C;
             C = (R + chunk - 1) / chunk;
             for (int j = 0; j < nw; j++) {
@@ -289,13 +300,15 @@ void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& s
 // 10% of the window's index traffic.
 void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
                         const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, uint32_t wmax, int vbytes,
-                        int ibytes, int cus, const TuneParams& tune, ScatterLists& L)
+                        int ibytes, int cus, const TuneParams& tune, ScatterLists& L,
+                        const std::vector<uint8_t>* on = nullptr)
 {
     L.work.clear();
     L.reduce.clear();
     L.widest = 0;
     pack_groups(sp, wmax, L);
     const int nsys = int(sys_col.size());
+    auto in = [&](int s) { return !on || (*on)[size_t(s)] != 0; };
     // Work of a window per row: its slot loads, and its atomics -- the share
     // of the group's entries that fall into the window, taken as its share of
     // the group's columns.  scatter_balance weighs the two (1: loads only).
@@ -305,21 +318,27 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
         return g.nslots * (lam + (1.0 - lam) * frac);
     };
     double total = 0;
-    for (int s = 0; s < nsys; s++)
-        for (auto& g : L.groups) {
-            const int64_t R = sys_rows[s + 1] - sys_rows[s];
+    int nrun = 0;  // systems planned here that have rows: their windows run side by side
+    for (int s = 0; s < nsys; s++) {
+        if (!in(s)) continue;
+        const int64_t R = sys_rows[s + 1] - sys_rows[s];
+        nrun += R > 0;
+        for (auto& g : L.groups)
             for (uint32_t wl = g.col_lo; wl < g.col_hi; wl += wmax)
                 total += double(R) * win_cost(g, std::min(g.col_hi, wl + wmax) - wl);
-        }
-    const double target = tune.blocks_per_cu * std::max(cus, 1);
+    }
+    // several systems side by side have tiles of very different sizes (their
+    // row counts differ): more tiles a CU let the launch balance them
+    const double target = tune.blocks_per_cu * std::max(cus, 1) * (nrun > 1 ? kMultiSystemTiles : 1.0);
     int windows = 0;
     for (auto& g : L.groups) windows += int((g.col_hi - g.col_lo + wmax - 1) / wmax);
     int64_t off = 0;
     if (tune.scatter_order == 1) {
-        make_scatter_lists_xcd(sp, sys_rows, sys_col, cols_per_sys, wmax, vbytes, ibytes, target, cus, tune, L);
+        make_scatter_lists_xcd(sp, sys_rows, sys_col, cols_per_sys, wmax, vbytes, ibytes, target, cus, tune, L, on);
         return;
     }
     for (int s = 0; s < nsys; s++) {
+        if (!in(s)) continue;
         const int64_t R = sys_rows[s + 1] - sys_rows[s];
         std::vector<std::pair<uint32_t, uint32_t>> covered;
         for (size_t gi = 0; gi < L.groups.size(); gi++) {
@@ -333,7 +352,7 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
                 if (R > 0) {
                     const double share = total > 0 ? double(R) * win_cost(g, W) / total : 1.0;
                     const double cap = tune.partial_cap * double(R) * g.nslots * ibytes / (2.0 * W * vbytes);
-                    C = window_chunks(R, target * share, cap, cus, windows, W, vbytes);
+                    C = window_chunks(R, target * share, cap, cus, windows * std::max(1, nrun), W, vbytes);
                     const int64_t chunk = (R + C - 1) / C;
                     C = (R + chunk - 1) / chunk;
                     for (int64_t c = 0; c < C; c++) {
@@ -384,6 +403,37 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
         }
     }
     L.part_elems = off;
+    // longest first: a launch of more tiles than CUs then starts its big
+    // chunks in the first wave and fills in with the small ones (each tile
+    // writes its own slice, so the order changes no sum)
+    auto cost = [&](const ScatterWork& w) {
+        return double(w.row_end - w.row_begin) * win_cost(L.groups[size_t(w.group)], w.win_hi - w.win_lo);
+    };
+    std::stable_sort(L.work.begin(), L.work.end(),
+                     [&](const ScatterWork& a, const ScatterWork& b) { return cost(a) > cost(b); });
+}
+
+// The host side of a list after planning: the bound on the terms a window
+// column receives in one work item (rows x slots of the group), for the
+// fixed-point scale of the FP32 scatter, and the reduce items ordered by
+// system with each system's range of them.
+void finish_lists(ScatterLists& L, int nsys)
+{
+    L.cbits = 0;
+    for (const ScatterWork& w : L.work) {
+        uint64_t n = uint64_t(std::max<int64_t>(0, w.row_end - w.row_begin)) * L.groups[size_t(w.group)].nslots;
+        int b = 0;
+        while (n) b++, n >>= 1;
+        L.cbits = std::max(L.cbits, b);
+    }
+    std::stable_sort(L.reduce.begin(), L.reduce.end(),
+                     [](const ReduceWork& a, const ReduceWork& b) { return a.sys < b.sys; });
+    L.red_first.assign(size_t(nsys), 0);
+    L.red_count.assign(size_t(nsys), 0);
+    for (size_t i = L.reduce.size(); i-- > 0;) {
+        L.red_first[size_t(L.reduce[i].sys)] = int32_t(i);
+        L.red_count[size_t(L.reduce[i].sys)]++;
+    }
 }
 
 }  // namespace
@@ -433,17 +483,9 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
     t4.scatter_balance = 1.0;
     make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 4), 4, ibytes, ctx.cus, t4, sp.f4);
     make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds / 8), 8, ibytes, ctx.cus, tune, sp.f8);
-    // bound on the terms a window column receives in one work item (rows x
-    // slots of the group), for the fixed-point scale of the FP32 scatter
-    for (ScatterLists* L : {&sp.f4, &sp.f8}) {
-        L->cbits = 0;
-        for (const ScatterWork& w : L->work) {
-            uint64_t n = uint64_t(std::max<int64_t>(0, w.row_end - w.row_begin)) * L->groups[size_t(w.group)].nslots;
-            int b = 0;
-            while (n) b++, n >>= 1;
-            L->cbits = std::max(L->cbits, b);
-        }
-    }
+    const int nsys = int(sys_rows.size()) - 1;
+    finish_lists(sp.f4, nsys);
+    finish_lists(sp.f8, nsys);
     sp.d_groups = to_device(sp.groups, ctx.stream);
     // slot lists padded to whole 32-bit words (the kernels read them as words)
     auto padded = [](std::vector<uint16_t> v) {
@@ -451,16 +493,7 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
         return v;
     };
     sp.d_slot_list = to_device(padded(sp.slot_list), ctx.stream);
-    const int nsys = int(sys_rows.size()) - 1;
     for (ScatterLists* L : {&sp.f4, &sp.f8}) {
-        std::stable_sort(L->reduce.begin(), L->reduce.end(),
-                         [](const ReduceWork& a, const ReduceWork& b) { return a.sys < b.sys; });
-        L->red_first.assign(size_t(nsys), 0);
-        L->red_count.assign(size_t(nsys), 0);
-        for (size_t i = L->reduce.size(); i-- > 0;) {
-            L->red_first[size_t(L->reduce[i].sys)] = int32_t(i);
-            L->red_count[size_t(L->reduce[i].sys)]++;
-        }
         L->d_groups = to_device(L->groups, ctx.stream);
         L->d_slot_list = to_device(padded(L->slot_list), ctx.stream);
         L->d_work = to_device(L->work, ctx.stream);
@@ -471,6 +504,15 @@ void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const 
         L->d_red_partials = DBuf(2 * sizeof(double) * std::max<size_t>(1, L->reduce.size()));
         L->part = DBuf();
     }
+}
+
+void plan_scatter_subset(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
+                         const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, int ibytes, const Context& ctx,
+                         const TuneParams& tune, const std::vector<uint8_t>& on, ScatterLists& L)
+{
+    make_scatter_lists(sp, sys_rows, sys_col, cols_per_sys, uint32_t(lds_budget(ctx) / 8), 8, ibytes, ctx.cus, tune, L,
+                       &on);
+    finish_lists(L, int(sys_rows.size()) - 1);
 }
 
 GridSys ScatterLists::reduce_sys() const

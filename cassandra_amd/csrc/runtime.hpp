@@ -235,5 +235,13 @@ void build_gather_plan(cad_matrix* m, GatherPlan& gp, int vbytes, int rpt, int b
 void build_scatter_plan(ScatterPlan& sp, const std::vector<uint32_t>& lo, const std::vector<uint32_t>& hi,
                         const std::vector<int64_t>& sys_rows, const std::vector<uint32_t>& sys_col,
                         int64_t cols_per_sys, int ibytes, const Context& ctx, const TuneParams& tune);
+// The 8-byte window lists of the systems in `on` alone (host vectors only:
+// work, reduce items ordered by system with each system's range, part_elems,
+// cbits), their row chunks set for the windows those systems run side by
+// side.  The groups are the plan's (the same packing), so the plan's device
+// groups and slot lists serve the work items.
+void plan_scatter_subset(const ScatterPlan& sp, const std::vector<int64_t>& sys_rows,
+                         const std::vector<uint32_t>& sys_col, int64_t cols_per_sys, int ibytes, const Context& ctx,
+                         const TuneParams& tune, const std::vector<uint8_t>& on, ScatterLists& L);
 void set_error(const std::string& s);
 }  // namespace cad

@@ -141,13 +141,14 @@ void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint
     // the fixed-point windows read max |y| per system outside an LSQR
     // iteration, whose terms are bounded by construction
     const FixedScale fx{fixed_point_scatter<VT, View>() && mode != kScatterLsqr ? sp.d_ymax.as<unsigned>() : nullptr,
-                        L.cbits};
+                        al ? al->cbits : L.cbits};
+    VT* part = al ? al->part.as<VT>() : partials<VT>(L);
     const size_t lds = size_t(L.widest) * sizeof(double);
     with_scatter_kernel<VT, YT, View>(rpt, bsum.partials != nullptr, [&](auto kern) {
         allow_lds(kern, lds);
         hipLaunchKernelGGL(kern, dim3(unsigned(nwork)), dim3(kScatterBlock), lds, s, view, work,
                            L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(), sys_col, y,
-                           const_cast<LsqrScalars*>(div), mode, fx, int(y_padded), partials<VT>(L), bsum);
+                           const_cast<LsqrScalars*>(div), mode, fx, int(y_padded), part, bsum);
     });
     CAD_HIP(hipGetLastError());
 }
@@ -531,7 +532,8 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s, const Ac
     const int nred = al ? al->nreduce : int(L.reduce.size());
     if (nred > 0)
         hipLaunchKernelGGL(k_lsqr_reduce_v<VT>, dim3(unsigned(nred)), dim3(kRowBlock), 0, s, red,
-                           (const VT*)partials<VT>(L), ws.D.as<double>(), ws.v.as<double>(), rsys, sc);
+                           al ? al->part.as<const VT>() : (const VT*)partials<VT>(L), ws.D.as<double>(),
+                           ws.v.as<double>(), rsys, sc);
     const ColWork* cw = al ? al->cols.as<ColWork>() : g.d_cols.as<ColWork>();
     const int ncw = al ? al->ncols : int(g.cols.size());
     if (ncw > 0)
@@ -569,11 +571,8 @@ static void build_active(cad_matrix* m, ActiveLists& al, const std::vector<int>&
     const std::vector<RowWork>& tiles = al.row_kernel ? fine : gp.tiles;
     const std::vector<int32_t>& tf = al.row_kernel ? fine_first : gp.first;
     const std::vector<int32_t>& tc = al.row_kernel ? fine_count : gp.count;
-    ScatterLists& L = lists<VT>(m->splan);
     std::vector<RowWork> t;
-    std::vector<int32_t> t_first(n, 0), t_count(n, 0), r_first(n, 0), r_count(n, 0);
-    std::vector<ScatterWork> sw;
-    std::vector<ReduceWork> rw;
+    std::vector<int32_t> t_first(n, 0), t_count(n, 0);
     std::vector<ColWork> cw;
     al.on.assign(n, 0);
     al.rows = 0;
@@ -584,26 +583,29 @@ static void build_active(cad_matrix* m, ActiveLists& al, const std::vector<int>&
         t_first[i] = int32_t(t.size());
         t_count[i] = tc[i];
         t.insert(t.end(), tiles.begin() + tf[i], tiles.begin() + tf[i] + tc[i]);
-        r_first[i] = int32_t(rw.size());
-        r_count[i] = L.red_count[i];
-        rw.insert(rw.end(), L.reduce.begin() + L.red_first[i], L.reduce.begin() + L.red_first[i] + L.red_count[i]);
         cw.insert(cw.end(), m->grid.cols.begin() + m->grid.col_first[i],
                   m->grid.cols.begin() + m->grid.col_first[i] + m->grid.col_count[i]);
     }
-    for (const ScatterWork& w : L.work)
-        if (on[w.sys]) sw.push_back(w);
+    // the window tiles planned for these systems alone: how many row chunks
+    // a window needs to fill the CUs depends on how many windows run side by
+    // side (few in the tail of a batch, where the systems left are small)
+    ScatterLists sub;
+    plan_scatter_subset(m->splan, m->sys_row_begin, m->sys_col, m->cols_per_sys, m->mf ? 2 : m->ibytes, *m->ctx,
+                        m->tune, al.on, sub);
     al.ntiles = int(t.size());
-    al.nscatter = int(sw.size());
-    al.nreduce = int(rw.size());
+    al.nscatter = int(sub.work.size());
+    al.nreduce = int(sub.reduce.size());
     al.ncols = int(cw.size());
     al.tiles = to_device(t, s);
     if (al.row_kernel) al.gpart = DBuf(2 * sizeof(double) * std::max<size_t>(1, t.size()));
     al.tfirst = to_device(t_first, s);
     al.tcount = to_device(t_count, s);
-    al.scatter = to_device(sw, s);
-    al.reduce = to_device(rw, s);
-    al.rfirst = to_device(r_first, s);
-    al.rcount = to_device(r_count, s);
+    al.scatter = to_device(sub.work, s);
+    al.part = DBuf(sizeof(double) * size_t(std::max<int64_t>(1, sub.part_elems)));
+    al.cbits = sub.cbits;
+    al.reduce = to_device(sub.reduce, s);
+    al.rfirst = to_device(sub.red_first, s);
+    al.rcount = to_device(sub.red_count, s);
     al.cols = to_device(cw, s);
     CAD_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
     al.gen++;

@@ -19,7 +19,9 @@ struct ActiveLists {
     bool row_kernel = false;      // the gather blocks are row blocks of the generic row kernel
     DBuf tiles, tfirst, tcount;   // gather blocks and each system's range of them
     DBuf gpart;                   // the row kernel's block partials
-    DBuf scatter;                 // A^T y window tiles
+    DBuf scatter;                 // A^T y window tiles, planned for these systems alone
+    DBuf part;                    // their partial slices
+    int cbits = 0;                // their fixed-point bound (ScatterLists::cbits)
     DBuf reduce, rfirst, rcount;  // column reduction items and each system's range
     DBuf cols;                    // column blocks
     int gen = 0;                  // bumped on every rebuild (captured graphs are keyed on it)
