@@ -1338,27 +1338,57 @@ __device__ __forceinline__ VT scatter_scale(LsqrScalars* div, int mode, int sys,
 // rounding of the sums).  ymax holds, per system, the bits of max |y| (a
 // non-negative float, NaN and inf compare above every finite value); null in
 // an LSQR iteration, where |u_r / beta| < 2 by construction.
+// The kYmax scatter instantiation forms each system's max |y| itself from
+// the per-row-block maxima of k_absmax_blocks (ypart, the system's blocks at
+// yfirst[sys], ycount[sys]) instead of reading ymax.
 struct FixedScale {
     const unsigned* ymax;
     int cbits;
+    const unsigned* ypart;
+    const int32_t* yfirst;
+    const int32_t* ycount;
 };
 
-// 2^s for system sys, or 0 where the FP64 window must be used (non-finite
-// values, or fewer than 32 bits left for the largest term).
-__device__ __forceinline__ double fixed_scale(const FixedScale& fx, int sys, double sc)
+// 2^s for a system whose max |y| has float bits b (bounded: b is given; else
+// |y * sc| < 2), or 0 where the FP64 window must be used (non-finite values,
+// or fewer than 32 bits left for the largest term).
+__device__ __forceinline__ double fixed_scale_bits(int cbits, bool bounded, unsigned b, double sc)
 {
     int ey;
-    if (fx.ymax) {
-        const unsigned b = fx.ymax[sys];
+    if (bounded) {
         if (b >= 0x7f800000u) return 0.0;
         ey = b == 0 ? -126 : ilogbf(__uint_as_float(b)) + 1;
     } else {
         if (!isfinite(sc)) return 0.0;
         ey = 1;
     }
-    const int sh = 62 - fx.cbits - ey;
+    const int sh = 62 - cbits - ey;
     if (sh + ey < 32) return 0.0;
     return ldexp(1.0, sh);
+}
+
+// The max of a system's row-block maxima (float bits), in every thread.
+template <int BS>
+__device__ __forceinline__ unsigned system_max(const FixedScale& fx, int sys, unsigned* sh)
+{
+    unsigned mb = 0;
+    const int first = fx.yfirst[sys], n = fx.ycount[sys];
+    for (int i = threadIdx.x; i < n; i += BS) mb = max(mb, fx.ypart[first + i]);
+    for (int o = 32; o > 0; o >>= 1) mb = max(mb, unsigned(__shfl_xor(int(mb), o, 64)));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < BS / 64; i++) mb = max(mb, sh[i]);
+        sh[0] = mb;
+    }
+    __syncthreads();
+    return sh[0];
+}
+
+// The same for system sys, with its bits read from fx.ymax where given.
+__device__ __forceinline__ double fixed_scale(const FixedScale& fx, int sys, double sc)
+{
+    return fixed_scale_bits(fx.cbits, fx.ymax != nullptr, fx.ymax ? fx.ymax[sys] : 0u, sc);
 }
 
 template <class WT>
@@ -1536,7 +1566,7 @@ __device__ __forceinline__ void scatter_rows(const SlotMajor<IdxT>& m, const Sca
 // For groups of up to kMaxSegSlots slots the slot numbers and window offsets
 // are hoisted out of the row loop and the loads are software-pipelined
 // (scatter_rows).
-template <class VT, class YT, class IdxT, int RPT, bool kBeta = false>
+template <class VT, class YT, class IdxT, int RPT, bool kBeta = false, bool kYmax = false>
 __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> m,
                                                                 const ScatterWork* __restrict__ work,
                                                                 const SlotGroup* __restrict__ groups,
@@ -1561,7 +1591,14 @@ __global__ __launch_bounds__(kScatterBlock) void k_scatter_tile(SlotMajor<IdxT> 
     const VT sc = scatter_scale<VT, kScatterBlock, kBeta>(div, mode, w.sys, bsum);
     const int64_t vld = m.ld / RPT;
     // FP32 windows accumulate in fixed point where the scale allows it
-    const double fscale = std::is_same_v<VT, float> && ns <= kMaxSegSlots ? fixed_scale(fx, w.sys, double(sc)) : 0.0;
+    double fscale = 0.0;
+    if constexpr (kYmax) {
+        __shared__ unsigned shm[kScatterBlock / 64];
+        const unsigned b = system_max<kScatterBlock>(fx, w.sys, shm);
+        if (std::is_same_v<VT, float> && ns <= kMaxSegSlots) fscale = fixed_scale_bits(fx.cbits, true, b, double(sc));
+    } else {
+        fscale = std::is_same_v<VT, float> && ns <= kMaxSegSlots ? fixed_scale(fx, w.sys, double(sc)) : 0.0;
+    }
     if (ns <= kMaxSegSlots) {
         int64_t slot[kMaxSegSlots];
         uint32_t off[kMaxSegSlots];

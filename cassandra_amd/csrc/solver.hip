@@ -86,25 +86,28 @@ PT* partials(ScatterLists& L)
 // The first-stage scatter kernel of a view: vector loads of rpt (4, or 8 for
 // u16 indices) consecutive rows for stored indices, 4 positions per step for
 // a matrix-free matrix, one row per step otherwise; f receives the kernel.
-// beta: the instantiation that forms an LSQR iteration's beta itself.
+// beta: the instantiation that forms an LSQR iteration's beta itself; ymax:
+// the stored-index instantiation that forms each system's max |y| from the
+// row-block maxima (FixedScale::ypart), used by FP32 PCG.
 template <class VT, class YT, class View, class F>
-void with_scatter_kernel(int rpt, bool beta, F&& f)
+void with_scatter_kernel(int rpt, bool beta, bool ymax, F&& f)
 {
-    auto pick = [&](auto kb) {
-        constexpr bool B = decltype(kb)::value;
+    auto pick = [&](auto kb, auto ky) {
+        constexpr bool B = decltype(kb)::value, Y = decltype(ky)::value;
         if constexpr (std::is_same_v<View, SlotMajor<uint16_t>>) {
-            if (rpt == 8) return f(k_scatter_tile<VT, YT, uint16_t, 8, B>);
-            return f(k_scatter_tile<VT, YT, uint16_t, 4, B>);
+            if (rpt == 8) return f(k_scatter_tile<VT, YT, uint16_t, 8, B, Y>);
+            return f(k_scatter_tile<VT, YT, uint16_t, 4, B, Y>);
         } else if constexpr (std::is_same_v<View, SlotMajor<uint32_t>>) {
-            return f(k_scatter_tile<VT, YT, uint32_t, 4, B>);
+            return f(k_scatter_tile<VT, YT, uint32_t, 4, B, Y>);
         } else if constexpr (std::is_same_v<View, PositionsView>) {
             return f(k_scatter_tile_mf<VT, YT, 4, B>);
         } else {
             return f(k_scatter_part<VT, YT, View, B>);
         }
     };
-    if (beta) return pick(std::true_type{});
-    return pick(std::false_type{});
+    if (beta) return pick(std::true_type{}, std::false_type{});
+    if (ymax) return pick(std::false_type{}, std::true_type{});
+    return pick(std::false_type{}, std::false_type{});
 }
 
 // A^T y in stages: the max |y| pre-pass of the fixed-point FP32 windows
@@ -118,21 +121,29 @@ constexpr bool fixed_point_scatter()
 }
 
 template <class YT>
-void launch_absmax(ScatterPlan& sp, const YT* y, const GridPlan& g, int nsys, hipStream_t s)
+void launch_absmax(ScatterPlan& sp, const YT* y, const GridPlan& g, int nsys, hipStream_t s, bool systems = true)
 {
     if (g.rows.empty()) return;
     hipLaunchKernelGGL(k_absmax_blocks<YT>, dim3(unsigned(g.rows.size())), dim3(kRowBlock), 0, s,
                        g.d_rows.as<RowWork>(), y, sp.d_ymax_part.as<unsigned>());
+    if (!systems) {
+        CAD_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_absmax_systems, dim3(unsigned(nsys)), dim3(kRowBlock), 0, s, g.d_row_first.as<int32_t>(),
                        g.d_row_count.as<int32_t>(), (const unsigned*)sp.d_ymax_part.as<unsigned>(),
                        sp.d_ymax.as<unsigned>());
     CAD_HIP(hipGetLastError());
 }
 
+// yrows: the row blocks whose maxima k_absmax_blocks left in sp.d_ymax_part,
+// for the instantiation that forms max |y| itself (FP32 windows of a stored
+// matrix only; null: read sp.d_ymax).
 template <class VT, class YT, class View>
 void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint32_t* sys_col, const YT* y,
                           const LsqrScalars* div, int mode, hipStream_t s, bool y_padded = false,
-                          const ActiveLists* al = nullptr, const GridSys& bsum = GridSys{})
+                          const ActiveLists* al = nullptr, const GridSys& bsum = GridSys{},
+                          const GridPlan* yrows = nullptr)
 {
     ScatterLists& L = lists<VT>(sp);
     const ScatterWork* work = al ? al->scatter.as<ScatterWork>() : L.d_work.as<ScatterWork>();
@@ -140,11 +151,14 @@ void launch_scatter_tiles(ScatterPlan& sp, const View& view, int rpt, const uint
     if (nwork == 0) return;
     // the fixed-point windows read max |y| per system outside an LSQR
     // iteration, whose terms are bounded by construction
+    const bool ymax = yrows && fixed_point_scatter<VT, View>() && mode != kScatterLsqr;
     const FixedScale fx{fixed_point_scatter<VT, View>() && mode != kScatterLsqr ? sp.d_ymax.as<unsigned>() : nullptr,
-                        al ? al->cbits : L.cbits};
+                        al ? al->cbits : L.cbits, ymax ? sp.d_ymax_part.as<unsigned>() : nullptr,
+                        ymax ? yrows->d_row_first.as<int32_t>() : nullptr,
+                        ymax ? yrows->d_row_count.as<int32_t>() : nullptr};
     VT* part = al ? al->part.as<VT>() : partials<VT>(L);
     const size_t lds = size_t(L.widest) * sizeof(double);
-    with_scatter_kernel<VT, YT, View>(rpt, bsum.partials != nullptr, [&](auto kern) {
+    with_scatter_kernel<VT, YT, View>(rpt, bsum.partials != nullptr, ymax, [&](auto kern) {
         allow_lds(kern, lds);
         hipLaunchKernelGGL(kern, dim3(unsigned(nwork)), dim3(kScatterBlock), lds, s, view, work,
                            L.d_groups.as<SlotGroup>(), L.d_slot_list.as<uint16_t>(), sys_col, y,
@@ -181,8 +195,9 @@ void prepare_scatter(cad_matrix* m)
     partials<VT>(L);
     const size_t lds = size_t(L.widest) * sizeof(double);
     with_view(m, [&](auto view) {
-        for (bool beta : {false, true})
-            with_scatter_kernel<VT, YT, decltype(view)>(m->tune.srpt, beta, [&](auto kern) { allow_lds(kern, lds); });
+        for (int v = 0; v < 3; v++)
+            with_scatter_kernel<VT, YT, decltype(view)>(m->tune.srpt, v == 1, v == 2,
+                                                        [&](auto kern) { allow_lds(kern, lds); });
     });
 }
 
@@ -511,13 +526,14 @@ static void lsqr_iteration(cad_matrix* m, Workspace& ws, hipStream_t s, const Ac
     const GridPlan& g = m->grid;
     LsqrScalars* sc = ws.scal.as<LsqrScalars>();
     // u_raw = A v - alpha u, leaving the block partials of |u_raw|^2, from
-    // which the tiles of A^T (u / beta) form beta -- except on a matrix-free
-    // matrix, whose scatter variant with that prologue runs about 10% slower (its
-    // index programs need the registers): there the gather's finish launch
-    // forms beta.  Then the column reduction fused with vbar = D z - beta v
-    // (u is padded to whole 16-byte groups), and the rotation with the
-    // column update.
-    const bool fuse = !m->mf;
+    // which the tiles of A^T (u / beta) form beta -- except where the scatter
+    // instantiation with that prologue runs short of registers: on a
+    // matrix-free matrix (its index programs need them; about 10% slower) and
+    // with 8 rows a step (128 VGPRs and a spill): there the gather's finish
+    // launch forms beta.  Then the column reduction fused with
+    // vbar = D z - beta v (u is padded to whole 16-byte groups), and the
+    // rotation with the column update.
+    const bool fuse = !m->mf && m->tune.srpt != 8;
     gather<VT>(m, ws.vd.as<VT>(), EpiLsqrU<VT>{ws.u.as<VT>(), sc, VT(0)}, s, al, !fuse);
     ScatterLists& L = lists<VT>(m->splan);
     const GridSys usys = fuse ? row_list<VT>(m, al).sys : GridSys{};
@@ -1063,14 +1079,15 @@ static void pcg_iteration(cad_matrix* m, Workspace& ws, hipStream_t s)
     const GridPlan& g = m->grid;
     PcgScalars* sc = ws.pscal.as<PcgScalars>();
     spmv<VT>(m, ws.p.as<VT>(), ws.t.as<VT>(), s);
-    // A^T t: the window tiles (after the max |t| pre-pass of the fixed-point
-    // windows), then the column sums with the partials of p.q
+    // A^T t: the window tiles (after the row blocks' max |t| of the
+    // fixed-point windows; each tile takes its system's max of those), then
+    // the column sums with the partials of p.q
     ScatterLists& L = lists<VT>(m->splan);
     with_view(m, [&](auto view) {
-        if (fixed_point_scatter<VT, decltype(view)>() && !L.work.empty())
-            launch_absmax(m->splan, ws.t.as<VT>(), g, m->nsys, s);
+        const bool fixed = fixed_point_scatter<VT, decltype(view)>() && !L.work.empty();
+        if (fixed) launch_absmax(m->splan, ws.t.as<VT>(), g, m->nsys, s, false);
         launch_scatter_tiles<VT, VT>(m->splan, view, m->tune.srpt, m->d_sys_col.as<uint32_t>(), ws.t.as<VT>(),
-                                     nullptr, kScatterAll, s);
+                                     nullptr, kScatterAll, s, false, nullptr, GridSys{}, fixed ? &g : nullptr);
     });
     if (!L.reduce.empty())
         hipLaunchKernelGGL(k_pcg_reduce_q<VT>, dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
