@@ -79,6 +79,14 @@ def test_schedule_solver_and_errors(ctx, oracle, fit8):
     xm, rep = A.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
     assert rep["converged"] == 1
     assert oracle.rel_normal_residual(idx, A.cols, b.cpu().numpy(), xm.cpu().numpy())[0] <= 1.5e-10
+    # both ways an LSQR iteration forms beta: in the window tiles (4 rows a
+    # step), or in the gather's finish launch (8 rows a step, whose window
+    # kernel has no registers left for that prologue)
+    for srpt, order in ((8, 1), (4, 0)):
+        A.set_schedule(scatter_rpt=srpt, scatter_order=order)
+        xm, rep = A.solve(b, precision=capi.PREC_MIXED, tol=1e-10)
+        assert rep["converged"] == 1, srpt
+        assert oracle.rel_normal_residual(idx, A.cols, b.cpu().numpy(), xm.cpu().numpy())[0] <= 1.5e-10, srpt
     before = A.schedule()
     for bad in (dict(gather_f32_rpt=3), dict(gather_f32_block=256), dict(gather_f64_rpt=4),
                 dict(gather_f32_rpt=8, gather_f32_sub=2), dict(gather_f32_rpt=2, gather_f32_sub=4),
