@@ -839,6 +839,10 @@ struct RefineState {
     int it = 0, refinements = 0;
 };
 
+// The smallest inner LSQR reduction asked of FP32 products (they stall near
+// 1e-7 relative; an inner_tol below it is raised to it).
+constexpr double kFp32InnerFloor = 1e-7;
+
 // One precision stage of the refined LSQR, from the FP64 x in ws.x64.  Every
 // system runs its own refinement loop -- r = b - A x and A^T r in FP64, an
 // inner LSQR with VT products on min |A d - r| that stops on the device once
@@ -1042,7 +1046,9 @@ static void lsqr_refined(cad_matrix* m, const double* b, double* x, const cad_so
     fill<double>(ws.x64.as<double>(), m->cols, 0.0, s);
     double* xs = ws.x64.as<double>();
     if (mixed) {
-        refine_stage<float>(m, ws, b, o, o.inner_tol, o.max_refinements, st, det, rep);
+        // an inner target FP32 products cannot reach would leave every inner
+        // solve iterating on rounding noise until the iteration budget ends
+        refine_stage<float>(m, ws, b, o, std::max(o.inner_tol, kFp32InnerFloor), o.max_refinements, st, det, rep);
         bool finish = false;  // systems the FP32 stage left unconverged
         for (int i = 0; i < m->nsys; i++) finish |= st.act[i] != 0;
         if (finish && st.it < o.max_iterations) {

@@ -197,7 +197,7 @@ typedef struct cad_solve_options {
     int precondition;     /* LSQR: column scaling by 1/|a_c|; PCG: Jacobi (always) */
     int max_iterations;   /* total (inner) iterations */
     double tol;           /* stop when |A^T(b-Ax)| / |A^T b| <= tol (0: run max_iterations) */
-    double inner_tol;     /* MIXED: inner LSQR reduction per refinement step */
+    double inner_tol;     /* MIXED: inner LSQR reduction per refinement step (raised to 1e-7, the FP32 floor) */
     int max_refinements;  /* MIXED */
     int check_every;      /* iterations between host convergence checks (graph size) */
     int use_graph;        /* capture check_every iterations in a hipGraph */

@@ -103,6 +103,17 @@ def test_mixed_needs_refinement_for_1e10(ctx, oracle, prob):
     assert repm["converged"] == 1 and repm["refinements"] >= 1
 
 
+def test_mixed_inner_target_below_fp32_floor(ctx, oracle, prob):
+    # an inner reduction FP32 products cannot reach is raised to their floor
+    # rather than leaving each inner solve to iterate on rounding noise
+    idx, cols, A, b = prob
+    say("mixed, inner_tol 1e-9")
+    x, rep = A.solve(b, precision=capi.PREC_MIXED, tol=1e-10, inner_tol=1e-9, max_iterations=20000)
+    assert rep["converged"] == 1, rep
+    rel, _, _ = oracle.rel_normal_residual(idx, cols, b.cpu().numpy(), x.cpu().numpy())
+    assert rel <= 1e-10, (rel, rep)
+
+
 # FP32 vectors and products move the 10-step PCG iterate by ~2e-4 (the
 # reference's own float PCG is 1.5e-3 from the FP64 iterate)
 @pytest.mark.parametrize("prec,tol", [(capi.PREC_FP64, 1e-9), (capi.PREC_FP32, 1e-3)])
