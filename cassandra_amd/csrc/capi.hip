@@ -238,21 +238,17 @@ const ScatterPlan& eval_plan(cad_patterns* p, int64_t B)
     return ref;
 }
 
+// The backward's first stage (B > 0): the window tiles of the evaluator's
+// scatter plan into its partial slices; returns the plan's lists and slices.
 template <class VT, class View>
-void eval_backward_view(cad_patterns* p, const View& view, int64_t B, const VT* g, VT* grad)
+ScatterLists& eval_backward_windows(cad_patterns* p, const View& view, int64_t B, const VT* g, VT*& part)
 {
     hipStream_t s = p->ctx->stream;
-    const int64_t C = int64_t(p->group->cols());
-    if (B <= 0) {
-        hipLaunchKernelGGL(k_fill<VT>, dim3(grid1(C, 256)), dim3(256), 0, s, grad, C, VT(0));
-        CAD_HIP(hipGetLastError());
-        return;
-    }
     ScatterPlan& sp = const_cast<ScatterPlan&>(eval_plan(p, B));
     ScatterLists& L = sp.f8;  // 8-byte LDS windows (k_scatter_part accumulates in FP64)
     const size_t need = sizeof(double) * size_t(std::max<int64_t>(1, L.part_elems));  // either precision
     if (L.part.bytes < need) L.part = DBuf(need);
-    VT* part = L.part.as<VT>();
+    part = L.part.as<VT>();
     if (!L.work.empty()) {
         const size_t lds = size_t(L.widest) * sizeof(double);
         auto kern = k_scatter_part<VT, VT, View>;
@@ -265,6 +261,21 @@ void eval_backward_view(cad_patterns* p, const View& view, int64_t B, const VT* 
                            (LsqrScalars*)nullptr, kScatterAll, FixedScale{nullptr, 0}, 0, part, GridSys{});
         CAD_HIP(hipGetLastError());
     }
+    return L;
+}
+
+template <class VT, class View>
+void eval_backward_view(cad_patterns* p, const View& view, int64_t B, const VT* g, VT* grad)
+{
+    hipStream_t s = p->ctx->stream;
+    const int64_t C = int64_t(p->group->cols());
+    if (B <= 0) {
+        hipLaunchKernelGGL(k_fill<VT>, dim3(grid1(C, 256)), dim3(256), 0, s, grad, C, VT(0));
+        CAD_HIP(hipGetLastError());
+        return;
+    }
+    VT* part = nullptr;
+    ScatterLists& L = eval_backward_windows<VT>(p, view, B, g, part);
     hipLaunchKernelGGL((k_scatter_reduce<VT, VT>), dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
                        L.d_reduce.as<ReduceWork>(), (const VT*)part, grad, (const LsqrScalars*)nullptr, kScatterAll);
     CAD_HIP(hipGetLastError());
@@ -279,7 +290,7 @@ void eval_backward(cad_patterns* p, const int32_t* idx, int64_t B, const VT* g, 
 // One Adam step of the Huber-loss fit on the minibatch rows sel[0..B): the
 // forward with the loss and its gradient (which also copies the minibatch's
 // rows contiguously), the backward over that copy (the evaluator's scatter
-// plan), the Adam update.
+// plan) whose column reduction applies the Adam update.
 void train_step(cad_patterns* p, const int32_t* idx, const float* target, const int64_t* sel, int64_t B,
                 float delta, float lr, float b1, float b2, float eps, int64_t step, float* w, float* m, float* v,
                 double* loss)
@@ -299,16 +310,23 @@ void train_step(cad_patterns* p, const int32_t* idx, const float* target, const 
     float* grad = p->d_train_grad.as<float>();
     double* part = p->d_train_part.as<double>();
     int32_t* rows = p->d_train_rows.as<int32_t>();
-    if (B > 0)
+    const AdamStep a{lr, b1, b2, eps, 1.0f - std::pow(b1, float(step)), std::sqrt(1.0f - std::pow(b2, float(step)))};
+    if (B > 0) {
         hipLaunchKernelGGL(k_train_forward, dim3(nblk), dim3(kRowBlock), 0, s, idx, sel, B, k, w, uint32_t(C), target,
                            delta, g, part, rows);
-    // the backward over the contiguous copy of the minibatch's rows
-    eval_backward_view<float>(p, RowMajor32{rows, k}, B, g, grad);
-    const float bias1 = 1.0f - std::pow(b1, float(step));
-    const float bias2_sqrt = std::sqrt(1.0f - std::pow(b2, float(step)));
-    hipLaunchKernelGGL(k_adam, dim3(grid1(C, kRowBlock)), dim3(kRowBlock), 0, s, w, (const float*)grad, m, v, C, lr,
-                       b1, b2, eps, bias1, bias2_sqrt, (const double*)part, B > 0 ? nblk : 0, std::max<int64_t>(1, B),
-                       loss);
+        // the backward's window tiles over the contiguous copy of the
+        // minibatch's rows, then their column sums with the Adam update
+        float* gpart = nullptr;
+        ScatterLists& L = eval_backward_windows<float>(p, RowMajor32{rows, k}, B, g, gpart);
+        hipLaunchKernelGGL(k_adam_reduce, dim3(unsigned(L.reduce.size())), dim3(kRowBlock), 0, s,
+                           L.d_reduce.as<ReduceWork>(), (const float*)gpart, w, m, v, a, (const double*)part, nblk, B,
+                           loss);
+    } else {
+        // no rows: a zero gradient (Adam still moves the weights by its moments)
+        eval_backward_view<float>(p, RowMajor32{rows, k}, B, g, grad);
+        hipLaunchKernelGGL(k_adam, dim3(grid1(C, kRowBlock)), dim3(kRowBlock), 0, s, w, (const float*)grad, m, v, C, a,
+                           (const double*)part, 0, int64_t(1), loss);
+    }
     CAD_HIP(hipGetLastError());
 }
 

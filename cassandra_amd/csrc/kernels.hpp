@@ -2435,27 +2435,54 @@ __global__ __launch_bounds__(kRowBlock) void k_train_forward(const int32_t* __re
 // m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, w -= lr / (1 - b1^t) * m /
 // (sqrt(v) / sqrt(1 - b2^t) + eps)); block 0 also adds the step's mean loss
 // (the forward's block sums, reduced in a fixed order, / B) to *loss.
+struct AdamStep {
+    float lr, b1, b2, eps, bias1, bias2_sqrt;
+};
+__device__ __forceinline__ void adam_update(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                                            int64_t c, float gc, const AdamStep& a)
+{
+    const float mc = a.b1 * m[c] + (1.0f - a.b1) * gc;
+    const float vc = a.b2 * v[c] + (1.0f - a.b2) * gc * gc;
+    m[c] = mc;
+    v[c] = vc;
+    w[c] -= (a.lr / a.bias1) * mc / (sqrtf(vc) / a.bias2_sqrt + a.eps);
+}
+
+// The loss of the step: the forward's block partials, in a fixed order, / B.
+__device__ __forceinline__ void add_loss(const double* __restrict__ part, int nparts, int64_t B,
+                                         double* __restrict__ loss)
+{
+    __shared__ double sh[kRowBlock / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kRowBlock) s += part[i];
+    s = block_sum<kRowBlock>(s, sh);
+    if (threadIdx.x == 0) *loss += s / double(B);
+}
+
 __global__ __launch_bounds__(kRowBlock) void k_adam(float* __restrict__ w, const float* __restrict__ grad,
                                                     float* __restrict__ m, float* __restrict__ v, int64_t n,
-                                                    float lr, float b1, float b2, float eps, float bias1,
-                                                    float bias2_sqrt, const double* __restrict__ part, int nparts,
+                                                    AdamStep a, const double* __restrict__ part, int nparts,
                                                     int64_t B, double* __restrict__ loss)
 {
-    for (int64_t c = blockIdx.x * int64_t(kRowBlock) + threadIdx.x; c < n; c += int64_t(gridDim.x) * kRowBlock) {
-        const float gc = grad[c];
-        const float mc = b1 * m[c] + (1.0f - b1) * gc;
-        const float vc = b2 * v[c] + (1.0f - b2) * gc * gc;
-        m[c] = mc;
-        v[c] = vc;
-        w[c] -= (lr / bias1) * mc / (sqrtf(vc) / bias2_sqrt + eps);
-    }
-    if (blockIdx.x == 0) {
-        __shared__ double sh[kRowBlock / 64];
-        double s = 0.0;
-        for (int i = threadIdx.x; i < nparts; i += kRowBlock) s += part[i];
-        s = block_sum<kRowBlock>(s, sh);
-        if (threadIdx.x == 0) *loss += s / double(B);
-    }
+    for (int64_t c = blockIdx.x * int64_t(kRowBlock) + threadIdx.x; c < n; c += int64_t(gridDim.x) * kRowBlock)
+        adam_update(w, m, v, c, grad[c], a);
+    if (blockIdx.x == 0) add_loss(part, nparts, B, loss);
+}
+
+// The backward's column reduction with the Adam step fused: per column the
+// gradient is the window slices' sum rounded to float (as k_scatter_reduce
+// writes it), then the step of k_adam -- one launch fewer a step, the same
+// values.  One block per reduce item (the items cover every column once).
+__global__ __launch_bounds__(kRowBlock) void k_adam_reduce(const ReduceWork* __restrict__ work,
+                                                           const float* __restrict__ gpart, float* __restrict__ w,
+                                                           float* __restrict__ m, float* __restrict__ v, AdamStep a,
+                                                           const double* __restrict__ part, int nparts, int64_t B,
+                                                           double* __restrict__ loss)
+{
+    const ReduceWork rw = work[blockIdx.x];
+    for (int64_t c = rw.col_begin + threadIdx.x; c < rw.col_end; c += kRowBlock)
+        adam_update(w, m, v, c, float(reduce_column(gpart, rw, c)), a);
+    if (blockIdx.x == 0) add_loss(part, nparts, B, loss);
 }
 
 // ------------------------------------------------------------ data ---------
