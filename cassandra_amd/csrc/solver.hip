@@ -1541,10 +1541,14 @@ std::string autotune(cad_matrix* m)
                           "scatter f32 order %d rpt %d blocks/CU %.0f cap %.2f balance %.2f (%zu tiles): %.1f us\n",
                           c.order, c.srpt, c.bpc, c.cap, c.bal, m->splan.f8.work.size(), us);
             log += line;
-            // a later candidate must win by 2 %: the chunk-shared XCD-major
-            // order comes first and, at equal time, moves half the HBM bytes
-            // (its windows' re-reads hit L2)
-            if (us < 0.98 * best_us) best_us = us, best = c;
+            // a later candidate must win by 2 %, and a per-window (order 0)
+            // one over the chunk-shared XCD-major order, which comes first, by
+            // 5 %: that order moves half the HBM bytes (its windows' re-reads
+            // hit L2), and a few % of isolated time are not worth twice the
+            // traffic (the two also trade places from run to run on the 5M-row
+            // matrix, where they are within 3 % of each other)
+            const double margin = (c.order == 0 && best.order == 1) ? 0.95 : 0.98;
+            if (us < margin * best_us) best_us = us, best = c;
         }
         tp = keep;
         tp.scatter_order = best.order;
