@@ -385,7 +385,14 @@ def leg_matrix_free(ctx, traffic=False):
         valu[name] = {"bound": "valu", "ops_per_row": ops[name], "achieved": achieved / 1e12,
                       "peak": VALU_PEAK_OPS / 1e12, "unit": "T lane-ops/s", "frac": achieved / VALU_PEAK_OPS}
     log(f"matrix-free VALU roofline ({ops['squares_per_row']} squares a row): {valu}")
+    # what a row keeps resident in HBM: its indices (or its position and
+    # ternary words), plus the solver's row vectors (u in FP32, b and r in
+    # FP64) -- the capacity the matrix-free matrix trades its speed for
+    row_vectors = 4 + 8 + 8
+    resident = {"stored": S.index_bytes * S.k + row_vectors, "matrix_free": 16 + 16 + row_vectors}
     return {"rows": S.rows, "k": S.k, "cols": S.cols, "seconds_matrix_free_mixed": t_mf,
+            "resident_bytes_per_row": resident,
+            "rows_per_288GB": {key: int(288e9 // val) for key, val in resident.items()},
             "iterations_matrix_free": rep_f["iterations"], "rel_matrix_free": rep_f["rel_normal_residual"],
             "seconds_stored_fp64": t_s, "iterations_stored_fp64": rep_s["iterations"],
             "rel_stored_fp64": rep_s["rel_normal_residual"], "speedup": t_s / t_mf,
