@@ -78,12 +78,16 @@ constexpr uint32_t kReduceCols = kRowBlock;  // columns of one scatter-reduce wo
 // Row chunks of one window of a system: the share of the target tile count,
 // capped so that the partial slices stay near partial_cap of the window's
 // index traffic -- and, where that cap would leave CUs idle, raised towards
-// one tile per CU while its slices stay under kPartialFloorBytes a window
-// (their write and read then cost less than the parallelism gains).
+// one tile per CU while its slices stay under the floor bytes a window
+// (their write and read then cost less than the parallelism gains): 8 MB for
+// a small system (at most kRowKernelRows rows, whose tiles are latency-bound:
+// a 100k-row PCG iteration 47 -> 42 µs), 2 MB for a large one (8 MB took the
+// 5M-row matrix's XCD-major plan from 240 to 360 tiles, 165 -> 245 µs).
 // `windows` counts the windows of every system planned together: those of
 // many systems already fill the CUs side by side.  At least kScatterBlock
 // rows a chunk.
 constexpr double kPartialFloorBytes = 2.0 * 1024 * 1024;
+constexpr double kSmallPartialFloorBytes = 8.0 * 1024 * 1024;
 constexpr double kMultiSystemTiles = 8.0;  // target tiles a CU, over blocks_per_cu, for several systems
 
 int64_t window_chunks(int64_t R, double share_chunks, double cap_chunks, int cus, int windows, uint32_t W,
@@ -91,7 +95,8 @@ int64_t window_chunks(int64_t R, double share_chunks, double cap_chunks, int cus
 {
     int64_t C = std::max<int64_t>(1, std::min<int64_t>(std::llround(share_chunks), int64_t(cap_chunks)));
     const int64_t par = std::max<int64_t>(1, cus / std::max(1, windows));
-    const int64_t floor_c = std::max<int64_t>(1, int64_t(kPartialFloorBytes / (2.0 * double(W) * vbytes)));
+    const double floor_bytes = R <= kRowKernelRows ? kSmallPartialFloorBytes : kPartialFloorBytes;
+    const int64_t floor_c = std::max<int64_t>(1, int64_t(floor_bytes / (2.0 * double(W) * vbytes)));
     C = std::max(C, std::min(par, floor_c));
     return std::max<int64_t>(1, std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock));
 }
