@@ -228,9 +228,13 @@ const ScatterPlan& eval_plan(cad_patterns* p, int64_t B)
     std::unique_ptr<ScatterPlan> sp(new ScatterPlan);
     // A minibatch has few rows per window: the row chunks are sized to give
     // every CU a tile (partial slices up to 8x the index bytes; the default
-    // cap of 1x left a 65,536-row minibatch of 4 patterns 28 tiles).
+    // cap of 1x left a 65,536-row minibatch of 4 patterns 28 tiles), with the
+    // 2 MB floor of a large system (the 8 MB one of a small system gave the
+    // training step more slices to sum than tiles to spread: 3.7 -> 4.0 ms an
+    // epoch).
     TuneParams tp;
     tp.partial_cap = 8.0;
+    tp.small_floor_bytes = 2.0 * 1024 * 1024;
     build_scatter_plan(*sp, p->slot_lo, p->slot_hi, std::vector<int64_t>{0, B}, std::vector<uint32_t>{0u},
                        int64_t(p->group->cols()), 4, *p->ctx, tp);
     const ScatterPlan& ref = *sp;

@@ -79,23 +79,23 @@ constexpr uint32_t kReduceCols = kRowBlock;  // columns of one scatter-reduce wo
 // capped so that the partial slices stay near partial_cap of the window's
 // index traffic -- and, where that cap would leave CUs idle, raised towards
 // one tile per CU while its slices stay under the floor bytes a window
-// (their write and read then cost less than the parallelism gains): 8 MB for
-// a small system (at most kRowKernelRows rows, whose tiles are latency-bound:
-// a 100k-row PCG iteration 47 -> 42 µs), 2 MB for a large one (8 MB took the
-// 5M-row matrix's XCD-major plan from 240 to 360 tiles, 165 -> 245 µs).
+// (their write and read then cost less than the parallelism gains):
+// small_floor (8 MB by default) for a small system (at most kRowKernelRows
+// rows, whose tiles are latency-bound: a 100k-row PCG iteration 47 -> 42 µs),
+// 2 MB for a large one (8 MB took the 5M-row matrix's XCD-major plan from 240
+// to 360 tiles, 165 -> 245 µs).
 // `windows` counts the windows of every system planned together: those of
 // many systems already fill the CUs side by side.  At least kScatterBlock
 // rows a chunk.
 constexpr double kPartialFloorBytes = 2.0 * 1024 * 1024;
-constexpr double kSmallPartialFloorBytes = 8.0 * 1024 * 1024;
 constexpr double kMultiSystemTiles = 8.0;  // target tiles a CU, over blocks_per_cu, for several systems
 
 int64_t window_chunks(int64_t R, double share_chunks, double cap_chunks, int cus, int windows, uint32_t W,
-                      int vbytes)
+                      int vbytes, double small_floor)
 {
     int64_t C = std::max<int64_t>(1, std::min<int64_t>(std::llround(share_chunks), int64_t(cap_chunks)));
     const int64_t par = std::max<int64_t>(1, cus / std::max(1, windows));
-    const double floor_bytes = R <= kRowKernelRows ? kSmallPartialFloorBytes : kPartialFloorBytes;
+    const double floor_bytes = R <= kRowKernelRows ? small_floor : kPartialFloorBytes;
     const int64_t floor_c = std::max<int64_t>(1, int64_t(floor_bytes / (2.0 * double(W) * vbytes)));
     C = std::max(C, std::min(par, floor_c));
     return std::max<int64_t>(1, std::min<int64_t>(C, (R + kScatterBlock - 1) / kScatterBlock));
@@ -243,7 +243,7 @@ void make_scatter_lists_xcd(const ScatterPlan& sp, const std::vector<int64_t>& s
             const double cap = tune.partial_cap * double(R) * work_per_row * ibytes /
                                (2.0 * width * vbytes * double(tiles_per_chunk) / std::max(1, nw));
             C = window_chunks(R, double(chunks) * share, cap, cus, tiles_per_chunk * std::max(1, nrun),
-                              uint32_t(width / std::max(1, nw)), vbytes);
+                              uint32_t(width / std::max(1, nw)), vbytes, tune.small_floor_bytes);
             const int64_t chunk = (R + C - 1) / C;
             C = (R + chunk - 1) / chunk;
             for (int j = 0; j < nw; j++) {
@@ -357,7 +357,8 @@ void make_scatter_lists(const ScatterPlan& sp, const std::vector<int64_t>& sys_r
                 if (R > 0) {
                     const double share = total > 0 ? double(R) * win_cost(g, W) / total : 1.0;
                     const double cap = tune.partial_cap * double(R) * g.nslots * ibytes / (2.0 * W * vbytes);
-                    C = window_chunks(R, target * share, cap, cus, windows * std::max(1, nrun), W, vbytes);
+                    C = window_chunks(R, target * share, cap, cus, windows * std::max(1, nrun), W, vbytes,
+                                      tune.small_floor_bytes);
                     const int64_t chunk = (R + C - 1) / C;
                     C = (R + chunk - 1) / chunk;
                     for (int64_t c = 0; c < C; c++) {

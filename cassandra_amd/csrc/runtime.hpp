@@ -129,6 +129,7 @@ struct TuneParams {
     double partial_cap = 1.0;   // partial-slice traffic cap as a fraction of index traffic
     int scatter_order = 0;      // 0: chunks per window by work share; 1: shared chunks, XCD-major
     double scatter_balance = 1.0; // order 0: weight of slot loads against atomics in a window's work (1: loads)
+    double small_floor_bytes = 8.0 * 1024 * 1024;  // partial slices a window may take to spread a small system
 };
 
 // A matrix (or, in a batch, the systems still iterating) of at most this many
